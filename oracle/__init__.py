@@ -1,0 +1,374 @@
+"""ctypes front end of the CPU oracle (liblineal_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+bench.py's ``cpu_baseline`` leg as the checker.  The product package
+``lineal_amd`` never imports it.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblineal_oracle.so")
+
+i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
+f32p = C.POINTER(C.c_float)
+u8p = C.POINTER(C.c_uint8)
+u32p = C.POINTER(C.c_uint32)
+
+
+class StencilStruct(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64 * 3),
+        ("len", C.c_double * 3),
+        ("kind", C.c_int),
+        ("zero_to_one", C.c_int),
+        ("coeff", f64p),
+        ("mat", u8p),
+        ("mat_coeffs", C.c_double * 4),
+        ("p_start", C.c_double),
+        ("p_end", C.c_double),
+    ]
+
+
+class CsrStruct(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("rowptr", i64p),
+        ("cols", i64p),
+        ("vals", f64p),
+        ("is_f32", C.c_int),
+        ("symmetric", C.c_int),
+    ]
+
+
+class MatStruct(C.Structure):
+    _fields_ = [
+        ("is_csr", C.c_int),
+        ("st", C.POINTER(StencilStruct)),
+        ("csr", C.POINTER(CsrStruct)),
+    ]
+
+
+class AmgParams(C.Structure):
+    _fields_ = [
+        ("max_level_num", C.c_int),
+        ("min_coarsen_size", C.c_int64),
+        ("min_coarsen_factor", C.c_double),
+        ("strong_threshold", C.c_double),
+        ("aggressive_strong_threshold", C.c_double),
+        ("min_aggregate_size", C.c_int64),
+        ("max_aggregate_size", C.c_int64),
+        ("coarse_f32", C.c_int),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    cmd = ["make", "-s", "-C", _HERE]
+    if force:
+        cmd.insert(1, "-B")
+    subprocess.run(cmd, check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        sp = C.POINTER(StencilStruct)
+        mp = C.POINTER(MatStruct)
+        cp = C.POINTER(CsrStruct)
+        hp = C.c_void_p
+        sigs = {
+            "or_axis_quotients": (None, [sp, f64p]),
+            "or_harmonic_mean_f64": (C.c_double, [C.c_double, C.c_double]),
+            "or_harmonic_mean_f32": (C.c_float, [C.c_float, C.c_float]),
+            "or_stencil_row_f64": (C.c_int, [sp, C.c_int64, i64p, f64p]),
+            "or_stencil_row_f32": (C.c_int, [sp, C.c_int64, i64p, f32p]),
+            "or_stencil_rhs_f64": (None, [sp, f64p]),
+            "or_stencil_rhs_f32": (None, [sp, f32p]),
+            "or_spmv_f64": (None, [mp, f64p, f64p]),
+            "or_spmv_f32": (None, [mp, f32p, f32p]),
+            "or_residual_f64": (None, [mp, f64p, f64p, f64p]),
+            "or_residual_f32": (None, [mp, f32p, f32p, f32p]),
+            "or_jacobi_f64": (None, [mp, f64p, f64p, f64p, C.c_double]),
+            "or_jacobi_f32": (None, [mp, f32p, f32p, f32p, C.c_float]),
+            "or_sor_f64": (None, [mp, f64p, f64p, f64p, C.c_double, C.c_int, C.c_int, C.c_int, i64p]),
+            "or_sor_f32": (None, [mp, f32p, f32p, f32p, C.c_float, C.c_int, C.c_int, C.c_int, i64p]),
+            "or_dot_f64": (C.c_double, [f64p, f64p, C.c_int64]),
+            "or_csr_from_stencil": (C.c_int64, [sp, C.c_int, i64p, i64p, i64p, f64p, f64p, i64p]),
+            "or_amg_setup": (hp, [mp, C.POINTER(AmgParams)]),
+            "or_amg_free": (None, [hp]),
+            "or_amg_level_num": (C.c_int, [hp]),
+            "or_amg_level_size": (C.c_int64, [hp, C.c_int]),
+            "or_amg_agg_map": (u32p, [hp, C.c_int]),
+            "or_amg_c2f_ptr": (i64p, [hp, C.c_int]),
+            "or_amg_c2f_idx": (i64p, [hp, C.c_int]),
+            "or_amg_matrix": (cp, [hp, C.c_int]),
+            "or_amg_cholesky": (cp, [hp]),
+            "or_amg_stats": (None, [hp, C.c_int, i64p]),
+            "or_cholesky_solve": (None, [cp, f64p, f64p]),
+            "or_cholesky_decompose": (C.c_int64, [cp, i64p, i64p, f64p]),
+            "or_vcycle_f64": (None, [hp, f64p, f64p, C.c_int]),
+            "or_vcycle_f32": (None, [hp, f32p, f32p, C.c_int]),
+            "or_pcg": (C.c_int, [mp, hp, C.c_int, C.c_int, f64p, f64p, C.c_double, C.c_int, f64p]),
+            "or_pbicgstab": (C.c_int, [mp, hp, C.c_int, C.c_int, f64p, f64p, C.c_double, C.c_int, f64p]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray | None, t):
+    if a is None:
+        return C.cast(None, C.POINTER(t))
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _pf(a: np.ndarray | None):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return _p(a, C.c_double)
+    assert a.dtype == np.float32
+    return _p(a, C.c_float)
+
+
+class Stencil:
+    """Matrix-free diffusion stencil (values or lookup valuator)."""
+
+    def __init__(self, dims, coeff=None, materials=None, mat_coeffs=(0.0, 1.0, 2.0, 3.0),
+                 lengths=None, zero_to_one=True, p_start=1.0, p_end=0.0):
+        self.dims = tuple(int(d) for d in dims)
+        self.N = int(np.prod(self.dims))
+        s = StencilStruct()
+        for d in range(3):
+            s.n[d] = self.dims[d]
+            s.len[d] = float(lengths[d]) if lengths is not None else float(self.dims[d])
+        s.zero_to_one = int(bool(zero_to_one))
+        s.p_start, s.p_end = float(p_start), float(p_end)
+        self.coeff = self.materials = None
+        if coeff is not None:
+            self.coeff = np.ascontiguousarray(coeff, dtype=np.float64).reshape(-1)
+            assert self.coeff.size == self.N
+            s.kind = 0
+            s.coeff = _p(self.coeff, C.c_double)
+        else:
+            self.materials = np.ascontiguousarray(materials, dtype=np.uint8).reshape(-1)
+            assert self.materials.size == self.N
+            s.kind = 1
+            s.mat = _p(self.materials, C.c_uint8)
+            for k in range(4):
+                s.mat_coeffs[k] = float(mat_coeffs[k])
+        self.mat_coeffs = tuple(float(c) for c in mat_coeffs)
+        self.lengths = tuple(s.len[d] for d in range(3))
+        self.zero_to_one = bool(zero_to_one)
+        self.p_start, self.p_end = float(p_start), float(p_end)
+        self.s = s
+        self.m = MatStruct(0, C.pointer(self.s), None)
+
+    @property
+    def mat(self):
+        return self.m
+
+    def quotients(self):
+        q = np.zeros(3)
+        lib().or_axis_quotients(C.byref(self.s), _p(q, C.c_double))
+        return q
+
+    def row(self, i, dtype=np.float64):
+        cols = np.zeros(8, np.int64)
+        vals = np.zeros(8, dtype)
+        fn = lib().or_stencil_row_f64 if dtype == np.float64 else lib().or_stencil_row_f32
+        k = fn(C.byref(self.s), int(i), _p(cols, C.c_int64), _pf(vals))
+        return cols[:k].copy(), vals[:k].copy()
+
+    def rhs(self, dtype=np.float64):
+        b = np.zeros(self.N, dtype)
+        fn = lib().or_stencil_rhs_f64 if dtype == np.float64 else lib().or_stencil_rhs_f32
+        fn(C.byref(self.s), _pf(b))
+        return b
+
+    def to_csr(self, compress=False):
+        L = lib()
+        rows = C.c_int64()
+        nnz = L.or_csr_from_stencil(C.byref(self.s), int(compress), C.byref(rows), None, None,
+                                    None, None, None)
+        rp = np.zeros(rows.value + 1, np.int64)
+        cols = np.zeros(max(nnz, 1), np.int64)
+        vals = np.zeros(max(nnz, 1), np.float64)
+        rhs = np.zeros(rows.value, np.float64)
+        cmap = np.zeros(self.N, np.int64)
+        L.or_csr_from_stencil(C.byref(self.s), int(compress), C.byref(rows), _p(rp, C.c_int64),
+                              _p(cols, C.c_int64), _p(vals, C.c_double), _p(rhs, C.c_double),
+                              _p(cmap, C.c_int64) if compress else None)
+        m = Csr(rp, cols[:nnz], vals[:nnz], symmetric=True)
+        return m, rhs, (cmap if compress else None)
+
+
+class Csr:
+    def __init__(self, rowptr, cols, vals, symmetric=True, is_f32=False):
+        self.rowptr = np.ascontiguousarray(rowptr, np.int64)
+        self.cols = np.ascontiguousarray(cols, np.int64)
+        self.vals = np.ascontiguousarray(vals, np.float64)
+        self.N = self.rowptr.size - 1
+        self.c = CsrStruct(self.N, _p(self.rowptr, C.c_int64), _p(self.cols, C.c_int64),
+                           _p(self.vals, C.c_double), int(is_f32), int(symmetric))
+        self.m = MatStruct(1, None, C.pointer(self.c))
+
+    @property
+    def mat(self):
+        return self.m
+
+    def dense(self):
+        A = np.zeros((self.N, self.N))
+        for i in range(self.N):
+            for k in range(self.rowptr[i], self.rowptr[i + 1]):
+                A[i, self.cols[k]] += self.vals[k]
+        return A
+
+    @staticmethod
+    def from_struct(cs: CsrStruct):
+        n = cs.n
+        rp = np.ctypeslib.as_array(cs.rowptr, shape=(n + 1,)).copy()
+        nnz = int(rp[-1])
+        cols = np.ctypeslib.as_array(cs.cols, shape=(max(nnz, 1),))[:nnz].copy()
+        vals = np.ctypeslib.as_array(cs.vals, shape=(max(nnz, 1),))[:nnz].copy()
+        return Csr(rp, cols, vals, symmetric=bool(cs.symmetric), is_f32=bool(cs.is_f32))
+
+
+def _fn(name, dtype):
+    return getattr(lib(), name + ("_f64" if dtype == np.float64 else "_f32"))
+
+
+def spmv(A, x):
+    y = np.zeros_like(x)
+    _fn("or_spmv", x.dtype)(C.byref(A.mat), _pf(x), _pf(y))
+    return y
+
+
+def residual(A, b, x):
+    r = np.zeros_like(x)
+    _fn("or_residual", x.dtype)(C.byref(A.mat), _pf(b), _pf(x), _pf(r))
+    return r
+
+
+def jacobi(A, x_in, b, omega):
+    x_out = np.zeros_like(b)
+    _fn("or_jacobi", b.dtype)(C.byref(A.mat), _pf(x_in), _pf(x_out), _pf(b), omega)
+    return x_out
+
+
+def sor(A, x_in, b, omega=1.0, forward=True, ultra=False, segments=None):
+    n = b.size
+    seg = np.asarray(segments if segments is not None else [0, n], np.int64)
+    x_out = np.zeros_like(b)
+    _fn("or_sor", b.dtype)(C.byref(A.mat), _pf(x_in), _pf(x_out), _pf(b), omega,
+                           int(forward), int(ultra), seg.size - 1, _p(seg, C.c_int64))
+    return x_out
+
+
+def dot(a, b):
+    return lib().or_dot_f64(_pf(a), _pf(b), a.size)
+
+
+def default_params(coarse_f32=False):
+    """Parameters of test/aux/default.hpp:170-188."""
+    return AmgParams(16, 32, 1.2, 1.0 / 3.0, 1.0 / 3.0, 6, 9, int(coarse_f32))
+
+
+class Hierarchy:
+    def __init__(self, A, params: AmgParams | None = None, coarse_f32=False):
+        self.A = A  # keep the finest operator alive
+        self.params = params or default_params(coarse_f32)
+        self.h = lib().or_amg_setup(C.byref(A.mat), C.byref(self.params))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_amg_free(self.h)
+            self.h = None
+
+    @property
+    def level_num(self):
+        return lib().or_amg_level_num(self.h)
+
+    def size(self, level):
+        return lib().or_amg_level_size(self.h, level)
+
+    def agg_map(self, level):
+        n = self.size(level)
+        return np.ctypeslib.as_array(lib().or_amg_agg_map(self.h, level), shape=(n,)).copy()
+
+    def c2f(self, level):
+        nc = self.size(level + 1)
+        ptr = np.ctypeslib.as_array(lib().or_amg_c2f_ptr(self.h, level), shape=(nc + 1,)).copy()
+        m = int(ptr[-1])
+        idx = np.ctypeslib.as_array(lib().or_amg_c2f_idx(self.h, level), shape=(max(m, 1),))[:m].copy()
+        return ptr, idx
+
+    def matrix(self, level):
+        return Csr.from_struct(lib().or_amg_matrix(self.h, level).contents)
+
+    def cholesky(self):
+        return Csr.from_struct(lib().or_amg_cholesky(self.h).contents)
+
+    def stats(self, level):
+        out = np.zeros(5, np.int64)
+        lib().or_amg_stats(self.h, level, _p(out, C.c_int64))
+        return out
+
+    def vcycle(self, r, ultra=False):
+        z = np.zeros_like(r)
+        _fn("or_vcycle", r.dtype)(self.h, _pf(r), _pf(z), int(ultra))
+        return z
+
+
+def cholesky_solve(L: Csr, b):
+    x = np.zeros_like(b)
+    lib().or_cholesky_solve(C.byref(L.c), _pf(b), _pf(x))
+    return x
+
+
+def cholesky_decompose(A: Csr):
+    n = A.N
+    cap = n * (n + 1) // 2 + 1
+    rp = np.zeros(n + 1, np.int64)
+    cols = np.zeros(cap, np.int64)
+    vals = np.zeros(cap, np.float64)
+    nnz = lib().or_cholesky_decompose(C.byref(A.c), _p(rp, C.c_int64), _p(cols, C.c_int64),
+                                      _p(vals, C.c_double))
+    if nnz < 0:
+        raise ValueError("matrix is not SPD")
+    return Csr(rp, cols[:nnz], vals[:nnz], symmetric=False)
+
+
+def pcg(A, b, hierarchy=None, precon=0, ultra=False, rel_tol=1e-8, max_it=500, x0=None):
+    x = np.zeros_like(b) if x0 is None else x0.astype(np.float64).copy()
+    hist = np.zeros(max_it + 1)
+    it = lib().or_pcg(C.byref(A.mat), hierarchy.h if hierarchy else None, precon, int(ultra),
+                      _pf(b), _pf(x), rel_tol, max_it, _pf(hist))
+    return x, it, hist[: it + 1]
+
+
+def pbicgstab(A, b, hierarchy=None, precon=0, ultra=False, rel_tol=1e-8, max_it=500, x0=None):
+    x = np.zeros_like(b) if x0 is None else x0.astype(np.float64).copy()
+    hist = np.zeros(max_it + 1)
+    it = lib().or_pbicgstab(C.byref(A.mat), hierarchy.h if hierarchy else None, precon,
+                            int(ultra), _pf(b), _pf(x), rel_tol, max_it, _pf(hist))
+    return x, it, hist[: it + 1]
