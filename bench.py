@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Benchmark: AMG-preconditioned CG to a relative residual of 1e-8 on a 256^3
+lognormal (sigma = 1) variable-coefficient diffusion stencil, mixed precision
+(FP32 hierarchy / smoothers / preconditioner SpMV inside an FP64 CG) and plain
+FP64, on one MI355X; plus the HBM roofline of the hot kernels.
+
+A step is one complete solve from x = 0 (the hierarchy is set up once,
+outside the timed region; the setup time is reported separately).  value is
+the mixed-precision time-to-solution in seconds (lower is better).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_ms(ctx, fn, launches):
+    """Average duration of one launch, HIP events on the library's stream."""
+    fn()
+    ctx.synchronize()
+    ctx.timer_start()
+    for k in range(launches):
+        fn(k)
+    return ctx.timer_stop() / launches
+
+
+def cpu_baseline(n_sample, sigma, n_target, iterations_target):
+    """The oracle (single-threaded C port) on a bounded sample: a complete
+    FP64 PCG-AMG solve at n_sample^3, scaled to the target by cell count and
+    iteration count."""
+    import oracle as O
+    from lineal_amd import problems
+
+    dims = (n_sample,) * 3
+    c = problems.lognormal(dims, sigma, 1)
+    S = O.Stencil(dims, coeff=c)
+    b = S.rhs()
+    t0 = time.perf_counter()
+    H = O.Hierarchy(S)
+    t1 = time.perf_counter()
+    _, it, _ = O.pcg(S, b, H, precon=1, rel_tol=1e-8, max_it=500)
+    t2 = time.perf_counter()
+    per_it_cell = (t2 - t1) / it / S.N
+    value = per_it_cell * n_target ** 3 * iterations_target
+    return {
+        "value": value,
+        "unit": "s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle FP64 PCG-AMG solve at {n_sample}^3 lognormal sigma={sigma}: {it} iterations "
+                   f"in {t2 - t1:.2f} s (+{t1 - t0:.2f} s setup), scaled to {n_target}^3 x "
+                   f"{iterations_target} iterations by cells and iterations"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--sigma", type=float, default=1.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=96)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this path has no multi-GPU implementation "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import lineal_amd as la
+    from lineal_amd import problems
+
+    if not os.path.exists(la.LIB_PATH):
+        from lineal_amd import build as hip_build
+
+        hip_build.build()
+
+    n = args.n
+    dims = (n, n, n)
+    N = n ** 3
+    ctx = la.default_context()
+    log(f"generating {n}^3 lognormal field sigma={args.sigma}")
+    c = problems.lognormal(dims, args.sigma, 1)
+    A = la.StencilMatrix.values(la.BoxInfo(dims), c, ctx)
+    b = A.rhs()
+    log("AMG setup (FP32 hierarchy)")
+    M32 = la.MultiGridSolver(precision=np.float32).instantiate(A)
+    log(f"  {M32.setup_seconds:.2f} s, levels {[M32.level_size(l) for l in range(M32.level_num)]}")
+    M64 = la.MultiGridSolver(precision=np.float64).instantiate(A)
+    log(f"AMG setup (FP64 hierarchy) {M64.setup_seconds:.2f} s")
+
+    x = la.DenseVector(N, ctx=ctx)
+
+    def solve(M):
+        x.fill(0.0)
+        return la.ConjugateGradientsSolver(8, M).solve(A, x, b, rel_tol=1e-8, max_it=500)
+
+    for _ in range(args.warmup):
+        r = solve(M32)
+        log(f"warmup mixed: {r.iterations} it, status {r.status}")
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    its32 = []
+    for _ in range(args.steps):
+        r = solve(M32)
+        its32.append(r.iterations)
+        assert r.converged, r
+    ctx.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    x_mixed = x.numpy()
+    log(f"mixed: {t_mixed * 1e3:.1f} ms/solve, iterations {its32}")
+
+    r = solve(M64)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    its64 = []
+    for _ in range(args.steps):
+        r = solve(M64)
+        its64.append(r.iterations)
+        assert r.converged, r
+    ctx.synchronize()
+    t_fp64 = (time.perf_counter() - t0) / args.steps
+    x_fp64 = x.numpy()
+    log(f"fp64: {t_fp64 * 1e3:.1f} ms/solve, iterations {its64}")
+    rel_err = float(np.abs(x_mixed - x_fp64).max() / np.abs(x_fp64).max())
+
+    # ---- kernel rooflines (HIP events on the library stream) ----
+    rot = 3  # distinct buffer sets so the 256 MB infinity cache cannot serve repeats
+    xs = [la.DenseVector(problems.uniform(N, 10 + k), ctx=ctx) for k in range(rot)]
+    ys = [la.DenseVector(N, ctx=ctx) for _ in range(rot)]
+    launches = 60
+    t_spmv = kernel_ms(ctx, lambda k=0: A.spmv(xs[k % rot], ys[k % rot]), launches)
+    spmv_bytes = 24 * N  # coefficient + x read, y written (f64)
+    xs32 = [la.DenseVector(problems.uniform(N, 20 + k, np.float32), ctx=ctx) for k in range(rot)]
+    ys32 = [la.DenseVector(N, np.float32, ctx=ctx) for _ in range(rot)]
+    t_spmv32 = kernel_ms(ctx, lambda k=0: A.spmv(xs32[k % rot], ys32[k % rot]), launches)
+    bs = [la.DenseVector(problems.uniform(N, 30 + k), ctx=ctx) for k in range(rot)]
+    t_sor = kernel_ms(ctx, lambda k=0: A.sor(xs[k % rot], ys[k % rot], bs[k % rot], 1.0, True, False), 10)
+    sor_bytes = 32 * N  # coefficient, x_in, b read, x_out written
+    bs32 = [la.DenseVector(problems.uniform(N, 40 + k, np.float32), ctx=ctx) for k in range(rot)]
+    t_sor32 = kernel_ms(ctx, lambda k=0: A.sor(None, ys32[k % rot], bs32[k % rot], 1.0, True, False), 10)
+    sor32_bytes = 12 * N  # coefficient, b read, x_out written (pre-smoother from zero)
+    kern = {
+        "spmv_values_f64": {"ms": t_spmv, "GB/s": spmv_bytes / t_spmv / 1e6},
+        "spmv_values_f32": {"ms": t_spmv32, "GB/s": 12 * N / t_spmv32 / 1e6},
+        "sor_fwd_values_f64": {"ms": t_sor, "GB/s": sor_bytes / t_sor / 1e6},
+        "sor_fwd_zero_values_f32": {"ms": t_sor32, "GB/s": sor32_bytes / t_sor32 / 1e6},
+    }
+    for k, v in kern.items():
+        log(f"{k}: {v['ms']:.3f} ms, {v['GB/s']:.0f} GB/s")
+
+    # the dominant kernel of the mixed solve is the finest-level SOR sweep
+    dom_ms, dom_bytes = t_sor32, sor32_bytes
+    achieved = dom_bytes / dom_ms / 1e6
+    result = {
+        "metric": "AMG-CG time-to-1e-8 at 256^3, mixed vs FP64 (s); SpMV HBM GB/s vs peak",
+        "value": t_mixed,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"AMG-PCG to rel. residual 1e-8, {n}^3 7-point diffusion stencil, lognormal "
+                        f"sigma={args.sigma} per-cell coefficients (seed 1), harmonic-mean faces, "
+                        f"Dirichlet 1/0 on x, matrix-free finest level, SOR(1) V-cycle",
+            "grid": [n, n, n],
+            "fp64_seconds": t_fp64,
+            "mixed_seconds": t_mixed,
+            "mixed_over_fp64": t_mixed / t_fp64,
+            "iterations_mixed": its32[-1],
+            "iterations_fp64": its64[-1],
+            "mixed_vs_fp64_rel_error": rel_err,
+            "setup_seconds_mixed": M32.setup_seconds,
+            "setup_seconds_fp64": M64.setup_seconds,
+            "levels": [M32.level_size(l) for l in range(M32.level_num)],
+            "kernels": kern,
+            "spmv_roofline": {"bound": "hbm", "achieved": spmv_bytes / t_spmv / 1e6, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": spmv_bytes / t_spmv / 1e6 / HBM_PEAK_GBS},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "sor_wavefront<float, values, forward, regular, x_in=0> (finest-level pre-smoother)",
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle)")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.sigma, n, its64[-1])
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * lineal_amd — C ABI of the MI355X (gfx950) implementation of lineal's
+ * solve-phase hot path.  Plain pointers and sizes only; every function returns
+ * an lgpu_status and no exception crosses the boundary.  Host arrays passed in
+ * are borrowed for the duration of the call (copied to the device); the library
+ * owns the device memory of the objects it creates.  All work of one context
+ * is ordered on that context's HIP stream; a call returns after its work has
+ * been enqueued unless it documents a host synchronisation.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference's include/lineal/).
+ */
+#ifndef LINEAL_AMD_H
+#define LINEAL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LGPU_OK = 0,
+  LGPU_INVALID_ARG = 1,
+  LGPU_OUT_OF_MEMORY = 2,
+  LGPU_HIP_ERROR = 3,
+  LGPU_BREAKDOWN = 4,      /* zero denominator in a Krylov recurrence */
+  LGPU_NONFINITE = 5,      /* NaN/Inf residual */
+  LGPU_NOT_CONVERGED = 6,  /* max_it reached before rel_tol */
+  LGPU_NOT_SUPPORTED = 7,
+  LGPU_SYNC_TIMEOUT = 8    /* a wavefront kernel's bounded wait expired */
+} lgpu_status;
+
+typedef enum { LGPU_F64 = 0, LGPU_F32 = 1 } lgpu_dtype;
+
+typedef struct lgpu_ctx_s* lgpu_ctx;
+typedef struct lgpu_vec_s* lgpu_vec;
+typedef struct lgpu_op_s* lgpu_op;
+typedef struct lgpu_amg_s* lgpu_amg;
+
+/* ---- errors and context ------------------------------------------------ */
+const char* lgpu_status_string(int status);
+/* message of the last failing call on this host thread */
+const char* lgpu_last_error(void);
+
+/* One context per host thread/stream.  Replaces the execution policy of the
+ * reference Env (parallel/execution-policy/adaptive.hpp:146-349): work that the
+ * reference forks over a thread pool is launched on the context's stream. */
+int lgpu_ctx_create(int device, lgpu_ctx* out);
+int lgpu_ctx_destroy(lgpu_ctx ctx);
+int lgpu_ctx_synchronize(lgpu_ctx ctx);
+/* HIP events on the context stream: elapsed milliseconds between start and stop */
+int lgpu_timer_start(lgpu_ctx ctx);
+int lgpu_timer_stop(lgpu_ctx ctx, double* ms);
+
+/* ---- vectors: DenseVector (tensor/dynamic/dense/vector/shared.hpp:35-362) --- */
+int lgpu_vec_create(lgpu_ctx ctx, int64_t n, int dtype, lgpu_vec* out);
+/* borrow existing device memory (not freed by lgpu_vec_destroy) */
+int lgpu_vec_wrap(lgpu_ctx ctx, void* device_ptr, int64_t n, int dtype, lgpu_vec* out);
+int lgpu_vec_destroy(lgpu_vec v);
+int lgpu_vec_size(lgpu_vec v, int64_t* n);
+void* lgpu_vec_data(lgpu_vec v);
+/* host <-> device; host array has the vector's dtype; synchronous */
+int lgpu_vec_upload(lgpu_vec v, const void* host, int64_t n);
+int lgpu_vec_download(lgpu_vec v, void* host, int64_t n);
+/* assign(v, constant_like(v, value)) */
+int lgpu_vec_fill(lgpu_vec v, double value);
+/* assign(dst, src) with a precision conversion when the dtypes differ */
+int lgpu_vec_copy(lgpu_vec src, lgpu_vec dst);
+/* dot<f64>(a, b), component-wise/sink/accumulator.hpp:175-190 (synchronous) */
+int lgpu_vec_dot(lgpu_vec a, lgpu_vec b, double* out);
+
+/* ---- operators ----------------------------------------------------------- */
+/* Box geometry and Dirichlet data, discretization/box-equidistant.hpp:57-111 and
+ * tensor/dynamic/stencil/matrix/diffusion.hpp:52-96 (flow axis 0). */
+typedef struct {
+  int64_t n[3];        /* cells per axis, index = (x*ny + y)*nz + z */
+  double len[3];       /* axis lengths */
+  int zero_to_one;     /* ZeroToOne diagonal handling (1) or ZeroToZero (0) */
+  double p_start;      /* solution at the x = 0 border */
+  double p_end;        /* solution at the x = nx border */
+} lgpu_box;
+
+/* AdjacentStencilMatrix over StationaryDiffusionValuesValuator
+ * (tensor/dynamic/stencil/valuator/stationary-diffusion-values.hpp:31-211),
+ * one per-cell coefficient for all axes.  coeff: host array of n[0]*n[1]*n[2]. */
+int lgpu_stencil_values_create(lgpu_ctx ctx, const lgpu_box* box, const double* coeff,
+                               lgpu_op* out);
+/* AdjacentStencilMatrix over StationaryDiffusionLookupValuator
+ * (tensor/dynamic/stencil/valuator/stationary-diffusion-lookup.hpp:29-151):
+ * u8 material ids (< 4) and the material coefficient table. */
+int lgpu_stencil_lookup_create(lgpu_ctx ctx, const lgpu_box* box, const uint8_t* materials,
+                               const double mat_coeffs[4], lgpu_op* out);
+/* CsrMatrix (tensor/dynamic/csr/shared.hpp:37-1103): sorted columns, diagonal
+ * stored; values given in f64 and stored in `dtype`. */
+int lgpu_csr_create(lgpu_ctx ctx, int64_t n, const int64_t* rowptr, const int32_t* cols,
+                    const double* vals, int dtype, int symmetric, lgpu_op* out);
+/* csr_from_adjacent_stencil (tensor/dynamic/csr/create-from.hpp:37-239) without
+ * compression: CSR copy of a stencil operator, values stored in `dtype`. */
+int lgpu_csr_from_stencil(lgpu_op stencil, int dtype, lgpu_op* out);
+int lgpu_op_destroy(lgpu_op op);
+int lgpu_op_rows(lgpu_op op, int64_t* n);
+
+/* AdjacentStencilVector (tensor/dynamic/stencil/vector/shared.hpp:181-205):
+ * Dirichlet right-hand side of a stencil operator, in b's dtype. */
+int lgpu_op_rhs(lgpu_op op, lgpu_vec b);
+/* y = A x, MatrixVectorProductExpr (component-wise/matrix-vector-product.hpp:53-84);
+ * x and y share a dtype, which selects the arithmetic precision. */
+int lgpu_op_spmv(lgpu_op op, lgpu_vec x, lgpu_vec y);
+/* r = b - A x, subtract<Real>(rhs, multiply<Real>(lhs, x)) */
+int lgpu_op_residual(lgpu_op op, lgpu_vec b, lgpu_vec x, lgpu_vec r);
+/* One Jacobi sweep x_in -> x_out, JacobiSweepSink
+ * (linear-solver/iterative-stationary/jacobi/sweep.hpp:66-87); x_in may be NULL (= 0). */
+int lgpu_op_jacobi(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega);
+/* One SOR sweep x_in -> x_out in lexicographic order (one thread segment),
+ * SorSweepSink (linear-solver/iterative-stationary/gauss-seidel-variant/sweep.hpp:134-200);
+ * forward 1/0, ultra 0 (regular) / 1 (l1 variant); x_in may be NULL (= 0) but
+ * must not alias x_out. */
+int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
+                int forward, int ultra);
+
+/* ---- aggregation AMG ----------------------------------------------------- */
+/* HierarchyWrightParams + AggregationParams + DuneDependencyCriterion
+ * (multigrid/hierarchy/wright.hpp:46-55, aggregation/params.hpp, test/aux/default.hpp:170-188) */
+typedef struct {
+  int max_level_num;            /* 16 */
+  int64_t min_coarsen_size;     /* 32 */
+  double min_coarsen_factor;    /* 1.2 */
+  double strong_threshold;      /* 1/3 */
+  double aggressive_strong_threshold; /* 1/3 */
+  int64_t min_aggregate_size;   /* 6 */
+  int64_t max_aggregate_size;   /* 9 */
+  int precision;                /* LGPU_F64: FP64 V-cycle; LGPU_F32: FP32 hierarchy,
+                                   smoothers and level operators, FP64 coarsest solve */
+  int ultra;                    /* SOR variant of the smoothers */
+} lgpu_amg_params;
+
+void lgpu_amg_default_params(lgpu_amg_params* p);
+/* MultiGridSolver::instantiate -> HierarchyWrightInstance::create
+ * (multigrid/solver.hpp:188-196, hierarchy/wright.hpp:145-309): the hierarchy
+ * (strength criterion, aggregation with one thread segment, Galerkin products,
+ * coarsest Cholesky factor) is built on the host and uploaded once. */
+int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out);
+int lgpu_amg_destroy(lgpu_amg amg);
+int lgpu_amg_level_num(lgpu_amg amg, int* levels);
+int lgpu_amg_level_size(lgpu_amg amg, int level, int64_t* n);
+/* host copy of level `level`'s aggregate map (0xFFFFFFFE = isolated) */
+int lgpu_amg_aggregates(lgpu_amg amg, int level, uint32_t* host_map);
+/* host copy of coarse level `level`'s CSR (level >= 1); pass NULL arrays to
+ * query nnz.  Values are returned in f64. */
+int lgpu_amg_matrix(lgpu_amg amg, int level, int64_t* nnz, int64_t* rowptr, int32_t* cols,
+                    double* vals);
+/* seconds spent in the host setup */
+int lgpu_amg_setup_seconds(lgpu_amg amg, double* seconds);
+/* z = M r, one V-cycle from zero (MultiGridSolver::Instance::apply with a
+ * constant-zero in_sol, multigrid/solver.hpp:47-53); r and z are f64. */
+int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z);
+
+/* ---- Krylov solvers ------------------------------------------------------- */
+typedef struct {
+  double rel_tol;    /* stop when ||r_k||_2 <= rel_tol ||r_0||_2 */
+  int max_it;
+  int rere_every;    /* true residual every k-th iteration (8, test/aux/iterative.hpp:97-99); 0 never */
+} lgpu_krylov_params;
+
+/* run_iterative_solver (linear-solver/runner.hpp:17-84) over
+ * ConjugateGradientsSolver (conjugate-gradients/solver.hpp:50-115), optionally
+ * AMG-preconditioned.  x holds the initial guess and receives the solution
+ * (f64).  res_hist (max_it + 1 entries) may be NULL.  Returns LGPU_OK on
+ * convergence, LGPU_NOT_CONVERGED, LGPU_BREAKDOWN or LGPU_NONFINITE. */
+int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
+                  const lgpu_krylov_params* params, int* iterations, double* res_hist);
+/* BiCgStabSolver (bicgstab/solver.hpp:51-151), right-preconditioned. */
+int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
+                        const lgpu_krylov_params* params, int* iterations, double* res_hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,553 @@
+"""lineal_amd: MI355X (gfx950) implementation of lineal's solve-phase hot path.
+
+This module loads ``liblineal_amd.so`` (hand-written HIP kernels behind a C
+ABI, declared in ``include/lineal_amd.h``) through ctypes and mirrors the
+reference's matrix / solver / preconditioner interface:
+
+========================================  =============================================
+reference (include/lineal/...)            here
+========================================  =============================================
+DenseVector                               DenseVector
+AdjacentStencilMatrix + Values valuator   StencilMatrix.values
+AdjacentStencilMatrix + Lookup valuator   StencilMatrix.lookup
+AdjacentStencilVector                     StencilMatrix.rhs
+CsrMatrix / csr_from_adjacent_stencil     CsrMatrix, StencilMatrix.to_csr
+JacobiSolver / SorSolver                  JacobiSolver, SorSolver (+ .instantiate/.apply)
+MultiGridSolver + HierarchyWright         MultiGridSolver(relax, HierarchyWrightParams)
+ConjugateGradientsSolver                  ConjugateGradientsSolver
+BiCgStabSolver                            BiCgStabSolver
+run_iterative_solver                      run_iterative_solver
+========================================  =============================================
+
+There is no CPU fallback: importing works without a GPU, but every call that
+reaches the library needs the built ``liblineal_amd.so`` and a GPU and raises
+otherwise.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblineal_amd.so")
+
+F64, F32 = 0, 1
+OK, INVALID_ARG, OUT_OF_MEMORY, HIP_ERROR, BREAKDOWN, NONFINITE, NOT_CONVERGED, NOT_SUPPORTED, SYNC_TIMEOUT = range(9)
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+
+
+class LinealError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+class _Box(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64 * 3),
+        ("len", C.c_double * 3),
+        ("zero_to_one", C.c_int),
+        ("p_start", C.c_double),
+        ("p_end", C.c_double),
+    ]
+
+
+class _AmgParams(C.Structure):
+    _fields_ = [
+        ("max_level_num", C.c_int),
+        ("min_coarsen_size", C.c_int64),
+        ("min_coarsen_factor", C.c_double),
+        ("strong_threshold", C.c_double),
+        ("aggressive_strong_threshold", C.c_double),
+        ("min_aggregate_size", C.c_int64),
+        ("max_aggregate_size", C.c_int64),
+        ("precision", C.c_int),
+        ("ultra", C.c_int),
+    ]
+
+
+class _KrylovParams(C.Structure):
+    _fields_ = [("rel_tol", C.c_double), ("max_it", C.c_int), ("rere_every", C.c_int)]
+
+
+# exported symbol -> (restype, argtypes); the list mirrors include/lineal_amd.h
+SIGNATURES = {
+    "lgpu_status_string": (C.c_char_p, [C.c_int]),
+    "lgpu_last_error": (C.c_char_p, []),
+    "lgpu_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "lgpu_ctx_destroy": (C.c_int, [_vp]),
+    "lgpu_ctx_synchronize": (C.c_int, [_vp]),
+    "lgpu_timer_start": (C.c_int, [_vp]),
+    "lgpu_timer_stop": (C.c_int, [_vp, _f64p]),
+    "lgpu_vec_create": (C.c_int, [_vp, _i64, C.c_int, C.POINTER(_vp)]),
+    "lgpu_vec_wrap": (C.c_int, [_vp, _vp, _i64, C.c_int, C.POINTER(_vp)]),
+    "lgpu_vec_destroy": (C.c_int, [_vp]),
+    "lgpu_vec_size": (C.c_int, [_vp, _i64p]),
+    "lgpu_vec_data": (_vp, [_vp]),
+    "lgpu_vec_upload": (C.c_int, [_vp, _vp, _i64]),
+    "lgpu_vec_download": (C.c_int, [_vp, _vp, _i64]),
+    "lgpu_vec_fill": (C.c_int, [_vp, C.c_double]),
+    "lgpu_vec_copy": (C.c_int, [_vp, _vp]),
+    "lgpu_vec_dot": (C.c_int, [_vp, _vp, _f64p]),
+    "lgpu_stencil_values_create": (C.c_int, [_vp, C.POINTER(_Box), _f64p, C.POINTER(_vp)]),
+    "lgpu_stencil_lookup_create": (C.c_int, [_vp, C.POINTER(_Box), _u8p, _f64p, C.POINTER(_vp)]),
+    "lgpu_csr_create": (C.c_int, [_vp, _i64, _i64p, _i32p, _f64p, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "lgpu_csr_from_stencil": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "lgpu_op_destroy": (C.c_int, [_vp]),
+    "lgpu_op_rows": (C.c_int, [_vp, _i64p]),
+    "lgpu_op_rhs": (C.c_int, [_vp, _vp]),
+    "lgpu_op_spmv": (C.c_int, [_vp, _vp, _vp]),
+    "lgpu_op_residual": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "lgpu_op_jacobi": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
+    "lgpu_op_sor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int]),
+    "lgpu_amg_default_params": (None, [C.POINTER(_AmgParams)]),
+    "lgpu_amg_create": (C.c_int, [_vp, C.POINTER(_AmgParams), C.POINTER(_vp)]),
+    "lgpu_amg_destroy": (C.c_int, [_vp]),
+    "lgpu_amg_level_num": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "lgpu_amg_level_size": (C.c_int, [_vp, C.c_int, _i64p]),
+    "lgpu_amg_aggregates": (C.c_int, [_vp, C.c_int, _u32p]),
+    "lgpu_amg_matrix": (C.c_int, [_vp, C.c_int, _i64p, _i64p, _i32p, _f64p]),
+    "lgpu_amg_setup_seconds": (C.c_int, [_vp, _f64p]),
+    "lgpu_amg_vcycle": (C.c_int, [_vp, _vp, _vp]),
+    "lgpu_cg_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
+    "lgpu_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI; raises if the HIP library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LinealError(HIP_ERROR, f"{LIB_PATH} is missing: build it with lineal_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int, allow=()):
+    if status != OK and status not in allow:
+        msg = lib().lgpu_last_error().decode()
+        raise LinealError(status, f"{lib().lgpu_status_string(status).decode()}: {msg}")
+    return status
+
+
+def _dtype_code(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return F64
+    if dt == np.float32:
+        return F32
+    raise ValueError(f"unsupported dtype {dt}")
+
+
+class Context:
+    """One HIP stream on one device (replaces the reference Env's execution policy)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().lgpu_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        _check(lib().lgpu_ctx_synchronize(self.h))
+
+    def timer_start(self):
+        _check(lib().lgpu_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = C.c_double()
+        _check(lib().lgpu_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lgpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class DenseVector:
+    """Device vector (tensor/dynamic/dense/vector/shared.hpp:35-362)."""
+
+    def __init__(self, size_or_array, dtype=np.float64, ctx: Context | None = None, fill=None):
+        self.ctx = ctx or default_context()
+        if isinstance(size_or_array, (int, np.integer)):
+            self.n = int(size_or_array)
+            self.dtype = np.dtype(dtype)
+            host = None
+        else:
+            host = np.ascontiguousarray(size_or_array, dtype=dtype).reshape(-1)
+            self.n = host.size
+            self.dtype = host.dtype
+        h = _vp()
+        _check(lib().lgpu_vec_create(self.ctx.h, self.n, _dtype_code(self.dtype), C.byref(h)))
+        self.h = h
+        if host is not None:
+            self.upload(host)
+        elif fill is not None:
+            self.fill(fill)
+
+    @property
+    def size(self):
+        return self.n
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype).reshape(-1)
+        assert host.size == self.n
+        _check(lib().lgpu_vec_upload(self.h, host.ctypes.data_as(_vp), self.n))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        _check(lib().lgpu_vec_download(self.h, out.ctypes.data_as(_vp), self.n))
+        return out
+
+    def fill(self, value: float):
+        _check(lib().lgpu_vec_fill(self.h, float(value)))
+
+    def assign(self, other: "DenseVector"):
+        _check(lib().lgpu_vec_copy(other.h, self.h))
+
+    def dot(self, other: "DenseVector") -> float:
+        out = C.c_double()
+        _check(lib().lgpu_vec_dot(self.h, other.h, C.byref(out)))
+        return out.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().lgpu_vec_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _vh(v):
+    return None if v is None else v.h
+
+
+class _Matrix:
+    def __init__(self, h, ctx):
+        self.h = h
+        self.ctx = ctx
+
+    @property
+    def row_num(self) -> int:
+        n = C.c_int64()
+        _check(lib().lgpu_op_rows(self.h, C.byref(n)))
+        return n.value
+
+    column_num = row_num
+
+    def spmv(self, x: DenseVector, y: DenseVector):
+        """y = A x (MatrixVectorProductExpr)."""
+        _check(lib().lgpu_op_spmv(self.h, x.h, y.h))
+        return y
+
+    def __mul__(self, x: DenseVector) -> DenseVector:
+        y = DenseVector(x.n, x.dtype, self.ctx)
+        return self.spmv(x, y)
+
+    def residual(self, b: DenseVector, x: DenseVector, r: DenseVector):
+        _check(lib().lgpu_op_residual(self.h, b.h, x.h, r.h))
+        return r
+
+    def jacobi(self, x_in, x_out, b, omega):
+        _check(lib().lgpu_op_jacobi(self.h, _vh(x_in), x_out.h, b.h, float(omega)))
+        return x_out
+
+    def sor(self, x_in, x_out, b, omega=1.0, forward=True, ultra=False):
+        _check(lib().lgpu_op_sor(self.h, _vh(x_in), x_out.h, b.h, float(omega), int(forward), int(ultra)))
+        return x_out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().lgpu_op_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+@dataclass
+class BoxInfo:
+    """Equidistant box (discretization/box-equidistant.hpp:57-111), flow axis 0."""
+
+    dims: tuple
+    lengths: tuple | None = None
+    zero_to_one: bool = True
+    p_start: float = 1.0
+    p_end: float = 0.0
+
+    def _struct(self):
+        b = _Box()
+        for d in range(3):
+            b.n[d] = int(self.dims[d])
+            b.len[d] = float(self.lengths[d]) if self.lengths is not None else float(self.dims[d])
+        b.zero_to_one = int(self.zero_to_one)
+        b.p_start, b.p_end = float(self.p_start), float(self.p_end)
+        return b
+
+
+class StencilMatrix(_Matrix):
+    """AdjacentStencilMatrix (tensor/dynamic/stencil/matrix/shared.hpp:35-360)."""
+
+    @classmethod
+    def values(cls, box: BoxInfo, coeff, ctx: Context | None = None):
+        ctx = ctx or default_context()
+        c = np.ascontiguousarray(coeff, dtype=np.float64).reshape(-1)
+        h = _vp()
+        bs = box._struct()
+        _check(lib().lgpu_stencil_values_create(ctx.h, C.byref(bs), c.ctypes.data_as(_f64p), C.byref(h)))
+        m = cls(h, ctx)
+        m.box = box
+        return m
+
+    @classmethod
+    def lookup(cls, box: BoxInfo, materials, mat_coeffs=(0.0, 1.0, 2.0, 3.0), ctx: Context | None = None):
+        ctx = ctx or default_context()
+        mats = np.ascontiguousarray(materials, dtype=np.uint8).reshape(-1)
+        mc = np.ascontiguousarray(mat_coeffs, dtype=np.float64)
+        h = _vp()
+        bs = box._struct()
+        _check(lib().lgpu_stencil_lookup_create(ctx.h, C.byref(bs), mats.ctypes.data_as(_u8p),
+                                                mc.ctypes.data_as(_f64p), C.byref(h)))
+        m = cls(h, ctx)
+        m.box = box
+        return m
+
+    def rhs(self, dtype=np.float64) -> DenseVector:
+        """AdjacentStencilVector: the Dirichlet right-hand side."""
+        b = DenseVector(self.row_num, dtype, self.ctx)
+        _check(lib().lgpu_op_rhs(self.h, b.h))
+        return b
+
+    def to_csr(self, dtype=np.float64) -> "CsrMatrix":
+        h = _vp()
+        _check(lib().lgpu_csr_from_stencil(self.h, _dtype_code(dtype), C.byref(h)))
+        return CsrMatrix(h, self.ctx)
+
+
+class CsrMatrix(_Matrix):
+    """CsrMatrix (tensor/dynamic/csr/shared.hpp:37-1103)."""
+
+    @classmethod
+    def from_arrays(cls, rowptr, cols, vals, dtype=np.float64, symmetric=True, ctx: Context | None = None):
+        ctx = ctx or default_context()
+        rp = np.ascontiguousarray(rowptr, np.int64)
+        ci = np.ascontiguousarray(cols, np.int32)
+        va = np.ascontiguousarray(vals, np.float64)
+        h = _vp()
+        _check(lib().lgpu_csr_create(ctx.h, rp.size - 1, rp.ctypes.data_as(_i64p), ci.ctypes.data_as(_i32p),
+                                     va.ctypes.data_as(_f64p), _dtype_code(dtype), int(symmetric), C.byref(h)))
+        return cls(h, ctx)
+
+
+# ---------------------------------------------------------------------------
+# stationary solvers (linear-solver/iterative-stationary)
+
+
+class JacobiSolver:
+    """JacobiSolver (iterative-stationary/jacobi/solver.hpp:21-39), one sweep per apply."""
+
+    def __init__(self, relax: float):
+        self.relax = relax
+
+    def instantiate(self, lhs: _Matrix):
+        solver = self
+
+        class Instance:
+            def apply(self, sol_in, sol_out, rhs):
+                lhs.jacobi(sol_in, sol_out, rhs, solver.relax)
+
+        return Instance()
+
+
+class SorSolver:
+    """SorSolver (gauss-seidel-variant/solver.hpp:22-41) with one thread segment."""
+
+    def __init__(self, relax: float, forward: bool = True, ultra: bool = False):
+        self.relax, self.forward, self.ultra = relax, forward, ultra
+
+    def instantiate(self, lhs: _Matrix):
+        solver = self
+
+        class Instance:
+            def apply(self, sol_in, sol_out, rhs):
+                lhs.sor(sol_in, sol_out, rhs, solver.relax, solver.forward, solver.ultra)
+
+        return Instance()
+
+
+# ---------------------------------------------------------------------------
+# AMG
+
+
+@dataclass
+class HierarchyWrightParams:
+    """HierarchyWright + aggregation + criterion parameters (test/aux/default.hpp:170-188)."""
+
+    max_level_num: int = 16
+    min_coarsen_size: int = 32
+    min_coarsen_factor: float = 1.2
+    strong_threshold: float = 1.0 / 3.0
+    aggressive_strong_threshold: float = 1.0 / 3.0
+    min_aggregate_size: int = 6
+    max_aggregate_size: int = 9
+
+    def _struct(self, precision: int, ultra: bool):
+        return _AmgParams(self.max_level_num, self.min_coarsen_size, self.min_coarsen_factor,
+                          self.strong_threshold, self.aggressive_strong_threshold,
+                          self.min_aggregate_size, self.max_aggregate_size, precision, int(ultra))
+
+
+class MultiGridSolver:
+    """MultiGridSolver{relax=1, DefaultCycle{REGULAR,1}, wright} (multigrid/solver.hpp:183-196).
+
+    precision: np.float64 for an FP64 V-cycle, np.float32 for the FP32 hierarchy,
+    smoothers and level operators with an FP64 coarsest solve.
+    """
+
+    def __init__(self, wright: HierarchyWrightParams | None = None, precision=np.float64, ultra=False):
+        self.wright = wright or HierarchyWrightParams()
+        self.precision = precision
+        self.ultra = ultra
+
+    def instantiate(self, lhs: _Matrix) -> "MultiGridInstance":
+        return MultiGridInstance(lhs, self)
+
+
+class MultiGridInstance:
+    def __init__(self, lhs: _Matrix, solver: MultiGridSolver):
+        self.lhs = lhs
+        p = solver.wright._struct(_dtype_code(solver.precision), solver.ultra)
+        h = _vp()
+        _check(lib().lgpu_amg_create(lhs.h, C.byref(p), C.byref(h)))
+        self.h = h
+
+    @property
+    def level_num(self) -> int:
+        n = C.c_int()
+        _check(lib().lgpu_amg_level_num(self.h, C.byref(n)))
+        return n.value
+
+    def level_size(self, level: int) -> int:
+        n = C.c_int64()
+        _check(lib().lgpu_amg_level_size(self.h, level, C.byref(n)))
+        return n.value
+
+    def aggregates(self, level: int) -> np.ndarray:
+        out = np.empty(self.level_size(level), np.uint32)
+        _check(lib().lgpu_amg_aggregates(self.h, level, out.ctypes.data_as(_u32p)))
+        return out
+
+    def matrix(self, level: int):
+        nnz = C.c_int64()
+        _check(lib().lgpu_amg_matrix(self.h, level, C.byref(nnz), None, None, None))
+        n = self.level_size(level)
+        rp = np.empty(n + 1, np.int64)
+        cols = np.empty(max(nnz.value, 1), np.int32)
+        vals = np.empty(max(nnz.value, 1), np.float64)
+        _check(lib().lgpu_amg_matrix(self.h, level, C.byref(nnz), rp.ctypes.data_as(_i64p),
+                                     cols.ctypes.data_as(_i32p), vals.ctypes.data_as(_f64p)))
+        return rp, cols[: nnz.value], vals[: nnz.value]
+
+    @property
+    def setup_seconds(self) -> float:
+        s = C.c_double()
+        _check(lib().lgpu_amg_setup_seconds(self.h, C.byref(s)))
+        return s.value
+
+    def apply(self, r: DenseVector, z: DenseVector):
+        """z = M r: one V-cycle from a zero initial guess."""
+        _check(lib().lgpu_amg_vcycle(self.h, r.h, z.h))
+        return z
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().lgpu_amg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Krylov solvers
+
+
+@dataclass
+class SolveResult:
+    status: int
+    iterations: int
+    residuals: np.ndarray
+
+    @property
+    def converged(self):
+        return self.status == OK
+
+
+class _Krylov:
+    _fn = ""
+
+    def __init__(self, rere_every: int = 8, precon: MultiGridInstance | None = None):
+        self.rere_every = rere_every
+        self.precon = precon
+
+    def solve(self, lhs: _Matrix, sol: DenseVector, rhs: DenseVector, rel_tol=1e-8, max_it=1000) -> SolveResult:
+        p = _KrylovParams(float(rel_tol), int(max_it), int(self.rere_every))
+        it = C.c_int()
+        hist = np.zeros(max_it + 1)
+        st = getattr(lib(), self._fn)(lhs.h, _vh(self.precon), rhs.h, sol.h, C.byref(p), C.byref(it),
+                                      hist.ctypes.data_as(_f64p))
+        _check(st, allow=(NOT_CONVERGED, BREAKDOWN, NONFINITE))
+        return SolveResult(st, it.value, hist[: it.value + 1])
+
+
+class ConjugateGradientsSolver(_Krylov):
+    """ConjugateGradientsSolver (conjugate-gradients/solver.hpp:25-221)."""
+
+    _fn = "lgpu_cg_solve"
+
+
+class BiCgStabSolver(_Krylov):
+    """BiCgStabSolver (bicgstab/solver.hpp:26-259)."""
+
+    _fn = "lgpu_bicgstab_solve"
+
+
+def run_iterative_solver(solver: _Krylov, lhs, sol, rhs, rel_tol=1e-8, max_it=1000) -> SolveResult:
+    """run_iterative_solver (linear-solver/runner.hpp:17-84) with a relative-residual stop."""
+    return solver.solve(lhs, sol, rhs, rel_tol, max_it)

@@ -1,0 +1,1097 @@
+// C ABI of lineal_amd: object lifetimes, dispatch, the V-cycle and the Krylov
+// drivers.  Every entry point catches every exception and returns a status.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <thread>
+
+#include "internal.hpp"
+#include "setup.hpp"
+
+namespace lgpu {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace lgpu
+
+using namespace lgpu;
+
+#define API_BEGIN try {
+#define API_END                                           \
+  }                                                       \
+  catch (const lgpu::Error& e) {                          \
+    lgpu::set_error(e.msg);                               \
+    return e.status;                                      \
+  }                                                       \
+  catch (const std::bad_alloc&) {                         \
+    lgpu::set_error("host allocation failed");            \
+    return LGPU_OUT_OF_MEMORY;                            \
+  }                                                       \
+  catch (const std::exception& e) {                       \
+    lgpu::set_error(e.what());                            \
+    return LGPU_HIP_ERROR;                                \
+  }                                                       \
+  catch (...) {                                           \
+    lgpu::set_error("unknown error");                     \
+    return LGPU_HIP_ERROR;                                \
+  }                                                       \
+  return LGPU_OK;
+
+// ---------------------------------------------------------------------------
+// objects
+
+struct lgpu_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf partials;        // reduction scratch
+  int64_t partial_cap = 0;
+  DevBuf scal;            // SC_COUNT device scalars
+  double* host_scal = nullptr;  // pinned mirror
+  double* partials_for(int64_t need) {
+    need *= 2;  // dual reductions
+    if (need > partial_cap) {
+      partials.alloc(size_t(need) * sizeof(double));
+      partial_cap = need;
+    }
+    return partials.as<double>();
+  }
+};
+
+struct lgpu_vec_s {
+  lgpu_ctx ctx = nullptr;
+  int64_t n = 0;
+  bool f32 = false;
+  void* ptr = nullptr;
+  DevBuf own;
+};
+
+struct CsrDev {
+  int64_t n = 0;
+  bool f32 = false;
+  bool symmetric = true;
+  DevBuf rowptr, cols, vals, sched, flags;
+  int epoch = 0;
+  CsrDesc desc() const { return CsrDesc{n, rowptr.as<int64_t>(), cols.as<int32_t>(), vals.ptr, f32 ? 1 : 0}; }
+  void upload(const setup::RowMatrix& m, bool as_f32) {
+    n = m.n;
+    f32 = as_f32;
+    symmetric = m.symmetric;
+    rowptr.alloc(size_t(n + 1) * sizeof(int64_t));
+    LGPU_HIP_CHECK(hipMemcpy(rowptr.ptr, m.rowptr.data(), size_t(n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    const size_t nnz = m.cols.size();
+    cols.alloc(nnz * sizeof(int32_t));
+    LGPU_HIP_CHECK(hipMemcpy(cols.ptr, m.cols.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (as_f32) {
+      std::vector<float> v(m.vals.begin(), m.vals.end());
+      vals.alloc(nnz * sizeof(float));
+      LGPU_HIP_CHECK(hipMemcpy(vals.ptr, v.data(), nnz * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+      vals.alloc(nnz * sizeof(double));
+      LGPU_HIP_CHECK(hipMemcpy(vals.ptr, m.vals.data(), nnz * sizeof(double), hipMemcpyHostToDevice));
+    }
+    sched.alloc(4 * sizeof(int));
+    flags.alloc(size_t(n) * sizeof(int));
+    LGPU_HIP_CHECK(hipMemset(sched.ptr, 0, 4 * sizeof(int)));
+    LGPU_HIP_CHECK(hipMemset(flags.ptr, 0, size_t(n) * sizeof(int)));
+    epoch = 0;
+  }
+};
+
+struct lgpu_op_s {
+  lgpu_ctx ctx = nullptr;
+  bool is_csr = false;
+  // stencil
+  lgpu_box box{};
+  StencilDesc desc{};
+  DevBuf c64, c32, mat;
+  std::vector<double> host_coeff;
+  std::vector<uint8_t> host_mat;
+  DevBuf sor_state, sor_order;
+  int64_t sor_cols = 0;
+  // csr
+  CsrDev csr;
+  setup::RowMatrix host_csr;
+
+  int64_t rows() const { return is_csr ? csr.n : desc.nx * desc.ny * desc.nz; }
+  void ensure_f32() {
+    if (is_csr || desc.kind != 0 || c32.ptr) return;
+    const int64_t n = rows();
+    c32.alloc(size_t(n) * sizeof(float));
+    vec_convert(c64.ptr, false, c32.ptr, true, n, ctx->stream);
+    desc.c32 = c32.as<float>();
+  }
+};
+
+struct AmgLevel {
+  CsrDev a;            // level operator (levels >= 1)
+  DevBuf agg;          // this level -> next level map
+  DevBuf c2f_ptr, c2f_idx;
+  DevBuf rhs, sol, tmp, aux;
+  int64_t n = 0;
+};
+
+struct lgpu_amg_s {
+  lgpu_ctx ctx = nullptr;
+  lgpu_op finest = nullptr;
+  lgpu_amg_params params{};
+  setup::Hierarchy host;
+  std::vector<AmgLevel> lv;
+  DevBuf chol;
+  double setup_seconds = 0.0;
+  bool f32 = false;
+  DevBuf r32, z32;  // finest f32 copies for the mixed-precision cycle
+};
+
+namespace {
+
+size_t esize(bool f32) { return f32 ? sizeof(float) : sizeof(double); }
+
+void check_vec(lgpu_vec v, int64_t n, const char* what) {
+  LGPU_REQUIRE(v != nullptr, std::string(what) + " is NULL");
+  LGPU_REQUIRE(v->n == n, std::string(what) + " has the wrong size");
+}
+
+double host_hmean(double a, double b) {
+  const double p = a * b;
+  return p != 0.0 ? (p + p) / (a + b) : 0.0;
+}
+float host_hmean(float a, float b) {
+  const float p = a * b;
+  return p != 0.0f ? (p + p) / (a + b) : 0.0f;
+}
+
+void fill_desc(lgpu_op op, const lgpu_box* box) {
+  StencilDesc& d = op->desc;
+  d.nx = box->n[0];
+  d.ny = box->n[1];
+  d.nz = box->n[2];
+  // discretization/box-equidistant.hpp:62-79
+  double h[3];
+  for (int k = 0; k < 3; ++k) h[k] = box->len[k] / double(box->n[k]);
+  for (int k = 0; k < 3; ++k) {
+    double area = 1.0;
+    for (int e = 0; e < 3; ++e)
+      if (e != k) area *= h[e];
+    d.q[k] = area / h[k];
+  }
+  d.zero_to_one = box->zero_to_one;
+  d.p_start = box->p_start;
+  d.p_end = box->p_end;
+}
+
+void init_sor_sched(lgpu_op op) {
+  std::vector<uint32_t> order;
+  sor_sched_order(op->desc, order);
+  op->sor_cols = int64_t(order.size());
+  op->sor_order.alloc(order.size() * sizeof(uint32_t));
+  LGPU_HIP_CHECK(hipMemcpy(op->sor_order.ptr, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  const size_t ints = sor_sched_ints(op->desc);
+  op->sor_state.alloc(ints * sizeof(int));
+  LGPU_HIP_CHECK(hipMemset(op->sor_state.ptr, 0, ints * sizeof(int)));
+}
+
+// Host evaluation of the stencil rows in f64 in iterate order (-x,-y,-z,+z,+y,+x,
+// diagonal last), tensor/dynamic/stencil/matrix/algorithm.hpp:91-227; the same
+// operations as the device's eval_row.
+void stencil_host_rows(const lgpu_op op, setup::RowMatrix& m) {
+  const StencilDesc& d = op->desc;
+  const int64_t nx = d.nx, ny = d.ny, nz = d.nz, n = nx * ny * nz, nyz = ny * nz;
+  m = setup::RowMatrix{};
+  m.n = n;
+  m.f32 = false;
+  m.symmetric = true;
+  m.rowptr.resize(size_t(n + 1));
+  m.rowptr[0] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t x = i / nyz, y = (i / nz) % ny, z = i % nz;
+    const int cnt = 1 + (x > 0) + (x + 1 < nx) + (y > 0) + (y + 1 < ny) + (z > 0) + (z + 1 < nz);
+    m.rowptr[i + 1] = m.rowptr[i] + cnt;
+  }
+  m.cols.resize(size_t(m.rowptr[n]));
+  m.vals.resize(size_t(m.rowptr[n]));
+  auto coeff = [&](int64_t i) {
+    return d.kind == 0 ? op->host_coeff[size_t(i)] : d.mc[op->host_mat[size_t(i)]];
+  };
+  auto conn = [&](int64_t i, int64_t j, int ax) {
+    if (d.kind == 0) return host_hmean(op->host_coeff[size_t(i)], op->host_coeff[size_t(j)]) * d.q[ax];
+    return d.tab64[ax][op->host_mat[size_t(i)] * 4 + op->host_mat[size_t(j)]];
+  };
+  auto work = [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const int64_t x = i / nyz, y = (i / nz) % ny, z = i % nz;
+      const int64_t pos[3] = {x, y, z}, off[3] = {nyz, nz, 1};
+      const double ci = coeff(i);
+      double diag = 0.0;
+      if (x == 0) diag += (2.0 * d.q[0]) * ci;
+      if (x + 1 == nx) diag += (2.0 * d.q[0]) * ci;
+      int64_t k = m.rowptr[i];
+      for (int ax = 0; ax < 3; ++ax)
+        if (pos[ax] > 0) {
+          const double w = conn(i, i - off[ax], ax);
+          m.cols[size_t(k)] = int32_t(i - off[ax]);
+          m.vals[size_t(k++)] = -w;
+          diag += w;
+        }
+      for (int ax = 2; ax >= 0; --ax) {
+        const int64_t ext = ax == 0 ? nx : (ax == 1 ? ny : nz);
+        if (pos[ax] + 1 < ext) {
+          const double w = conn(i, i + off[ax], ax);
+          m.cols[size_t(k)] = int32_t(i + off[ax]);
+          m.vals[size_t(k++)] = -w;
+          diag += w;
+        }
+      }
+      if (d.zero_to_one && diag == 0.0) diag = 1.0;
+      m.cols[size_t(k)] = int32_t(i);
+      m.vals[size_t(k)] = diag;
+    }
+  };
+  const int nt = int(std::max(1u, std::thread::hardware_concurrency()));
+  if (n < 65536 || nt == 1) {
+    work(0, n);
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) pool.emplace_back(work, n * t / nt, n * (t + 1) / nt);
+    for (auto& th : pool) th.join();
+  }
+}
+
+// dispatch of one matrix pass
+void op_apply(lgpu_op op, int mode, bool f32, const void* x, const void* b, void* y, double omega,
+              double* partials) {
+  hipStream_t st = op->ctx->stream;
+  if (op->is_csr) {
+    LGPU_REQUIRE(op->csr.f32 == f32, "CSR value type and vector type differ");
+    csr_apply(op->csr.desc(), mode, x, b, y, omega, partials, st);
+  } else {
+    if (f32) op->ensure_f32();
+    stencil_apply(op->desc, mode, f32, x, b, y, omega, partials, st);
+  }
+}
+
+int64_t op_partials(lgpu_op op) {
+  return op->is_csr ? csr_partials(op->csr.desc()) : stencil_partials(op->desc);
+}
+
+void csr_sweep(CsrDev& a, const void* x_in, void* x_out, const void* b, double omega, bool fwd,
+               bool ultra, hipStream_t st) {
+  if (++a.epoch == 0x7fffffff) {
+    vec_fill_int(a.flags.as<int>(), a.n, 0, st);
+    a.epoch = 1;
+  }
+  csr_sor(a.desc(), x_in, x_out, b, omega, fwd, ultra, a.sched.as<int>(), a.flags.as<int>(), a.epoch, st);
+}
+
+void op_sor(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, double omega,
+            bool fwd, bool ultra) {
+  hipStream_t st = op->ctx->stream;
+  if (op->is_csr) {
+    LGPU_REQUIRE(op->csr.f32 == f32, "CSR value type and vector type differ");
+    csr_sweep(op->csr, x_in, x_out, b, omega, fwd, ultra, st);
+  } else {
+    if (f32) op->ensure_f32();
+    SorSched s;
+    s.state = op->sor_state.as<int>();
+    s.order = op->sor_order.as<uint32_t>();
+    s.ncols = op->sor_cols;
+    stencil_sor(op->desc, f32, x_in, x_out, b, omega, fwd, ultra, s, st);
+  }
+}
+
+// sticky timeout words of the wavefront kernels
+int read_int(hipStream_t st, const int* dev) {
+  int v = 0;
+  LGPU_HIP_CHECK(hipMemcpyAsync(&v, dev, sizeof(int), hipMemcpyDeviceToHost, st));
+  LGPU_HIP_CHECK(hipStreamSynchronize(st));
+  return v;
+}
+
+bool op_sync_error(lgpu_op op) {
+  hipStream_t st = op->ctx->stream;
+  if (op->is_csr) return read_int(st, op->csr.sched.as<int>() + 1) != 0;
+  return read_int(st, op->sor_state.as<int>() + 1) != 0;
+}
+
+double read_scalar(lgpu_ctx ctx, int slot) {
+  LGPU_HIP_CHECK(hipMemcpyAsync(ctx->host_scal, ctx->scal.as<double>() + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return ctx->host_scal[0];
+}
+
+// r = b - A x (f64), ||r||^2 into scal[slot]
+void residual_norm(lgpu_op op, const double* b, const double* x, double* r, int slot) {
+  lgpu_ctx ctx = op->ctx;
+  const int64_t np = op_partials(op);
+  double* part = ctx->partials_for(np);
+  op_apply(op, SM_RESID_NORM, false, x, b, r, 0.0, part);
+  reduce_partials(part, np, ctx->scal.as<double>() + slot, ctx->stream);
+}
+
+void dot_into(lgpu_ctx ctx, const void* a, const void* b, bool f32, int64_t n, int slot) {
+  const int64_t np = dot_partials(n);
+  double* part = ctx->partials_for(np);
+  vec_dot(a, b, f32, n, part, ctx->stream);
+  reduce_partials(part, np, ctx->scal.as<double>() + slot, ctx->stream);
+}
+
+// One V-cycle in precision T from zero: z = M r (multigrid/solver.hpp:56-176)
+void vcycle(lgpu_amg amg, const void* r, void* z) {
+  const bool f32 = amg->f32;
+  hipStream_t st = amg->ctx->stream;
+  const int L = int(amg->lv.size());
+  const bool ultra = amg->params.ultra != 0;
+  const double relax = 1.0, inv_relax = 1.0 / relax;
+  auto xs = [&](int l) { return l == 0 ? z : amg->lv[size_t(l)].sol.ptr; };
+  auto bs = [&](int l) { return l == 0 ? r : amg->lv[size_t(l)].rhs.ptr; };
+  for (int l = 0; l + 1 < L; ++l) {
+    AmgLevel& cur = amg->lv[size_t(l)];
+    if (l == 0) {
+      op_sor(amg->finest, f32, nullptr, xs(0), bs(0), relax, true, ultra);
+      op_apply(amg->finest, SM_RESID, f32, xs(0), bs(0), cur.tmp.ptr, 0.0, nullptr);
+    } else {
+      csr_sweep(cur.a, nullptr, xs(l), bs(l), relax, true, ultra, st);
+      csr_apply(cur.a.desc(), SM_RESID, xs(l), bs(l), cur.tmp.ptr, 0.0, nullptr, st);
+    }
+    const int64_t nc = amg->lv[size_t(l + 1)].n;
+    restrict_sum(cur.tmp.ptr, f32, cur.c2f_ptr.as<int64_t>(), cur.c2f_idx.as<int32_t>(), nc,
+                 amg->lv[size_t(l + 1)].rhs.ptr, st);
+  }
+  {
+    AmgLevel& c = amg->lv[size_t(L - 1)];
+    dense_cholesky_solve(amg->chol.as<double>(), c.n, c.rhs.ptr, c.sol.ptr, f32, st);
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    AmgLevel& cur = amg->lv[size_t(l)];
+    prolong_add(xs(l), amg->lv[size_t(l + 1)].sol.ptr, f32, cur.agg.as<uint32_t>(), cur.n,
+                inv_relax, cur.aux.ptr, st);
+    if (l == 0) op_sor(amg->finest, f32, cur.aux.ptr, xs(0), bs(0), relax, false, ultra);
+    else csr_sweep(cur.a, cur.aux.ptr, xs(l), bs(l), relax, false, ultra, st);
+  }
+}
+
+// z (f64) = M r (f64)
+void precondition(lgpu_amg amg, const double* r, double* z) {
+  if (!amg->f32) {
+    vcycle(amg, r, z);
+    return;
+  }
+  const int64_t n = amg->lv[0].n;
+  hipStream_t st = amg->ctx->stream;
+  vec_convert(r, false, amg->r32.ptr, true, n, st);
+  vcycle(amg, amg->r32.ptr, amg->z32.ptr);
+  vec_convert(amg->z32.ptr, true, z, false, n, st);
+}
+
+bool amg_sync_error(lgpu_amg amg) {
+  bool e = op_sync_error(amg->finest);
+  for (size_t l = 1; l < amg->lv.size(); ++l)
+    e = e || read_int(amg->ctx->stream, amg->lv[l].a.sched.as<int>() + 1) != 0;
+  return e;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// errors, context
+
+extern "C" const char* lgpu_status_string(int status) {
+  switch (status) {
+    case LGPU_OK: return "ok";
+    case LGPU_INVALID_ARG: return "invalid argument";
+    case LGPU_OUT_OF_MEMORY: return "out of memory";
+    case LGPU_HIP_ERROR: return "HIP error";
+    case LGPU_BREAKDOWN: return "Krylov breakdown";
+    case LGPU_NONFINITE: return "non-finite residual";
+    case LGPU_NOT_CONVERGED: return "not converged";
+    case LGPU_NOT_SUPPORTED: return "not supported";
+    case LGPU_SYNC_TIMEOUT: return "wavefront wait timed out";
+    default: return "unknown status";
+  }
+}
+
+extern "C" const char* lgpu_last_error(void) { return lgpu::g_last_error.c_str(); }
+
+extern "C" int lgpu_ctx_create(int device, lgpu_ctx* out) {
+  API_BEGIN
+  LGPU_REQUIRE(out != nullptr, "out is NULL");
+  auto ctx = std::make_unique<lgpu_ctx_s>();
+  ctx->device = device;
+  LGPU_HIP_CHECK(hipSetDevice(device));
+  LGPU_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  LGPU_HIP_CHECK(hipEventCreate(&ctx->ev0));
+  LGPU_HIP_CHECK(hipEventCreate(&ctx->ev1));
+  ctx->scal.alloc(SC_COUNT * sizeof(double));
+  LGPU_HIP_CHECK(hipMemset(ctx->scal.ptr, 0, SC_COUNT * sizeof(double)));
+  LGPU_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_scal), SC_COUNT * sizeof(double), hipHostMallocDefault));
+  ctx->partials_for(4096);
+  *out = ctx.release();
+  API_END
+}
+
+extern "C" int lgpu_ctx_destroy(lgpu_ctx ctx) {
+  API_BEGIN
+  if (!ctx) return LGPU_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  API_END
+}
+
+extern "C" int lgpu_ctx_synchronize(lgpu_ctx ctx) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx, "ctx is NULL");
+  LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  API_END
+}
+
+extern "C" int lgpu_timer_start(lgpu_ctx ctx) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx, "ctx is NULL");
+  LGPU_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+  API_END
+}
+
+extern "C" int lgpu_timer_stop(lgpu_ctx ctx, double* ms) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && ms, "NULL argument");
+  LGPU_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+  LGPU_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+  float f = 0;
+  LGPU_HIP_CHECK(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+  *ms = f;
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// vectors
+
+extern "C" int lgpu_vec_create(lgpu_ctx ctx, int64_t n, int dtype, lgpu_vec* out) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && out && n >= 0, "invalid vector arguments");
+  LGPU_REQUIRE(dtype == LGPU_F64 || dtype == LGPU_F32, "unknown dtype");
+  auto v = std::make_unique<lgpu_vec_s>();
+  v->ctx = ctx;
+  v->n = n;
+  v->f32 = dtype == LGPU_F32;
+  v->own.alloc(size_t(n) * esize(v->f32));
+  v->ptr = v->own.ptr;
+  *out = v.release();
+  API_END
+}
+
+extern "C" int lgpu_vec_wrap(lgpu_ctx ctx, void* device_ptr, int64_t n, int dtype, lgpu_vec* out) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && out && device_ptr && n >= 0, "invalid vector arguments");
+  LGPU_REQUIRE(dtype == LGPU_F64 || dtype == LGPU_F32, "unknown dtype");
+  auto v = std::make_unique<lgpu_vec_s>();
+  v->ctx = ctx;
+  v->n = n;
+  v->f32 = dtype == LGPU_F32;
+  v->ptr = device_ptr;
+  *out = v.release();
+  API_END
+}
+
+extern "C" int lgpu_vec_destroy(lgpu_vec v) {
+  API_BEGIN
+  if (v) {
+    (void)hipStreamSynchronize(v->ctx->stream);
+    delete v;
+  }
+  API_END
+}
+
+extern "C" int lgpu_vec_size(lgpu_vec v, int64_t* n) {
+  API_BEGIN
+  LGPU_REQUIRE(v && n, "NULL argument");
+  *n = v->n;
+  API_END
+}
+
+extern "C" void* lgpu_vec_data(lgpu_vec v) { return v ? v->ptr : nullptr; }
+
+extern "C" int lgpu_vec_upload(lgpu_vec v, const void* host, int64_t n) {
+  API_BEGIN
+  LGPU_REQUIRE(v && (host || n == 0), "NULL argument");
+  LGPU_REQUIRE(n == v->n, "size mismatch");
+  LGPU_HIP_CHECK(hipMemcpyAsync(v->ptr, host, size_t(n) * esize(v->f32), hipMemcpyHostToDevice, v->ctx->stream));
+  LGPU_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+  API_END
+}
+
+extern "C" int lgpu_vec_download(lgpu_vec v, void* host, int64_t n) {
+  API_BEGIN
+  LGPU_REQUIRE(v && (host || n == 0), "NULL argument");
+  LGPU_REQUIRE(n == v->n, "size mismatch");
+  LGPU_HIP_CHECK(hipMemcpyAsync(host, v->ptr, size_t(n) * esize(v->f32), hipMemcpyDeviceToHost, v->ctx->stream));
+  LGPU_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+  API_END
+}
+
+extern "C" int lgpu_vec_fill(lgpu_vec v, double value) {
+  API_BEGIN
+  LGPU_REQUIRE(v, "NULL vector");
+  vec_fill(v->ptr, v->n, v->f32, value, v->ctx->stream);
+  API_END
+}
+
+extern "C" int lgpu_vec_copy(lgpu_vec src, lgpu_vec dst) {
+  API_BEGIN
+  LGPU_REQUIRE(src && dst && src->n == dst->n, "invalid copy");
+  vec_convert(src->ptr, src->f32, dst->ptr, dst->f32, src->n, src->ctx->stream);
+  API_END
+}
+
+extern "C" int lgpu_vec_dot(lgpu_vec a, lgpu_vec b, double* out) {
+  API_BEGIN
+  LGPU_REQUIRE(a && b && out && a->n == b->n && a->f32 == b->f32, "invalid dot");
+  dot_into(a->ctx, a->ptr, b->ptr, a->f32, a->n, SC_RR);
+  *out = read_scalar(a->ctx, SC_RR);
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// operators
+
+namespace {
+
+void check_box(const lgpu_box* box) {
+  LGPU_REQUIRE(box != nullptr, "box is NULL");
+  for (int k = 0; k < 3; ++k) {
+    LGPU_REQUIRE(box->n[k] > 0, "box extents must be positive");
+    LGPU_REQUIRE(box->len[k] > 0, "box lengths must be positive");
+  }
+  LGPU_REQUIRE(box->n[0] * box->n[1] * box->n[2] < (int64_t(1) << 31), "box has more than 2^31 cells");
+}
+
+}  // namespace
+
+extern "C" int lgpu_stencil_values_create(lgpu_ctx ctx, const lgpu_box* box, const double* coeff,
+                                          lgpu_op* out) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && out && coeff, "NULL argument");
+  check_box(box);
+  auto op = std::make_unique<lgpu_op_s>();
+  op->ctx = ctx;
+  op->box = *box;
+  fill_desc(op.get(), box);
+  op->desc.kind = 0;
+  const int64_t n = op->rows();
+  op->host_coeff.assign(coeff, coeff + n);
+  op->c64.alloc(size_t(n) * sizeof(double));
+  LGPU_HIP_CHECK(hipMemcpy(op->c64.ptr, coeff, size_t(n) * sizeof(double), hipMemcpyHostToDevice));
+  op->desc.c64 = op->c64.as<double>();
+  init_sor_sched(op.get());
+  LGPU_HIP_CHECK(hipDeviceSynchronize());
+  *out = op.release();
+  API_END
+}
+
+extern "C" int lgpu_stencil_lookup_create(lgpu_ctx ctx, const lgpu_box* box,
+                                          const uint8_t* materials, const double mat_coeffs[4],
+                                          lgpu_op* out) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && out && materials && mat_coeffs, "NULL argument");
+  check_box(box);
+  auto op = std::make_unique<lgpu_op_s>();
+  op->ctx = ctx;
+  op->box = *box;
+  fill_desc(op.get(), box);
+  StencilDesc& d = op->desc;
+  d.kind = 1;
+  const int64_t n = op->rows();
+  for (int64_t i = 0; i < n; ++i) LGPU_REQUIRE(materials[i] < 4, "material id out of range");
+  for (int m = 0; m < 4; ++m) d.mc[m] = mat_coeffs[m];
+  // connection tables, lookup-wright.hpp:73-87
+  for (int ax = 0; ax < 3; ++ax)
+    for (int m = 0; m < 16; ++m) {
+      d.tab64[ax][m] = host_hmean(mat_coeffs[m / 4], mat_coeffs[m % 4]) * d.q[ax];
+      d.tab32[ax][m] = host_hmean(float(mat_coeffs[m / 4]), float(mat_coeffs[m % 4])) * float(d.q[ax]);
+    }
+  op->host_mat.assign(materials, materials + n);
+  op->mat.alloc(size_t(n));
+  LGPU_HIP_CHECK(hipMemcpy(op->mat.ptr, materials, size_t(n), hipMemcpyHostToDevice));
+  d.mat = op->mat.as<uint8_t>();
+  init_sor_sched(op.get());
+  LGPU_HIP_CHECK(hipDeviceSynchronize());
+  *out = op.release();
+  API_END
+}
+
+extern "C" int lgpu_csr_create(lgpu_ctx ctx, int64_t n, const int64_t* rowptr, const int32_t* cols,
+                               const double* vals, int dtype, int symmetric, lgpu_op* out) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && out && rowptr && cols && vals && n > 0, "invalid CSR arguments");
+  LGPU_REQUIRE(dtype == LGPU_F64 || dtype == LGPU_F32, "unknown dtype");
+  auto op = std::make_unique<lgpu_op_s>();
+  op->ctx = ctx;
+  op->is_csr = true;
+  setup::RowMatrix& m = op->host_csr;
+  m.n = n;
+  m.symmetric = symmetric != 0;
+  m.f32 = dtype == LGPU_F32;
+  m.rowptr.assign(rowptr, rowptr + n + 1);
+  const int64_t nnz = rowptr[n];
+  LGPU_REQUIRE(rowptr[0] == 0 && nnz >= n, "malformed row pointer");
+  m.cols.assign(cols, cols + nnz);
+  m.vals.resize(size_t(nnz));
+  for (int64_t i = 0; i < n; ++i) {
+    bool diag = false;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      LGPU_REQUIRE(cols[k] >= 0 && cols[k] < n, "column index out of range");
+      if (k > rowptr[i]) LGPU_REQUIRE(cols[k] > cols[k - 1], "columns must be sorted and unique");
+      diag = diag || cols[k] == i;
+      m.vals[size_t(k)] = m.f32 ? double(float(vals[k])) : vals[k];
+    }
+    LGPU_REQUIRE(diag, "every row must store its diagonal");
+  }
+  op->csr.upload(m, m.f32);
+  LGPU_HIP_CHECK(hipDeviceSynchronize());
+  *out = op.release();
+  API_END
+}
+
+extern "C" int lgpu_csr_from_stencil(lgpu_op stencil, int dtype, lgpu_op* out) {
+  API_BEGIN
+  LGPU_REQUIRE(stencil && out && !stencil->is_csr, "needs a stencil operator");
+  LGPU_REQUIRE(dtype == LGPU_F64 || dtype == LGPU_F32, "unknown dtype");
+  setup::RowMatrix rows;
+  stencil_host_rows(stencil, rows);
+  // create-from.hpp:152-220: zero off-diagonals dropped, columns sorted, values cast
+  auto op = std::make_unique<lgpu_op_s>();
+  op->ctx = stencil->ctx;
+  op->is_csr = true;
+  setup::RowMatrix& m = op->host_csr;
+  m.n = rows.n;
+  m.f32 = dtype == LGPU_F32;
+  m.symmetric = true;
+  m.rowptr.assign(1, 0);
+  std::vector<std::pair<int32_t, double>> row;
+  for (int64_t i = 0; i < rows.n; ++i) {
+    row.clear();
+    for (int64_t k = rows.rowptr[i]; k < rows.rowptr[i + 1]; ++k) {
+      if (rows.cols[k] != i && rows.vals[k] == 0.0) continue;
+      row.emplace_back(rows.cols[k], m.f32 ? double(float(rows.vals[k])) : rows.vals[k]);
+    }
+    std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (const auto& [c, v] : row) {
+      m.cols.push_back(c);
+      m.vals.push_back(v);
+    }
+    m.rowptr.push_back(int64_t(m.cols.size()));
+  }
+  op->csr.upload(m, m.f32);
+  LGPU_HIP_CHECK(hipDeviceSynchronize());
+  *out = op.release();
+  API_END
+}
+
+extern "C" int lgpu_op_destroy(lgpu_op op) {
+  API_BEGIN
+  if (op) {
+    (void)hipStreamSynchronize(op->ctx->stream);
+    delete op;
+  }
+  API_END
+}
+
+extern "C" int lgpu_op_rows(lgpu_op op, int64_t* n) {
+  API_BEGIN
+  LGPU_REQUIRE(op && n, "NULL argument");
+  *n = op->rows();
+  API_END
+}
+
+extern "C" int lgpu_op_rhs(lgpu_op op, lgpu_vec b) {
+  API_BEGIN
+  LGPU_REQUIRE(op && !op->is_csr, "the Dirichlet right-hand side needs a stencil operator");
+  check_vec(b, op->rows(), "b");
+  if (b->f32) op->ensure_f32();
+  stencil_rhs(op->desc, b->f32, b->ptr, op->ctx->stream);
+  API_END
+}
+
+extern "C" int lgpu_op_spmv(lgpu_op op, lgpu_vec x, lgpu_vec y) {
+  API_BEGIN
+  LGPU_REQUIRE(op, "NULL operator");
+  check_vec(x, op->rows(), "x");
+  check_vec(y, op->rows(), "y");
+  LGPU_REQUIRE(x->f32 == y->f32, "x and y must share a dtype");
+  LGPU_REQUIRE(x->ptr != y->ptr, "x and y must not alias");
+  op_apply(op, SM_SPMV, x->f32, x->ptr, nullptr, y->ptr, 0.0, nullptr);
+  API_END
+}
+
+extern "C" int lgpu_op_residual(lgpu_op op, lgpu_vec b, lgpu_vec x, lgpu_vec r) {
+  API_BEGIN
+  LGPU_REQUIRE(op, "NULL operator");
+  check_vec(b, op->rows(), "b");
+  check_vec(x, op->rows(), "x");
+  check_vec(r, op->rows(), "r");
+  LGPU_REQUIRE(x->f32 == b->f32 && x->f32 == r->f32, "vectors must share a dtype");
+  LGPU_REQUIRE(x->ptr != r->ptr, "x and r must not alias");
+  op_apply(op, SM_RESID, x->f32, x->ptr, b->ptr, r->ptr, 0.0, nullptr);
+  API_END
+}
+
+extern "C" int lgpu_op_jacobi(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega) {
+  API_BEGIN
+  LGPU_REQUIRE(op, "NULL operator");
+  check_vec(x_out, op->rows(), "x_out");
+  check_vec(b, op->rows(), "b");
+  if (x_in) {
+    check_vec(x_in, op->rows(), "x_in");
+    LGPU_REQUIRE(x_in->f32 == x_out->f32 && x_in->ptr != x_out->ptr, "x_in must match x_out and not alias it");
+  }
+  LGPU_REQUIRE(b->f32 == x_out->f32, "vectors must share a dtype");
+  op_apply(op, SM_JACOBI, x_out->f32, x_in ? x_in->ptr : nullptr, b->ptr, x_out->ptr, omega, nullptr);
+  API_END
+}
+
+extern "C" int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
+                           int forward, int ultra) {
+  API_BEGIN
+  LGPU_REQUIRE(op, "NULL operator");
+  check_vec(x_out, op->rows(), "x_out");
+  check_vec(b, op->rows(), "b");
+  if (x_in) {
+    check_vec(x_in, op->rows(), "x_in");
+    LGPU_REQUIRE(x_in->f32 == x_out->f32 && x_in->ptr != x_out->ptr, "x_in must match x_out and not alias it");
+  }
+  LGPU_REQUIRE(b->f32 == x_out->f32, "vectors must share a dtype");
+  op_sor(op, x_out->f32, x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega, forward != 0, ultra != 0);
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// AMG
+
+extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {
+  if (!p) return;
+  p->max_level_num = 16;
+  p->min_coarsen_size = 32;
+  p->min_coarsen_factor = 1.2;
+  p->strong_threshold = 1.0 / 3.0;
+  p->aggressive_strong_threshold = 1.0 / 3.0;
+  p->min_aggregate_size = 6;
+  p->max_aggregate_size = 9;
+  p->precision = LGPU_F64;
+  p->ultra = 0;
+}
+
+extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out) {
+  API_BEGIN
+  LGPU_REQUIRE(finest && params && out, "NULL argument");
+  LGPU_REQUIRE(params->precision == LGPU_F64 || params->precision == LGPU_F32, "unknown precision");
+  LGPU_REQUIRE(params->max_aggregate_size >= 1 && params->min_aggregate_size >= 1, "invalid aggregate sizes");
+  LGPU_REQUIRE(!finest->is_csr || !finest->csr.f32, "the finest CSR operator must hold f64 values");
+  const auto t0 = std::chrono::steady_clock::now();
+  auto amg = std::make_unique<lgpu_amg_s>();
+  amg->ctx = finest->ctx;
+  amg->finest = finest;
+  amg->params = *params;
+  amg->f32 = params->precision == LGPU_F32;
+  setup::Params sp;
+  sp.max_level_num = params->max_level_num;
+  sp.min_coarsen_size = params->min_coarsen_size;
+  sp.min_coarsen_factor = params->min_coarsen_factor;
+  sp.strong_threshold = params->strong_threshold;
+  sp.aggressive_strong_threshold = params->aggressive_strong_threshold;
+  sp.min_aggregate_size = params->min_aggregate_size;
+  sp.max_aggregate_size = params->max_aggregate_size;
+  sp.coarse_f32 = amg->f32;
+  if (finest->is_csr) {
+    amg->host = setup::build(finest->host_csr, sp);
+  } else {
+    setup::RowMatrix rows;
+    stencil_host_rows(finest, rows);
+    amg->host = setup::build(rows, sp);
+  }
+  LGPU_REQUIRE(amg->host.spd, "coarsest matrix is not positive definite");
+  const setup::Hierarchy& h = amg->host;
+  const int L = int(h.sizes.size());
+  LGPU_REQUIRE(h.sizes.back() <= 4096, "coarsest level has more than 4096 rows");
+  amg->lv.resize(size_t(L));
+  const size_t es = esize(amg->f32);
+  for (int l = 0; l < L; ++l) {
+    AmgLevel& a = amg->lv[size_t(l)];
+    a.n = h.sizes[size_t(l)];
+    if (l > 0) {
+      a.a.upload(h.coarse[size_t(l - 1)], amg->f32);
+      a.rhs.alloc(size_t(a.n) * es);
+      a.sol.alloc(size_t(a.n) * es);
+    }
+    if (l + 1 < L) {
+      const setup::Level& t = h.transfer[size_t(l)];
+      a.agg.alloc(t.agg.size() * sizeof(uint32_t));
+      LGPU_HIP_CHECK(hipMemcpy(a.agg.ptr, t.agg.data(), t.agg.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      a.c2f_ptr.alloc(t.c2f_ptr.size() * sizeof(int64_t));
+      LGPU_HIP_CHECK(hipMemcpy(a.c2f_ptr.ptr, t.c2f_ptr.data(), t.c2f_ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+      a.c2f_idx.alloc(t.c2f_idx.size() * sizeof(int32_t));
+      LGPU_HIP_CHECK(hipMemcpy(a.c2f_idx.ptr, t.c2f_idx.data(), t.c2f_idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      a.tmp.alloc(size_t(a.n) * es);
+      a.aux.alloc(size_t(a.n) * es);
+    }
+  }
+  const int64_t nc = h.sizes.back();
+  amg->chol.alloc(size_t(nc * nc) * sizeof(double));
+  LGPU_HIP_CHECK(hipMemcpy(amg->chol.ptr, h.chol_dense.data(), size_t(nc * nc) * sizeof(double), hipMemcpyHostToDevice));
+  if (amg->f32) {
+    finest->ensure_f32();
+    amg->r32.alloc(size_t(h.sizes[0]) * sizeof(float));
+    amg->z32.alloc(size_t(h.sizes[0]) * sizeof(float));
+  }
+  LGPU_HIP_CHECK(hipDeviceSynchronize());
+  amg->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out = amg.release();
+  API_END
+}
+
+extern "C" int lgpu_amg_destroy(lgpu_amg amg) {
+  API_BEGIN
+  if (amg) {
+    (void)hipStreamSynchronize(amg->ctx->stream);
+    delete amg;
+  }
+  API_END
+}
+
+extern "C" int lgpu_amg_level_num(lgpu_amg amg, int* levels) {
+  API_BEGIN
+  LGPU_REQUIRE(amg && levels, "NULL argument");
+  *levels = int(amg->lv.size());
+  API_END
+}
+
+extern "C" int lgpu_amg_level_size(lgpu_amg amg, int level, int64_t* n) {
+  API_BEGIN
+  LGPU_REQUIRE(amg && n && level >= 0 && level < int(amg->lv.size()), "invalid level");
+  *n = amg->lv[size_t(level)].n;
+  API_END
+}
+
+extern "C" int lgpu_amg_aggregates(lgpu_amg amg, int level, uint32_t* host_map) {
+  API_BEGIN
+  LGPU_REQUIRE(amg && host_map && level >= 0 && level + 1 < int(amg->lv.size()), "invalid level");
+  const auto& agg = amg->host.transfer[size_t(level)].agg;
+  std::memcpy(host_map, agg.data(), agg.size() * sizeof(uint32_t));
+  API_END
+}
+
+extern "C" int lgpu_amg_matrix(lgpu_amg amg, int level, int64_t* nnz, int64_t* rowptr, int32_t* cols,
+                               double* vals) {
+  API_BEGIN
+  LGPU_REQUIRE(amg && nnz && level >= 1 && level < int(amg->lv.size()), "invalid level");
+  const setup::RowMatrix& m = amg->host.coarse[size_t(level - 1)];
+  *nnz = int64_t(m.cols.size());
+  if (rowptr) std::memcpy(rowptr, m.rowptr.data(), m.rowptr.size() * sizeof(int64_t));
+  if (cols) std::memcpy(cols, m.cols.data(), m.cols.size() * sizeof(int32_t));
+  if (vals) std::memcpy(vals, m.vals.data(), m.vals.size() * sizeof(double));
+  API_END
+}
+
+extern "C" int lgpu_amg_setup_seconds(lgpu_amg amg, double* seconds) {
+  API_BEGIN
+  LGPU_REQUIRE(amg && seconds, "NULL argument");
+  *seconds = amg->setup_seconds;
+  API_END
+}
+
+extern "C" int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z) {
+  API_BEGIN
+  LGPU_REQUIRE(amg, "NULL AMG");
+  check_vec(r, amg->lv[0].n, "r");
+  check_vec(z, amg->lv[0].n, "z");
+  LGPU_REQUIRE(r->ptr != z->ptr, "r and z must not alias");
+  if (r->f32 || z->f32) {
+    LGPU_REQUIRE(r->f32 && z->f32 && amg->f32, "f32 vectors need an f32 hierarchy");
+    vcycle(amg, r->ptr, z->ptr);
+  } else {
+    precondition(amg, static_cast<const double*>(r->ptr), static_cast<double*>(z->ptr));
+  }
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// Krylov solvers
+
+namespace {
+
+struct Work {
+  std::vector<DevBuf> v;
+  Work(int k, int64_t n) {
+    for (int i = 0; i < k; ++i) v.emplace_back(size_t(n) * sizeof(double));
+  }
+  double* operator[](int i) { return v[size_t(i)].as<double>(); }
+};
+
+bool rere(int iteration, int every) { return every > 0 && iteration > 0 && iteration % every == 0; }
+
+int finish(lgpu_op op, lgpu_amg amg, int status) {
+  if (op_sync_error(op) || (amg && amg_sync_error(amg))) {
+    set_error("a wavefront kernel's bounded wait expired");
+    return LGPU_SYNC_TIMEOUT;
+  }
+  return status;
+}
+
+}  // namespace
+
+extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
+                             const lgpu_krylov_params* params, int* iterations, double* res_hist) {
+  API_BEGIN
+  LGPU_REQUIRE(op && params, "NULL argument");
+  const int64_t n = op->rows();
+  check_vec(b, n, "b");
+  check_vec(x, n, "x");
+  LGPU_REQUIRE(!b->f32 && !x->f32, "the Krylov vectors are f64");
+  LGPU_REQUIRE(!precon || precon->finest == op, "the preconditioner belongs to another operator");
+  lgpu_ctx ctx = op->ctx;
+  hipStream_t st = ctx->stream;
+  double* scal = ctx->scal.as<double>();
+  Work w(3, n);
+  double *r = w[0], *p = w[1], *q = w[2];
+  double* xv = static_cast<double*>(x->ptr);
+  const double* bv = static_cast<const double*>(b->ptr);
+  // conjugate-gradients/solver.hpp:58-71
+  residual_norm(op, bv, xv, r, SC_RR);
+  int cur = SC_RHO0, nxt = SC_RHO1;
+  if (precon) {
+    precondition(precon, r, p);
+    dot_into(ctx, p, r, false, n, cur);
+  } else {
+    LGPU_HIP_CHECK(hipMemcpyAsync(p, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st));
+    LGPU_HIP_CHECK(hipMemcpyAsync(scal + cur, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  const double r0 = std::sqrt(read_scalar(ctx, SC_RR));
+  if (res_hist) res_hist[0] = r0;
+  int it = 0, status = LGPU_NOT_CONVERGED;
+  if (!std::isfinite(r0)) status = LGPU_NONFINITE;
+  else if (r0 == 0.0) status = LGPU_OK;
+  while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
+    // conjugate-gradients/solver.hpp:75-115
+    const int64_t np = op_partials(op);
+    double* part = ctx->partials_for(np);
+    op_apply(op, SM_SPMV_DOT, false, p, nullptr, q, 0.0, part);
+    reduce_partials(part, np, scal + SC_PQ, st);
+    if (rere(it, params->rere_every)) {
+      axpy_scal(xv, p, scal, cur, SC_PQ, 1.0, n, st);
+      residual_norm(op, bv, xv, r, SC_RR);
+    } else {
+      const int64_t nd = dot_partials(n);
+      double* pd = ctx->partials_for(nd);
+      cg_update(xv, r, p, q, scal, cur, SC_PQ, n, pd, st);
+      reduce_partials(pd, nd, scal + SC_RR, st);
+    }
+    if (precon) {
+      precondition(precon, r, q);
+      dot_into(ctx, r, q, false, n, nxt);
+      xpby_scal(p, q, scal, nxt, cur, n, st);
+    } else {
+      LGPU_HIP_CHECK(hipMemcpyAsync(scal + nxt, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st));
+      xpby_scal(p, r, scal, nxt, cur, n, st);
+    }
+    std::swap(cur, nxt);
+    ++it;
+    const double rn = std::sqrt(read_scalar(ctx, SC_RR));
+    if (res_hist) res_hist[it] = rn;
+    if (!std::isfinite(rn)) status = LGPU_NONFINITE;
+    else if (rn <= params->rel_tol * r0) status = LGPU_OK;
+    else if (read_scalar(ctx, cur) == 0.0) status = LGPU_BREAKDOWN;
+  }
+  if (iterations) *iterations = it;
+  return finish(op, precon, status);
+  API_END
+}
+
+extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
+                                   const lgpu_krylov_params* params, int* iterations,
+                                   double* res_hist) {
+  API_BEGIN
+  LGPU_REQUIRE(op && params, "NULL argument");
+  const int64_t n = op->rows();
+  check_vec(b, n, "b");
+  check_vec(x, n, "x");
+  LGPU_REQUIRE(!b->f32 && !x->f32, "the Krylov vectors are f64");
+  LGPU_REQUIRE(!precon || precon->finest == op, "the preconditioner belongs to another operator");
+  lgpu_ctx ctx = op->ctx;
+  hipStream_t st = ctx->stream;
+  double* scal = ctx->scal.as<double>();
+  Work w(6, n);
+  double *r = w[0], *r0 = w[1], *p = w[2], *v = w[3], *t = w[4], *px = w[5];
+  double* xv = static_cast<double*>(x->ptr);
+  const double* bv = static_cast<const double*>(b->ptr);
+  // bicgstab/solver.hpp:64-68
+  residual_norm(op, bv, xv, r, SC_RR);
+  LGPU_HIP_CHECK(hipMemcpyAsync(r0, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st));
+  LGPU_HIP_CHECK(hipMemcpyAsync(p, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st));
+  int cur = SC_RHO0, nxt = SC_RHO1;
+  LGPU_HIP_CHECK(hipMemcpyAsync(scal + cur, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st));
+  const double rn0 = std::sqrt(read_scalar(ctx, SC_RR));
+  if (res_hist) res_hist[0] = rn0;
+  int it = 0, status = LGPU_NOT_CONVERGED;
+  if (!std::isfinite(rn0)) status = LGPU_NONFINITE;
+  else if (rn0 == 0.0) status = LGPU_OK;
+  const int64_t nd = dot_partials(n);
+  while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
+    // bicgstab/solver.hpp:71-151
+    double* y = p;
+    if (precon) {
+      precondition(precon, p, px);
+      y = px;
+    }
+    op_apply(op, SM_SPMV, false, y, nullptr, v, 0.0, nullptr);
+    dot_into(ctx, r0, v, false, n, SC_R0V);
+    axpy_scal(xv, y, scal, cur, SC_R0V, 1.0, n, st);
+    double* pd = ctx->partials_for(nd);
+    bicg_r_update(r, v, scal, cur, SC_R0V, SC_ALPHA, n, pd, st);
+    reduce_partials(pd, nd, scal + SC_RR, st);
+    double* z = r;
+    if (precon) {
+      precondition(precon, r, px);
+      z = px;
+    }
+    op_apply(op, SM_SPMV, false, z, nullptr, t, 0.0, nullptr);
+    pd = ctx->partials_for(nd);
+    vec_dot2(t, r, n, pd, st);
+    reduce_partials2(pd, nd, scal + SC_TR, scal + SC_TT, st);
+    axpy_scal(xv, z, scal, SC_TR, SC_TT, 1.0, n, st);
+    if (rere(it, params->rere_every)) {
+      // omega still has to be stored for the direction update
+      pd = ctx->partials_for(nd);
+      bicg_r_update(t, t, scal, SC_TR, SC_TT, SC_OMEGA, 0, pd, st);
+      residual_norm(op, bv, xv, r, SC_RR);
+    } else {
+      pd = ctx->partials_for(nd);
+      bicg_r_update(r, t, scal, SC_TR, SC_TT, SC_OMEGA, n, pd, st);
+    }
+    pd = ctx->partials_for(nd);
+    vec_dot2(r, r0, n, pd, st);
+    reduce_partials2(pd, nd, scal + nxt, scal + SC_RR, st);
+    // dot2 gives (r.r0, r.r); the second is ||r||^2
+    bicg_p_update(p, r, v, scal, nxt, cur, n, st);
+    std::swap(cur, nxt);
+    ++it;
+    const double rn = std::sqrt(read_scalar(ctx, SC_RR));
+    if (res_hist) res_hist[it] = rn;
+    if (!std::isfinite(rn)) status = LGPU_NONFINITE;
+    else if (rn <= params->rel_tol * rn0) status = LGPU_OK;
+    else if (read_scalar(ctx, cur) == 0.0) status = LGPU_BREAKDOWN;
+  }
+  if (iterations) *iterations = it;
+  return finish(op, precon, status);
+  API_END
+}

@@ -1,0 +1,140 @@
+// Device-side helpers: stencil row evaluation, reductions, XCD-aware tile mapping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace lgpu {
+
+// Harmonic mean H(a,b) = (ab != 0) ? (ab + ab) / (a + b) : 0,
+// math/harmonic-mean.hpp:14-25.
+template <typename T>
+__device__ __forceinline__ T hmean(T a, T b) {
+  const T p = a * b;
+  return p != T(0) ? (p + p) / (a + b) : T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ const T* coeff_ptr(const StencilDesc& s);
+template <>
+__device__ __forceinline__ const double* coeff_ptr<double>(const StencilDesc& s) { return s.c64; }
+template <>
+__device__ __forceinline__ const float* coeff_ptr<float>(const StencilDesc& s) { return s.c32; }
+
+// Cell information of the two valuators: a coefficient (values valuator) or a
+// material id (lookup valuator).
+template <typename T, int K>
+struct CellInfo;
+template <typename T>
+struct CellInfo<T, 0> {
+  using Raw = T;
+  __device__ __forceinline__ static Raw load(const StencilDesc& s, int64_t i) {
+    return coeff_ptr<T>(s)[i];
+  }
+  __device__ __forceinline__ static T value(const StencilDesc&, Raw r) { return r; }
+  // connection value H(c_i, c_j) * q_d
+  // (tensor/dynamic/stencil/valuator/stationary-diffusion-values.hpp:131-135)
+  __device__ __forceinline__ static T conn(const StencilDesc&, Raw a, Raw b, T qd, int) {
+    return hmean(a, b) * qd;
+  }
+};
+template <typename T>
+struct CellInfo<T, 1> {
+  using Raw = uint8_t;
+  __device__ __forceinline__ static Raw load(const StencilDesc& s, int64_t i) { return s.mat[i]; }
+  __device__ __forceinline__ static T value(const StencilDesc& s, Raw r) { return T(s.mc[r]); }
+  // table gather T_d[m_i * 4 + m_j]
+  // (tensor/dynamic/stencil/valuator/stationary-diffusion-lookup.hpp:129-136)
+  __device__ __forceinline__ static T conn(const StencilDesc& s, Raw a, Raw b, T, int d);
+};
+template <>
+__device__ __forceinline__ double CellInfo<double, 1>::conn(const StencilDesc& s, uint8_t a,
+                                                           uint8_t b, double, int d) {
+  return s.tab64[d][a * 4 + b];
+}
+template <>
+__device__ __forceinline__ float CellInfo<float, 1>::conn(const StencilDesc& s, uint8_t a,
+                                                         uint8_t b, float, int d) {
+  return s.tab32[d][a * 4 + b];
+}
+
+// Face weights and diagonal of one row in the order of the reference's valued,
+// unordered iterate (tensor/dynamic/stencil/matrix/algorithm.hpp:91-227):
+// diag = 0 (+2 q0 c at x=0) (+2 q0 c at x=nx-1) + w(-x) + w(-y) + w(-z) + w(+z)
+// + w(+y) + w(+x); ZeroToOne replaces a zero diagonal by 1.  Absent faces have
+// weight +0, which leaves every sum unchanged.
+template <typename T>
+struct Row {
+  T wmx, wmy, wmz, wpz, wpy, wpx, d;
+};
+
+template <typename T, int K>
+__device__ __forceinline__ Row<T> eval_row(const StencilDesc& s, int64_t x, int64_t y, int64_t z,
+                                           typename CellInfo<T, K>::Raw cc,
+                                           typename CellInfo<T, K>::Raw cmx,
+                                           typename CellInfo<T, K>::Raw cmy,
+                                           typename CellInfo<T, K>::Raw cmz,
+                                           typename CellInfo<T, K>::Raw cpz,
+                                           typename CellInfo<T, K>::Raw cpy,
+                                           typename CellInfo<T, K>::Raw cpx) {
+  using CI = CellInfo<T, K>;
+  const T q0 = T(s.q[0]), q1 = T(s.q[1]), q2 = T(s.q[2]);
+  const T ci = CI::value(s, cc);
+  Row<T> r;
+  T d = T(0);
+  const T b = (T(2) * q0) * ci;
+  if (x == 0) d += b;
+  if (x + 1 == s.nx) d += b;
+  r.wmx = x > 0 ? CI::conn(s, cc, cmx, q0, 0) : T(0);
+  r.wmy = y > 0 ? CI::conn(s, cc, cmy, q1, 1) : T(0);
+  r.wmz = z > 0 ? CI::conn(s, cc, cmz, q2, 2) : T(0);
+  r.wpz = z + 1 < s.nz ? CI::conn(s, cc, cpz, q2, 2) : T(0);
+  r.wpy = y + 1 < s.ny ? CI::conn(s, cc, cpy, q1, 1) : T(0);
+  r.wpx = x + 1 < s.nx ? CI::conn(s, cc, cpx, q0, 0) : T(0);
+  d += r.wmx;
+  d += r.wmy;
+  d += r.wmz;
+  d += r.wpz;
+  d += r.wpy;
+  d += r.wpx;
+  if (s.zero_to_one && d == T(0)) d = T(1);
+  r.d = d;
+  return r;
+}
+
+// Wave-level sum of a double (64 lanes).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block sum; every thread passes its value, thread 0 receives the total.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[NT / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) t += red[w];
+  }
+  return t;
+}
+
+// Map the hardware dispatch index to a logical tile index such that each of
+// the 8 XCDs (which receive workgroups round robin) processes a contiguous
+// range of logical tiles and so shares halo data through its own L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t hw, int64_t total) {
+  constexpr int64_t kXcd = 8;
+  const int64_t full = total - total % kXcd;
+  if (hw >= full) return hw;
+  const int64_t per = full / kXcd;
+  return (hw % kXcd) * per + hw / kXcd;
+}
+
+}  // namespace lgpu

@@ -1,0 +1,189 @@
+// Internal declarations shared by the HIP kernels and the C ABI of lineal_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lineal_amd.h"
+
+namespace lgpu {
+
+// ---------------------------------------------------------------------------
+// error reporting
+
+void set_error(const std::string& msg);
+
+struct Error {
+  int status;
+  std::string msg;
+};
+
+#define LGPU_HIP_CHECK(expr)                                                             \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::lgpu::Error{e_ == hipErrorOutOfMemory ? LGPU_OUT_OF_MEMORY : LGPU_HIP_ERROR, \
+                          std::string(#expr) + ": " + hipGetErrorString(e_)};            \
+  } while (0)
+
+#define LGPU_REQUIRE(cond, msg)                                  \
+  do {                                                           \
+    if (!(cond)) throw ::lgpu::Error{LGPU_INVALID_ARG, (msg)};   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffers
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) { alloc(b); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) b = 16;
+    LGPU_HIP_CHECK(hipMalloc(&ptr, b));
+    bytes = b;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ---------------------------------------------------------------------------
+// stencil description passed to kernels by value
+
+struct StencilDesc {
+  int64_t nx, ny, nz;
+  double q[3];           // axis quotients, discretization/box-equidistant.hpp:62-79
+  int kind;              // 0: per-cell coefficients, 1: u8 material lookup
+  int zero_to_one;
+  double p_start, p_end;
+  const double* c64;     // kind 0
+  const float* c32;      // kind 0, f32 copy of c64
+  const uint8_t* mat;    // kind 1
+  double mc[4];          // kind 1 material coefficients
+  double tab64[3][16];   // kind 1 connection tables H(C[m],C[m'])*q_d in f64
+  float tab32[3][16];    //                               in f32
+};
+
+// kernel modes of the matrix-free stencil pass
+enum StencilMode : int {
+  SM_SPMV = 0,       // y = A x
+  SM_RESID = 1,      // y = b - A x
+  SM_SPMV_DOT = 2,   // y = A x,     partial += x_i y_i
+  SM_RESID_NORM = 3, // y = b - A x, partial += y_i^2
+  SM_JACOBI = 4,     // y = Jacobi(x -> y), b rhs, scalar omega
+};
+
+// CSR on the device
+struct CsrDesc {
+  int64_t n;
+  const int64_t* rowptr;
+  const int32_t* cols;
+  const void* vals;    // f64 or f32
+  int f32;
+};
+
+// ---------------------------------------------------------------------------
+// kernel launchers (stream ordered)
+
+// number of partial sums written by stencil reductions
+int64_t stencil_partials(const StencilDesc& s);
+void stencil_apply(const StencilDesc& s, int mode, bool f32, const void* x, const void* b,
+                   void* y, double omega, double* partials, hipStream_t st);
+void stencil_rhs(const StencilDesc& s, bool f32, void* b, hipStream_t st);
+
+// lexicographic SOR sweep on the stencil (tile-column wavefront)
+struct SorSched {
+  int* state = nullptr;        // [0] ticket, [1] error, [2..] column progress
+  const uint32_t* order = nullptr;  // ticket -> column id (forward topological order)
+  int64_t ncols = 0;
+};
+size_t sor_sched_ints(const StencilDesc& s);
+void sor_sched_order(const StencilDesc& s, std::vector<uint32_t>& order);
+void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, const void* b,
+                 double omega, bool forward, bool ultra, const SorSched& sched, hipStream_t st);
+
+// CSR kernels
+int64_t csr_partials(const CsrDesc& a);
+void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y, double omega,
+               double* partials, hipStream_t st);
+void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+             bool forward, bool ultra, int* sched, int* flags, int epoch, hipStream_t st);
+
+// vector kernels
+void vec_fill(void* x, int64_t n, bool f32, double v, hipStream_t st);
+void vec_fill_int(int* x, int64_t n, int v, hipStream_t st);
+void vec_convert(const void* src, bool src_f32, void* dst, bool dst_f32, int64_t n, hipStream_t st);
+int64_t dot_partials(int64_t n);
+void vec_dot(const void* a, const void* b, bool f32, int64_t n, double* partials, hipStream_t st);
+// out[slot] = sum of partials[0..np)  (deterministic order)
+void reduce_partials(const double* partials, int64_t np, double* out, hipStream_t st);
+
+// AMG transfer: coarse[c] = sum_{k in c2f[c]} fine[idx[k]]
+void restrict_sum(const void* fine, bool f32, const int64_t* ptr, const int32_t* idx, int64_t nc,
+                  void* coarse, hipStream_t st);
+// out[v] = x[v] + (agg[v] aggregate ? xc[agg[v]] : 0) * inv_relax
+void prolong_add(const void* x, const void* xc, bool f32, const uint32_t* agg, int64_t n,
+                 double inv_relax, void* out, hipStream_t st);
+// dense lower-triangular Cholesky solve, n <= 4096, f64 factor; rhs/x in f64 or f32
+void dense_cholesky_solve(const double* L, int64_t n, const void* b, void* x, bool f32,
+                          hipStream_t st);
+
+// Krylov helper kernels operating on device scalars
+// scal layout indices
+enum : int {
+  SC_RR = 0,      // ||r||^2
+  SC_PQ = 1,      // <p, A p>
+  SC_RHO0 = 2,    // rho ping
+  SC_RHO1 = 3,    // rho pong
+  SC_R0V = 4,     // <r0, v>
+  SC_TR = 5,      // <t, r>
+  SC_TT = 6,      // <t, t>
+  SC_ALPHA = 7,
+  SC_OMEGA = 8,
+  SC_COUNT = 16,
+};
+// x += (num/den) p, r -= (num/den) q, partial += r^2
+void cg_update(double* x, double* r, const double* p, const double* q, const double* scal,
+               int num, int den, int64_t n, double* partials, hipStream_t st);
+// x += (num/den) p only
+void axpy_scal(double* x, const double* p, const double* scal, int num, int den, double sign,
+               int64_t n, hipStream_t st);
+// p = z + (num/den) p
+void xpby_scal(double* p, const double* z, const double* scal, int num, int den, int64_t n,
+               hipStream_t st);
+// BiCGStab: p = r + ((rho1/rho0)*(alpha/omega)) (p - omega v)
+void bicg_p_update(double* p, const double* r, const double* v, const double* scal, int rho_new,
+                   int rho_old, int64_t n, hipStream_t st);
+// r = r - a v (a = scal[num]/scal[den] stored into scal[out]), partial += r^2
+void bicg_r_update(double* r, const double* v, double* scal, int num, int den, int out,
+                   int64_t n, double* partials, hipStream_t st);
+// dual dot: partial0 += a.b, partial1 += a.a   (partials holds 2*np)
+void vec_dot2(const double* a, const double* b, int64_t n, double* partials, hipStream_t st);
+void reduce_partials2(const double* partials, int64_t np, double* out0, double* out1,
+                      hipStream_t st);
+
+}  // namespace lgpu

@@ -1,0 +1,288 @@
+// Vector kernels: fills, conversions, deterministic two-pass reductions, the
+// fused Krylov updates, AMG transfer operators and the coarsest direct solve.
+#include "device.hpp"
+
+namespace lgpu {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int64_t kMaxBlocks = 2048;
+
+int64_t grid_for(int64_t n) {
+  const int64_t b = (n + NT - 1) / NT;
+  return b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b);
+}
+
+template <typename T>
+__global__ void fill_kernel(T* x, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT)
+    x[i] = v;
+}
+
+template <typename S, typename D>
+__global__ void convert_kernel(const S* s, D* d, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT)
+    d[i] = D(s[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void dot_kernel(const T* a, const T* b, int64_t n, double* part) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT)
+    s += double(a[i]) * double(b[i]);
+  const double t = block_sum<NT>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(NT) void dot2_kernel(const double* a, const double* b, int64_t n,
+                                                  double* part) {
+  double s0 = 0, s1 = 0;
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT) {
+    const double ai = a[i];
+    s0 += ai * b[i];
+    s1 += ai * ai;
+  }
+  const double t0 = block_sum<NT>(s0);
+  __syncthreads();
+  const double t1 = block_sum<NT>(s1);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = t0;
+    part[gridDim.x + blockIdx.x] = t1;
+  }
+}
+
+// single block: fixed-order reduction of np partials
+__global__ __launch_bounds__(NT) void reduce_kernel(const double* part, int64_t np, double* out,
+                                                    const double* part2, double* out2) {
+  double s = 0;
+  for (int64_t i = threadIdx.x; i < np; i += NT) s += part[i];
+  const double t = block_sum<NT>(s);
+  if (threadIdx.x == 0) *out = t;
+  if (part2) {
+    __syncthreads();
+    double s2 = 0;
+    for (int64_t i = threadIdx.x; i < np; i += NT) s2 += part2[i];
+    const double t2 = block_sum<NT>(s2);
+    if (threadIdx.x == 0) *out2 = t2;
+  }
+}
+
+// CG: alpha = scal[num] / scal[den]; x += alpha p; r -= alpha q; partial += r^2
+// (conjugate-gradients/solver.hpp:84-95: sol + dir*alpha, res - prod*alpha)
+__global__ __launch_bounds__(NT) void cg_update_kernel(double* x, double* r, const double* p,
+                                                       const double* q, const double* scal,
+                                                       int num, int den, int64_t n,
+                                                       double* part) {
+  const double alpha = scal[num] / scal[den];
+  double s = 0;
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT) {
+    x[i] = x[i] + p[i] * alpha;
+    const double ri = r[i] - q[i] * alpha;
+    r[i] = ri;
+    s += ri * ri;
+  }
+  const double t = block_sum<NT>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void axpy_scal_kernel(double* x, const double* p, const double* scal, int num, int den,
+                                 double sign, int64_t n) {
+  const double alpha = scal[num] / scal[den];
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT) {
+    if (sign > 0) x[i] = x[i] + p[i] * alpha;
+    else x[i] = x[i] - p[i] * alpha;
+  }
+}
+
+// p = z + p * beta, beta = scal[num] / scal[den] (conjugate-gradients/solver.hpp:106-107)
+__global__ void xpby_scal_kernel(double* p, const double* z, const double* scal, int num, int den,
+                                 int64_t n) {
+  const double beta = scal[num] / scal[den];
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT)
+    p[i] = z[i] + p[i] * beta;
+}
+
+// bicgstab/solver.hpp:145-146: beta = (rho/old_rho) * (alpha/omega); p = r + (p - v*omega)*beta
+__global__ void bicg_p_kernel(double* p, const double* r, const double* v, const double* scal,
+                              int rho_new, int rho_old, int64_t n) {
+  const double omega = scal[SC_OMEGA];
+  const double beta = (scal[rho_new] / scal[rho_old]) * (scal[SC_ALPHA] / omega);
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT)
+    p[i] = r[i] + (p[i] - v[i] * omega) * beta;
+}
+
+// r = r - v * a with a = scal[num]/scal[den]; the thread of block 0 stores a into scal[out]
+__global__ __launch_bounds__(NT) void bicg_r_kernel(double* r, const double* v, double* scal,
+                                                    int num, int den, int out, int64_t n,
+                                                    double* part) {
+  const double a = scal[num] / scal[den];
+  double s = 0;
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT) {
+    const double ri = r[i] - v[i] * a;
+    r[i] = ri;
+    s += ri * ri;
+  }
+  const double t = block_sum<NT>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void store_ratio_kernel(double* scal, int num, int den, int out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) scal[out] = scal[num] / scal[den];
+}
+
+// transport/vector-shared.hpp:49-56: sum in coarse->fine list order
+template <typename T>
+__global__ void restrict_kernel(const T* fine, const int64_t* ptr, const int32_t* idx, int64_t nc,
+                                T* coarse) {
+  for (int64_t c = blockIdx.x * int64_t(NT) + threadIdx.x; c < nc; c += int64_t(gridDim.x) * NT) {
+    T s = T(0);
+    for (int64_t k = ptr[c]; k < ptr[c + 1]; ++k) s += fine[idx[k]];
+    coarse[c] = s;
+  }
+}
+
+// multigrid/solver.hpp:125-137 with transport/vector-shared.hpp:121-125
+template <typename T>
+__global__ void prolong_kernel(const T* x, const T* xc, const uint32_t* agg, int64_t n,
+                               T inv_relax, T* out) {
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT) {
+    const uint32_t g = agg[i];
+    const T refined = g < 0xFFFFFFFEu ? xc[g] : T(0);
+    out[i] = x[i] + refined * inv_relax;
+  }
+}
+
+// Coarsest solve with a dense copy of the lower Cholesky factor, one block.
+// Forward substitution is evaluated column by column: for every row i the
+// updates s_i -= L_ij aux_j happen for j = 0, 1, ... in the same order as the
+// row-oriented substitution of direct/substitution.hpp:27-52 (zero entries of
+// the dense copy leave s_i unchanged); the transposed backward substitution is
+// column oriented in the reference as well (direct/substitution.hpp:82-113).
+template <typename T>
+__global__ __launch_bounds__(NT) void chol_solve_kernel(const double* L, int64_t n, const T* b,
+                                                        T* x) {
+  extern __shared__ double sh[];
+  double* s = sh;  // n entries
+  for (int64_t i = threadIdx.x; i < n; i += NT) s[i] = double(b[i]);
+  __syncthreads();
+  for (int64_t j = 0; j < n; ++j) {
+    const double aj = s[j] / L[j * n + j];
+    __syncthreads();
+    if (threadIdx.x == 0) s[j] = aj;
+    for (int64_t i = j + 1 + threadIdx.x; i < n; i += NT) s[i] = s[i] - L[i * n + j] * aj;
+    __syncthreads();
+  }
+  // s holds aux; backward: sol = aux; for j = n-1..0: sol_j /= L_jj; sol_i -= L_ji sol_j (i < j)
+  for (int64_t j = n - 1; j >= 0; --j) {
+    const double sj = s[j] / L[j * n + j];
+    __syncthreads();
+    if (threadIdx.x == 0) s[j] = sj;
+    for (int64_t i = threadIdx.x; i < j; i += NT) s[i] = s[i] - L[j * n + i] * sj;
+    __syncthreads();
+  }
+  for (int64_t i = threadIdx.x; i < n; i += NT) x[i] = T(s[i]);
+}
+
+}  // namespace
+
+void vec_fill(void* x, int64_t n, bool f32, double v, hipStream_t st) {
+  if (f32) hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(NT), 0, st, static_cast<float*>(x), n, float(v));
+  else hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n)), dim3(NT), 0, st, static_cast<double*>(x), n, v);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void vec_fill_int(int* x, int64_t n, int v, hipStream_t st) {
+  hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(n)), dim3(NT), 0, st, x, n, v);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void vec_convert(const void* src, bool src_f32, void* dst, bool dst_f32, int64_t n, hipStream_t st) {
+  const dim3 g(unsigned(grid_for(n))), b(NT);
+  if (src_f32 && dst_f32) hipLaunchKernelGGL((convert_kernel<float, float>), g, b, 0, st, static_cast<const float*>(src), static_cast<float*>(dst), n);
+  else if (src_f32) hipLaunchKernelGGL((convert_kernel<float, double>), g, b, 0, st, static_cast<const float*>(src), static_cast<double*>(dst), n);
+  else if (dst_f32) hipLaunchKernelGGL((convert_kernel<double, float>), g, b, 0, st, static_cast<const double*>(src), static_cast<float*>(dst), n);
+  else hipLaunchKernelGGL((convert_kernel<double, double>), g, b, 0, st, static_cast<const double*>(src), static_cast<double*>(dst), n);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+int64_t dot_partials(int64_t n) { return grid_for(n); }
+
+void vec_dot(const void* a, const void* b, bool f32, int64_t n, double* partials, hipStream_t st) {
+  if (f32) hipLaunchKernelGGL(dot_kernel<float>, dim3(grid_for(n)), dim3(NT), 0, st, static_cast<const float*>(a), static_cast<const float*>(b), n, partials);
+  else hipLaunchKernelGGL(dot_kernel<double>, dim3(grid_for(n)), dim3(NT), 0, st, static_cast<const double*>(a), static_cast<const double*>(b), n, partials);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void vec_dot2(const double* a, const double* b, int64_t n, double* partials, hipStream_t st) {
+  hipLaunchKernelGGL(dot2_kernel, dim3(grid_for(n)), dim3(NT), 0, st, a, b, n, partials);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void reduce_partials(const double* partials, int64_t np, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(NT), 0, st, partials, np, out,
+                     static_cast<const double*>(nullptr), static_cast<double*>(nullptr));
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void reduce_partials2(const double* partials, int64_t np, double* out0, double* out1,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(NT), 0, st, partials, np, out0, partials + np, out1);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void cg_update(double* x, double* r, const double* p, const double* q, const double* scal, int num,
+               int den, int64_t n, double* partials, hipStream_t st) {
+  hipLaunchKernelGGL(cg_update_kernel, dim3(grid_for(n)), dim3(NT), 0, st, x, r, p, q, scal, num, den, n, partials);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void axpy_scal(double* x, const double* p, const double* scal, int num, int den, double sign,
+               int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(axpy_scal_kernel, dim3(grid_for(n)), dim3(NT), 0, st, x, p, scal, num, den, sign, n);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void xpby_scal(double* p, const double* z, const double* scal, int num, int den, int64_t n,
+               hipStream_t st) {
+  hipLaunchKernelGGL(xpby_scal_kernel, dim3(grid_for(n)), dim3(NT), 0, st, p, z, scal, num, den, n);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void bicg_p_update(double* p, const double* r, const double* v, const double* scal, int rho_new,
+                   int rho_old, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(bicg_p_kernel, dim3(grid_for(n)), dim3(NT), 0, st, p, r, v, scal, rho_new, rho_old, n);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void bicg_r_update(double* r, const double* v, double* scal, int num, int den, int out, int64_t n,
+                   double* partials, hipStream_t st) {
+  hipLaunchKernelGGL(store_ratio_kernel, dim3(1), dim3(64), 0, st, scal, num, den, out);
+  hipLaunchKernelGGL(bicg_r_kernel, dim3(grid_for(n)), dim3(NT), 0, st, r, v, scal, num, den, out, n, partials);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void restrict_sum(const void* fine, bool f32, const int64_t* ptr, const int32_t* idx, int64_t nc,
+                  void* coarse, hipStream_t st) {
+  if (f32) hipLaunchKernelGGL(restrict_kernel<float>, dim3(grid_for(nc)), dim3(NT), 0, st, static_cast<const float*>(fine), ptr, idx, nc, static_cast<float*>(coarse));
+  else hipLaunchKernelGGL(restrict_kernel<double>, dim3(grid_for(nc)), dim3(NT), 0, st, static_cast<const double*>(fine), ptr, idx, nc, static_cast<double*>(coarse));
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void prolong_add(const void* x, const void* xc, bool f32, const uint32_t* agg, int64_t n,
+                 double inv_relax, void* out, hipStream_t st) {
+  if (f32) hipLaunchKernelGGL(prolong_kernel<float>, dim3(grid_for(n)), dim3(NT), 0, st, static_cast<const float*>(x), static_cast<const float*>(xc), agg, n, float(inv_relax), static_cast<float*>(out));
+  else hipLaunchKernelGGL(prolong_kernel<double>, dim3(grid_for(n)), dim3(NT), 0, st, static_cast<const double*>(x), static_cast<const double*>(xc), agg, n, inv_relax, static_cast<double*>(out));
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void dense_cholesky_solve(const double* L, int64_t n, const void* b, void* x, bool f32,
+                          hipStream_t st) {
+  LGPU_REQUIRE(n <= 4096, "coarsest level too large for the dense direct solve");
+  const size_t sh = size_t(n) * sizeof(double);
+  if (f32) hipLaunchKernelGGL(chol_solve_kernel<float>, dim3(1), dim3(NT), sh, st, L, n, static_cast<const float*>(b), static_cast<float*>(x));
+  else hipLaunchKernelGGL(chol_solve_kernel<double>, dim3(1), dim3(NT), sh, st, L, n, static_cast<const double*>(b), static_cast<double*>(x));
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace lgpu

@@ -1,0 +1,216 @@
+// CSR passes (coarse AMG levels and the assembled finest operator): SpMV,
+// residual, fused reductions, Jacobi, and lexicographic SOR.
+//
+// One thread per row accumulates the row in column order with fma, exactly as
+// MatrixVectorProductExpr's row path (component-wise/matrix-vector-product.hpp:71-82).
+//
+// SOR: a sync-free wavefront over the row dependency graph.  Blocks of 256
+// consecutive rows are claimed through an atomic ticket in sweep order; a row
+// waits until every lower (forward) or higher (backward) neighbour row carries
+// the current sweep's epoch in its flag, then computes, publishes its value and
+// raises its own flag.  Dependencies only point to rows claimed earlier, so
+// every wait ends; waits are bounded and report a timeout on the error word.
+#include "device.hpp"
+
+namespace lgpu {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int SPIN_LIMIT = 1 << 22;
+
+template <typename T, typename V, int MODE>
+__global__ __launch_bounds__(NT) void csr_kernel(int64_t n, const int64_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci,
+                                                 const V* __restrict__ va, const T* __restrict__ x,
+                                                 const T* __restrict__ b, T* __restrict__ y,
+                                                 T omega, double* __restrict__ partials) {
+  const int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x;
+  double part = 0.0;
+  if (i < n) {
+    const int64_t k0 = rp[i], k1 = rp[i + 1];
+    if constexpr (MODE == SM_JACOBI) {
+      // jacobi/sweep.hpp:66-87
+      T acc = b[i], d = T(1);
+      for (int64_t k = k0; k < k1; ++k) {
+        const int64_t j = ci[k];
+        const T aij = T(va[k]);
+        if (j == i) d = aij;
+        else acc = fma(-aij, x ? x[j] : T(0), acc);
+      }
+      const T quot = acc / d;
+      y[i] = fma(omega, quot, (T(1) - omega) * (x ? x[i] : T(0)));
+    } else {
+      T sum = T(0);
+      for (int64_t k = k0; k < k1; ++k) sum = fma(T(va[k]), x[ci[k]], sum);
+      if constexpr (MODE == SM_SPMV) {
+        y[i] = sum;
+      } else if constexpr (MODE == SM_SPMV_DOT) {
+        y[i] = sum;
+        part = double(x[i]) * double(sum);
+      } else if constexpr (MODE == SM_RESID) {
+        y[i] = b[i] - sum;
+      } else {
+        const T r = b[i] - sum;
+        y[i] = r;
+        part = double(r) * double(r);
+      }
+    }
+  }
+  if constexpr (MODE == SM_SPMV_DOT || MODE == SM_RESID_NORM) {
+    const double t = block_sum<NT>(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  }
+}
+
+// gauss-seidel-variant/sweep.hpp:134-200 with one segment covering all rows
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
+__global__ __launch_bounds__(NT) void csr_sor_kernel(int64_t n, const int64_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ ci,
+                                                     const V* __restrict__ va,
+                                                     const T* __restrict__ xin, T* xout,
+                                                     const T* __restrict__ b, T omega, int* sched,
+                                                     int* flags, int epoch) {
+  __shared__ int s_ticket;
+  if (threadIdx.x == 0) s_ticket = atomicAdd(sched, 1);
+  __syncthreads();
+  const int64_t nblk = (n + NT - 1) / NT;
+  const int64_t blk = FWD ? s_ticket : nblk - 1 - s_ticket;
+  // rows inside the block in sweep order
+  const int64_t base = blk * NT;
+  const int64_t i = FWD ? base + threadIdx.x : base + (NT - 1 - threadIdx.x);
+  bool done = !(i >= 0 && i < n);
+  const T m1w = T(1) - omega;
+  int spins = 0;
+  while (true) {
+    if (!done) {
+      const int64_t k0 = rp[i], k1 = rp[i + 1];
+      bool ready = true;
+      for (int64_t k = k0; k < k1 && ready; ++k) {
+        const int64_t j = ci[k];
+        if (FWD ? (j < i) : (j > i))
+          ready = __hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+      }
+      if (!ready) {
+        // after any expired wait every row proceeds, so the sweep still ends
+        if (__hip_atomic_load(sched + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          ready = true;
+        } else if (++spins > SPIN_LIMIT) {
+          __hip_atomic_store(sched + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ready = true;
+        }
+      }
+      if (ready) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const T xi = HAS_X ? xin[i] : T(0);
+        T acc = b[i], d = T(1);
+        for (int64_t k = k0; k < k1; ++k) {
+          const int64_t j = ci[k];
+          const T aij = T(va[k]);
+          if (j == i) {
+            d = aij;
+            if (ULTRA) acc = fma(-aij, xi, acc);
+            continue;
+          }
+          const bool use_new = FWD ? (j < i) : (j > i);
+          T xj;
+          if (use_new) xj = __hip_atomic_load(xout + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          else xj = HAS_X ? xin[j] : T(0);
+          acc = fma(-aij, xj, acc);
+        }
+        T xn;
+        if (ULTRA) {
+          const T ad = d < T(0) ? -d : d;
+          xn = xi + (omega * acc) / (ad + T(0.5) * T(0));
+        } else {
+          const T quot = acc / d;
+          xn = fma(omega, quot, m1w * xi);
+        }
+        __hip_atomic_store(xout + i, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(flags + i, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        done = true;
+      } else {
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    // the whole wave leaves together once all its rows are done
+    if (__all(done)) break;
+  }
+}
+
+template <typename T, typename V, int MODE>
+void launch_apply(const CsrDesc& a, const void* x, const void* b, void* y, double omega,
+                  double* partials, hipStream_t st) {
+  const int64_t blocks = (a.n + NT - 1) / NT;
+  hipLaunchKernelGGL((csr_kernel<T, V, MODE>), dim3(blocks), dim3(NT), 0, st, a.n, a.rowptr,
+                     a.cols, static_cast<const V*>(a.vals), static_cast<const T*>(x),
+                     static_cast<const T*>(b), static_cast<T*>(y), T(omega), partials);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, typename V>
+void dispatch_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y,
+                    double omega, double* partials, hipStream_t st) {
+  switch (mode) {
+    case SM_SPMV: launch_apply<T, V, SM_SPMV>(a, x, b, y, omega, partials, st); break;
+    case SM_RESID: launch_apply<T, V, SM_RESID>(a, x, b, y, omega, partials, st); break;
+    case SM_SPMV_DOT: launch_apply<T, V, SM_SPMV_DOT>(a, x, b, y, omega, partials, st); break;
+    case SM_RESID_NORM: launch_apply<T, V, SM_RESID_NORM>(a, x, b, y, omega, partials, st); break;
+    case SM_JACOBI: launch_apply<T, V, SM_JACOBI>(a, x, b, y, omega, partials, st); break;
+    default: throw Error{LGPU_INVALID_ARG, "unknown CSR mode"};
+  }
+}
+
+template <typename T, typename V, bool FWD, bool ULTRA>
+void launch_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+                int* sched, int* flags, int epoch, hipStream_t st) {
+  const int64_t blocks = (a.n + NT - 1) / NT;
+  if (x_in) {
+    hipLaunchKernelGGL((csr_sor_kernel<T, V, FWD, ULTRA, true>), dim3(blocks), dim3(NT), 0, st,
+                       a.n, a.rowptr, a.cols, static_cast<const V*>(a.vals),
+                       static_cast<const T*>(x_in), static_cast<T*>(x_out),
+                       static_cast<const T*>(b), T(omega), sched, flags, epoch);
+  } else {
+    hipLaunchKernelGGL((csr_sor_kernel<T, V, FWD, ULTRA, false>), dim3(blocks), dim3(NT), 0, st,
+                       a.n, a.rowptr, a.cols, static_cast<const V*>(a.vals),
+                       static_cast<const T*>(x_in), static_cast<T*>(x_out),
+                       static_cast<const T*>(b), T(omega), sched, flags, epoch);
+  }
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, typename V>
+void dispatch_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+                  bool fwd, bool ultra, int* sched, int* flags, int epoch, hipStream_t st) {
+  if (fwd) {
+    if (ultra) launch_sor<T, V, true, true>(a, x_in, x_out, b, omega, sched, flags, epoch, st);
+    else launch_sor<T, V, true, false>(a, x_in, x_out, b, omega, sched, flags, epoch, st);
+  } else {
+    if (ultra) launch_sor<T, V, false, true>(a, x_in, x_out, b, omega, sched, flags, epoch, st);
+    else launch_sor<T, V, false, false>(a, x_in, x_out, b, omega, sched, flags, epoch, st);
+  }
+}
+
+}  // namespace
+
+int64_t csr_partials(const CsrDesc& a) { return (a.n + NT - 1) / NT; }
+
+// Arithmetic precision follows the vectors: f64 vectors with f64 or f32
+// values, f32 vectors with f32 values.
+void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y, double omega,
+               double* partials, hipStream_t st) {
+  (void)0;
+  if (a.f32) dispatch_apply<float, float>(a, mode, x, b, y, omega, partials, st);
+  else dispatch_apply<double, double>(a, mode, x, b, y, omega, partials, st);
+}
+
+void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+             bool forward, bool ultra, int* sched, int* flags, int epoch, hipStream_t st) {
+  LGPU_REQUIRE(x_in != x_out, "SOR sweep needs distinct input and output vectors");
+  vec_fill_int(sched, 1, 0, st);
+  if (a.f32) dispatch_sor<float, float>(a, x_in, x_out, b, omega, forward, ultra, sched, flags, epoch, st);
+  else dispatch_sor<double, double>(a, x_in, x_out, b, omega, forward, ultra, sched, flags, epoch, st);
+}
+
+}  // namespace lgpu

@@ -1,0 +1,63 @@
+// Host-side AMG setup: strength criterion, greedy aggregation, Galerkin
+// products and the coarsest Cholesky factor.  The hierarchy it produces is
+// uploaded once and consumed by the GPU V-cycle.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace lgpu::setup {
+
+// Rows of a level's matrix with the entries of each row in the reference's
+// iterate order (stencil rows: -x,-y,-z,+z,+y,+x, diagonal last; CSR rows:
+// column order with the diagonal inline).
+struct RowMatrix {
+  int64_t n = 0;
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> cols;
+  std::vector<double> vals;  // values of the level's type (f32 levels hold floats)
+  bool f32 = false;
+  bool symmetric = true;
+};
+
+struct AggregationStats {
+  int64_t aggregates = 0, multi = 0, single = 0, merged = 0, isolated = 0;
+};
+
+struct Params {
+  int max_level_num = 16;
+  int64_t min_coarsen_size = 32;
+  double min_coarsen_factor = 1.2;
+  double strong_threshold = 1.0 / 3.0;
+  double aggressive_strong_threshold = 1.0 / 3.0;
+  int64_t min_aggregate_size = 6;
+  int64_t max_aggregate_size = 9;
+  bool coarse_f32 = false;
+  int threads = 0;  // host threads for the order-independent phases (0: hardware)
+};
+
+struct Level {
+  std::vector<uint32_t> agg;      // fine vertex -> aggregate (0xFFFFFFFE isolated)
+  std::vector<int64_t> c2f_ptr;   // aggregate -> fine vertices, insertion order
+  std::vector<int32_t> c2f_idx;
+  AggregationStats stats;
+};
+
+struct Hierarchy {
+  std::vector<int64_t> sizes;     // level sizes, finest first
+  std::vector<Level> transfer;    // transfer[l] maps level l to level l+1
+  std::vector<RowMatrix> coarse;  // coarse[l] is level l+1's matrix (column order)
+  std::vector<double> chol_dense; // coarsest lower Cholesky factor, dense row major
+  bool spd = true;
+};
+
+constexpr uint32_t kUnaggregated = 0xFFFFFFFFu;
+constexpr uint32_t kIsolated = 0xFFFFFFFEu;
+
+Hierarchy build(const RowMatrix& finest, const Params& p);
+
+// Sparse Cholesky-Banachiewicz of a CSR matrix (column order rows); returns
+// false if a pivot is not positive.  Output rows in column order, diagonal last.
+bool cholesky(const RowMatrix& a, RowMatrix& lower);
+
+}  // namespace lgpu::setup

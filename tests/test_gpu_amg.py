@@ -1,0 +1,135 @@
+"""GPU parity of the AMG hierarchy, the V-cycle and the Krylov solvers."""
+
+import numpy as np
+import pytest
+
+import lineal_amd as la
+import oracle as O
+from lineal_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(dims, kind="values", sigma=1.0, seed=1):
+    box = la.BoxInfo(dims)
+    if kind == "values":
+        c = problems.lognormal(dims, sigma, seed)
+        return la.StencilMatrix.values(box, c), O.Stencil(dims, coeff=c)
+    if kind == "spheres":
+        m = problems.spheres(dims, seed, rmin=2, rmax=4)
+        mc = (1.0, 1e-6, 0.0, 0.0)
+    else:
+        m = problems.diagonal_lines(dims)
+        mc = (0.0, 1.0, 2.0, 3.0)
+    return la.StencilMatrix.lookup(box, m, mc), O.Stencil(dims, materials=m, mat_coeffs=mc)
+
+
+CASES = [((4, 9, 7), "lookup"), ((24, 20, 16), "values"), ((32, 32, 32), "values"), ((24, 24, 24), "spheres")]
+
+
+@pytest.mark.parametrize("dims,kind", CASES)
+@pytest.mark.parametrize("f32", [False, True])
+def test_hierarchy_equals_oracle(dims, kind, f32):
+    A, S = pair(dims, kind)
+    H = O.Hierarchy(S, coarse_f32=f32)
+    M = la.MultiGridSolver(precision=np.float32 if f32 else np.float64).instantiate(A)
+    assert M.level_num == H.level_num
+    for l in range(H.level_num):
+        assert M.level_size(l) == H.size(l)
+    for l in range(H.level_num - 1):
+        np.testing.assert_array_equal(M.aggregates(l), H.agg_map(l))
+    for l in range(1, H.level_num):
+        rp, cols, vals = M.matrix(l)
+        ref = H.matrix(l)
+        np.testing.assert_array_equal(rp, ref.rowptr)
+        np.testing.assert_array_equal(cols, ref.cols)
+        np.testing.assert_array_equal(vals, ref.vals)
+
+
+@pytest.mark.parametrize("dims,kind", CASES)
+@pytest.mark.parametrize("ultra", [False, True])
+def test_vcycle_equals_oracle(dims, kind, ultra):
+    A, S = pair(dims, kind)
+    for f32 in (False, True):
+        H = O.Hierarchy(S, coarse_f32=f32)
+        M = la.MultiGridSolver(precision=np.float32 if f32 else np.float64, ultra=ultra).instantiate(A)
+        r = problems.uniform(S.N, 5)
+        if f32:
+            r32 = r.astype(np.float32)
+            z = M.apply(la.DenseVector(r32), la.DenseVector(S.N, np.float32)).numpy()
+            ref = H.vcycle(r32, ultra)
+        else:
+            z = M.apply(la.DenseVector(r), la.DenseVector(S.N)).numpy()
+            ref = H.vcycle(r, ultra)
+        np.testing.assert_array_equal(z, ref)
+
+
+def test_reference_pcg_amg_thresholds():
+    """test/shared-stencil.hpp:154-190: known solution 2, pcg_amg <= 1e-13 within 32 iterations."""
+    dims = (4, 9, 7)
+    A, S = pair(dims, "lookup")
+    two = la.DenseVector(np.full(S.N, 2.0))
+    b = A * two
+    for ultra in (False, True):
+        M = la.MultiGridSolver(ultra=ultra).instantiate(A)
+        for solver in (la.ConjugateGradientsSolver(8, M), la.BiCgStabSolver(8, M)):
+            x = la.DenseVector(S.N, fill=0.0)
+            res = solver.solve(A, x, b, rel_tol=1e-15, max_it=32)
+            xv = x.numpy()
+            assert res.residuals[-1] <= 1e-13
+            assert np.abs(xv - 2.0).max() <= 1e-13
+
+
+@pytest.mark.parametrize("dims,kind", CASES[1:])
+@pytest.mark.parametrize("precision", [np.float64, np.float32])
+def test_pcg_amg_matches_oracle(dims, kind, precision):
+    A, S = pair(dims, kind)
+    f32 = precision == np.float32
+    H = O.Hierarchy(S, coarse_f32=f32)
+    M = la.MultiGridSolver(precision=precision).instantiate(A)
+    b = S.rhs()
+    x_ref, it_ref, hist_ref = O.pcg(S, b, H, precon=2 if f32 else 1, rel_tol=1e-8, max_it=200)
+    x = la.DenseVector(S.N, fill=0.0)
+    res = la.ConjugateGradientsSolver(8, M).solve(A, x, la.DenseVector(b), rel_tol=1e-8, max_it=200)
+    assert res.converged
+    assert abs(res.iterations - it_ref) <= 1
+    xv = x.numpy()
+    assert np.abs(xv - x_ref).max() / np.abs(x_ref).max() <= 1e-7
+    # the same recurrence: residual histories agree to rounding
+    k = min(len(hist_ref), len(res.residuals))
+    np.testing.assert_allclose(res.residuals[:k], hist_ref[:k], rtol=1e-6)
+
+
+@pytest.mark.parametrize("dims,kind", CASES[1:3])
+def test_bicgstab_amg_matches_oracle(dims, kind):
+    A, S = pair(dims, kind)
+    H = O.Hierarchy(S)
+    M = la.MultiGridSolver().instantiate(A)
+    b = S.rhs()
+    x_ref, it_ref, _ = O.pbicgstab(S, b, H, precon=1, rel_tol=1e-8, max_it=200)
+    x = la.DenseVector(S.N, fill=0.0)
+    res = la.BiCgStabSolver(8, M).solve(A, x, la.DenseVector(b), rel_tol=1e-8, max_it=200)
+    assert res.converged
+    assert abs(res.iterations - it_ref) <= 1
+    assert np.abs(x.numpy() - x_ref).max() / np.abs(x_ref).max() <= 1e-7
+
+
+def test_unpreconditioned_cg_exact_termination():
+    dims = (3, 4, 2)
+    A, S = pair(dims, "values")
+    b = S.rhs()
+    x = la.DenseVector(S.N, fill=0.0)
+    res = la.ConjugateGradientsSolver(8).solve(A, x, la.DenseVector(b), rel_tol=1e-14, max_it=S.N + 5)
+    assert res.converged
+    dense = np.zeros((S.N, S.N))
+    for i in range(S.N):
+        c, v = S.row(i)
+        dense[i, c] = v
+    np.testing.assert_allclose(x.numpy(), np.linalg.solve(dense, b), rtol=1e-12)
+
+
+def test_solver_status_zero_rhs():
+    A, S = pair((8, 8, 8), "values")
+    x = la.DenseVector(S.N, fill=0.0)
+    res = la.ConjugateGradientsSolver(8).solve(A, x, la.DenseVector(S.N, fill=0.0), rel_tol=1e-8, max_it=10)
+    assert res.converged and res.iterations == 0

@@ -1,0 +1,102 @@
+"""GPU parity of the single kernels against the oracle (bit-exact).
+
+Every kernel performs the oracle's operations in the oracle's order (explicit
+fma, no contraction), so results must be identical, not merely close.
+"""
+
+import numpy as np
+import pytest
+
+import lineal_amd as la
+import oracle as O
+from lineal_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(4, 9, 7), (1, 1, 1), (3, 1, 70), (13, 17, 5), (33, 17, 65), (40, 40, 40)]
+
+
+def make_pair(dims, kind, seed=1, sigma=1.0, p_start=1.0, p_end=0.0, mat_coeffs=(0.0, 1.0, 2.0, 3.0)):
+    box = la.BoxInfo(dims, p_start=p_start, p_end=p_end)
+    if kind == "values":
+        c = problems.lognormal(dims, sigma, seed)
+        return la.StencilMatrix.values(box, c), O.Stencil(dims, coeff=c, p_start=p_start, p_end=p_end)
+    m = problems.diagonal_lines(dims) if kind == "lookup" else problems.spheres(dims, seed, rmin=1, rmax=3)
+    mc = mat_coeffs if kind == "lookup" else (1.0, 1e-6, 0.0, 0.0)
+    return (la.StencilMatrix.lookup(box, m, mc),
+            O.Stencil(dims, materials=m, mat_coeffs=mc, p_start=p_start, p_end=p_end))
+
+
+def same(a, b):
+    assert a.shape == b.shape
+    bad = np.flatnonzero(~((a == b) | (np.isnan(a) & np.isnan(b))))
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: {a[bad[:5]]} vs {b[bad[:5]]}"
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+@pytest.mark.parametrize("kind", ["values", "lookup", "spheres"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_residual_rhs(dims, kind, dtype):
+    A, S = make_pair(dims, kind)
+    n = S.N
+    x = problems.uniform(n, 2, dtype)
+    xd = la.DenseVector(x)
+    same(A.spmv(xd, la.DenseVector(n, dtype)).numpy(), O.spmv(S, x))
+    b = S.rhs(dtype)
+    same(A.rhs(dtype).numpy(), b)
+    r = A.residual(la.DenseVector(b), xd, la.DenseVector(n, dtype)).numpy()
+    same(r, O.residual(S, b, x))
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+@pytest.mark.parametrize("kind", ["values", "lookup"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("with_x", [True, False])
+def test_jacobi(dims, kind, dtype, with_x):
+    A, S = make_pair(dims, kind)
+    n = S.N
+    x = problems.uniform(n, 2, dtype)
+    b = problems.uniform(n, 3, dtype)
+    omega = 2.0 / 3.0
+    out = A.jacobi(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype), la.DenseVector(b), omega)
+    same(out.numpy(), O.jacobi(S, x if with_x else None, b, dtype(omega)))
+
+
+@pytest.mark.parametrize("dims", SHAPES + [(17, 64, 64)])
+@pytest.mark.parametrize("kind", ["values", "lookup", "spheres"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("forward", [True, False])
+@pytest.mark.parametrize("ultra", [False, True])
+def test_sor_wavefront(dims, kind, dtype, forward, ultra):
+    A, S = make_pair(dims, kind)
+    n = S.N
+    x = problems.uniform(n, 2, dtype)
+    b = problems.uniform(n, 3, dtype)
+    for omega, with_x in ((1.0, True), (1.3, True), (1.0, False)):
+        out = A.sor(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype), la.DenseVector(b),
+                    omega, forward, ultra)
+        ref = O.sor(S, x if with_x else None, b, dtype(omega), forward, ultra)
+        same(out.numpy(), ref)
+
+
+@pytest.mark.parametrize("dims", [(4, 9, 7), (13, 17, 5), (33, 17, 65)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_csr_kernels(dims, dtype):
+    A, S = make_pair(dims, "values")
+    C, _, _ = S.to_csr()
+    if dtype == np.float32:
+        C = O.Csr(C.rowptr, C.cols, C.vals.astype(np.float32).astype(np.float64), is_f32=True)
+    G = A.to_csr(dtype)
+    n = S.N
+    x = problems.uniform(n, 2, dtype)
+    b = problems.uniform(n, 3, dtype)
+    same(G.spmv(la.DenseVector(x), la.DenseVector(n, dtype)).numpy(), O.spmv(C, x))
+    same(G.residual(la.DenseVector(b), la.DenseVector(x), la.DenseVector(n, dtype)).numpy(), O.residual(C, b, x))
+    same(G.jacobi(la.DenseVector(x), la.DenseVector(n, dtype), la.DenseVector(b), 0.7).numpy(),
+         O.jacobi(C, x, b, dtype(0.7)))
+    for forward in (True, False):
+        for ultra in (False, True):
+            for with_x in (True, False):
+                out = G.sor(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype), la.DenseVector(b),
+                            1.0, forward, ultra)
+                same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.0), forward, ultra))
