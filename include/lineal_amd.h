@@ -119,6 +119,13 @@ int lgpu_op_jacobi(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double
  * must not alias x_out. */
 int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
                 int forward, int ultra);
+/* One SOR sweep of a CSR operator with the reference's thread-segment
+ * semantics (gauss-seidel-variant/sweep.hpp:153-169; the segments of
+ * parallel/execution-policy/adaptive.hpp:266-279): rows are cut into segments
+ * of segment_rows (1..64) consecutive rows, each swept in order with new values
+ * only from its own segment. */
+int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
+                          int forward, int ultra, int64_t segment_rows);
 
 /* ---- aggregation AMG ----------------------------------------------------- */
 /* HierarchyWrightParams + AggregationParams + DuneDependencyCriterion
@@ -134,6 +141,10 @@ typedef struct {
   int precision;                /* LGPU_F64: FP64 V-cycle; LGPU_F32: FP32 hierarchy,
                                    smoothers and level operators, FP64 coarsest solve */
   int ultra;                    /* SOR variant of the smoothers */
+  int64_t coarse_segment_rows;  /* coarse-level (CSR) smoothers: rows per thread segment
+                                   (1..64, default 32); 0 sweeps each coarse level in one
+                                   segment (lexicographic). The finest level is always
+                                   swept lexicographically. */
 } lgpu_amg_params;
 
 void lgpu_amg_default_params(lgpu_amg_params* p);

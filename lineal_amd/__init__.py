@@ -74,6 +74,7 @@ class _AmgParams(C.Structure):
         ("max_aggregate_size", C.c_int64),
         ("precision", C.c_int),
         ("ultra", C.c_int),
+        ("coarse_segment_rows", C.c_int64),
     ]
 
 
@@ -111,6 +112,7 @@ SIGNATURES = {
     "lgpu_op_residual": (C.c_int, [_vp, _vp, _vp, _vp]),
     "lgpu_op_jacobi": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
     "lgpu_op_sor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int]),
+    "lgpu_op_sor_segmented": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _i64]),
     "lgpu_amg_default_params": (None, [C.POINTER(_AmgParams)]),
     "lgpu_amg_create": (C.c_int, [_vp, C.POINTER(_AmgParams), C.POINTER(_vp)]),
     "lgpu_amg_destroy": (C.c_int, [_vp]),
@@ -202,13 +204,16 @@ def default_context() -> Context:
 class DenseVector:
     """Device vector (tensor/dynamic/dense/vector/shared.hpp:35-362)."""
 
-    def __init__(self, size_or_array, dtype=np.float64, ctx: Context | None = None, fill=None):
+    def __init__(self, size_or_array, dtype=None, ctx: Context | None = None, fill=None):
         self.ctx = ctx or default_context()
         if isinstance(size_or_array, (int, np.integer)):
             self.n = int(size_or_array)
-            self.dtype = np.dtype(dtype)
+            self.dtype = np.dtype(dtype or np.float64)
             host = None
         else:
+            if dtype is None:
+                src = np.asarray(size_or_array)
+                dtype = src.dtype if src.dtype in (np.float32, np.float64) else np.float64
             host = np.ascontiguousarray(size_or_array, dtype=dtype).reshape(-1)
             self.n = host.size
             self.dtype = host.dtype
@@ -288,8 +293,13 @@ class _Matrix:
         _check(lib().lgpu_op_jacobi(self.h, _vh(x_in), x_out.h, b.h, float(omega)))
         return x_out
 
-    def sor(self, x_in, x_out, b, omega=1.0, forward=True, ultra=False):
-        _check(lib().lgpu_op_sor(self.h, _vh(x_in), x_out.h, b.h, float(omega), int(forward), int(ultra)))
+    def sor(self, x_in, x_out, b, omega=1.0, forward=True, ultra=False, segment_rows=0):
+        """One SOR sweep; segment_rows > 0 (CSR only) uses thread segments of that many rows."""
+        if segment_rows:
+            _check(lib().lgpu_op_sor_segmented(self.h, _vh(x_in), x_out.h, b.h, float(omega), int(forward),
+                                               int(ultra), int(segment_rows)))
+        else:
+            _check(lib().lgpu_op_sor(self.h, _vh(x_in), x_out.h, b.h, float(omega), int(forward), int(ultra)))
         return x_out
 
     def __del__(self):
@@ -427,10 +437,11 @@ class HierarchyWrightParams:
     min_aggregate_size: int = 6
     max_aggregate_size: int = 9
 
-    def _struct(self, precision: int, ultra: bool):
+    def _struct(self, precision: int, ultra: bool, coarse_segment_rows: int):
         return _AmgParams(self.max_level_num, self.min_coarsen_size, self.min_coarsen_factor,
                           self.strong_threshold, self.aggressive_strong_threshold,
-                          self.min_aggregate_size, self.max_aggregate_size, precision, int(ultra))
+                          self.min_aggregate_size, self.max_aggregate_size, precision, int(ultra),
+                          int(coarse_segment_rows))
 
 
 class MultiGridSolver:
@@ -438,12 +449,16 @@ class MultiGridSolver:
 
     precision: np.float64 for an FP64 V-cycle, np.float32 for the FP32 hierarchy,
     smoothers and level operators with an FP64 coarsest solve.
+    coarse_segment_rows: rows per thread segment of the coarse-level smoothers
+    (0: one lexicographic segment per level).
     """
 
-    def __init__(self, wright: HierarchyWrightParams | None = None, precision=np.float64, ultra=False):
+    def __init__(self, wright: HierarchyWrightParams | None = None, precision=np.float64, ultra=False,
+                 coarse_segment_rows=32):
         self.wright = wright or HierarchyWrightParams()
         self.precision = precision
         self.ultra = ultra
+        self.coarse_segment_rows = coarse_segment_rows
 
     def instantiate(self, lhs: _Matrix) -> "MultiGridInstance":
         return MultiGridInstance(lhs, self)
@@ -452,7 +467,7 @@ class MultiGridSolver:
 class MultiGridInstance:
     def __init__(self, lhs: _Matrix, solver: MultiGridSolver):
         self.lhs = lhs
-        p = solver.wright._struct(_dtype_code(solver.precision), solver.ultra)
+        p = solver.wright._struct(_dtype_code(solver.precision), solver.ultra, solver.coarse_segment_rows)
         h = _vp()
         _check(lib().lgpu_amg_create(lhs.h, C.byref(p), C.byref(h)))
         self.h = h

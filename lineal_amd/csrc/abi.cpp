@@ -344,6 +344,15 @@ void dot_into(lgpu_ctx ctx, const void* a, const void* b, bool f32, int64_t n, i
   reduce_partials(part, np, ctx->scal.as<double>() + slot, ctx->stream);
 }
 
+// smoother of a coarse level: thread segments of coarse_segment_rows rows, or
+// one lexicographic segment
+void coarse_sweep(lgpu_amg amg, CsrDev& a, const void* x_in, void* x_out, const void* b,
+                  double omega, bool fwd, bool ultra) {
+  const int64_t seg = amg->params.coarse_segment_rows;
+  if (seg > 0) csr_sor_segmented(a.desc(), x_in, x_out, b, omega, fwd, ultra, int(seg), amg->ctx->stream);
+  else csr_sweep(a, x_in, x_out, b, omega, fwd, ultra, amg->ctx->stream);
+}
+
 // One V-cycle in precision T from zero: z = M r (multigrid/solver.hpp:56-176)
 void vcycle(lgpu_amg amg, const void* r, void* z) {
   const bool f32 = amg->f32;
@@ -359,7 +368,7 @@ void vcycle(lgpu_amg amg, const void* r, void* z) {
       op_sor(amg->finest, f32, nullptr, xs(0), bs(0), relax, true, ultra);
       op_apply(amg->finest, SM_RESID, f32, xs(0), bs(0), cur.tmp.ptr, 0.0, nullptr);
     } else {
-      csr_sweep(cur.a, nullptr, xs(l), bs(l), relax, true, ultra, st);
+      coarse_sweep(amg, cur.a, nullptr, xs(l), bs(l), relax, true, ultra);
       csr_apply(cur.a.desc(), SM_RESID, xs(l), bs(l), cur.tmp.ptr, 0.0, nullptr, st);
     }
     const int64_t nc = amg->lv[size_t(l + 1)].n;
@@ -375,7 +384,7 @@ void vcycle(lgpu_amg amg, const void* r, void* z) {
     prolong_add(xs(l), amg->lv[size_t(l + 1)].sol.ptr, f32, cur.agg.as<uint32_t>(), cur.n,
                 inv_relax, cur.aux.ptr, st);
     if (l == 0) op_sor(amg->finest, f32, cur.aux.ptr, xs(0), bs(0), relax, false, ultra);
-    else csr_sweep(cur.a, cur.aux.ptr, xs(l), bs(l), relax, false, ultra, st);
+    else coarse_sweep(amg, cur.a, cur.aux.ptr, xs(l), bs(l), relax, false, ultra);
   }
 }
 
@@ -776,6 +785,23 @@ extern "C" int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b
   API_END
 }
 
+extern "C" int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b,
+                                     double omega, int forward, int ultra, int64_t segment_rows) {
+  API_BEGIN
+  LGPU_REQUIRE(op && op->is_csr, "segmented SOR needs a CSR operator");
+  LGPU_REQUIRE(segment_rows >= 1 && segment_rows <= kMaxSegmentRows, "segment_rows must lie in 1..64");
+  check_vec(x_out, op->rows(), "x_out");
+  check_vec(b, op->rows(), "b");
+  if (x_in) {
+    check_vec(x_in, op->rows(), "x_in");
+    LGPU_REQUIRE(x_in->f32 == x_out->f32 && x_in->ptr != x_out->ptr, "x_in must match x_out and not alias it");
+  }
+  LGPU_REQUIRE(b->f32 == x_out->f32 && op->csr.f32 == x_out->f32, "vectors and values must share a dtype");
+  csr_sor_segmented(op->csr.desc(), x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega, forward != 0,
+                    ultra != 0, int(segment_rows), op->ctx->stream);
+  API_END
+}
+
 // ---------------------------------------------------------------------------
 // AMG
 
@@ -790,6 +816,7 @@ extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {
   p->max_aggregate_size = 9;
   p->precision = LGPU_F64;
   p->ultra = 0;
+  p->coarse_segment_rows = 32;
 }
 
 extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out) {
@@ -797,6 +824,8 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
   LGPU_REQUIRE(finest && params && out, "NULL argument");
   LGPU_REQUIRE(params->precision == LGPU_F64 || params->precision == LGPU_F32, "unknown precision");
   LGPU_REQUIRE(params->max_aggregate_size >= 1 && params->min_aggregate_size >= 1, "invalid aggregate sizes");
+  LGPU_REQUIRE(params->coarse_segment_rows >= 0 && params->coarse_segment_rows <= kMaxSegmentRows,
+               "coarse_segment_rows must lie in 0..64");
   LGPU_REQUIRE(!finest->is_csr || !finest->csr.f32, "the finest CSR operator must hold f64 values");
   const auto t0 = std::chrono::steady_clock::now();
   auto amg = std::make_unique<lgpu_amg_s>();

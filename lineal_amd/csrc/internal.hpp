@@ -132,6 +132,10 @@ void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y
                double* partials, hipStream_t st);
 void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
              bool forward, bool ultra, int* sched, int* flags, int epoch, hipStream_t st);
+// thread-segment SOR: segments of `seg` consecutive rows, 1 <= seg <= kMaxSegmentRows
+constexpr int kMaxSegmentRows = 64;
+void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
+                       double omega, bool forward, bool ultra, int seg, hipStream_t st);
 
 // vector kernels
 void vec_fill(void* x, int64_t n, bool f32, double v, hipStream_t st);

@@ -139,6 +139,89 @@ __global__ __launch_bounds__(NT) void csr_sor_kernel(int64_t n, const int64_t* _
   }
 }
 
+// SOR with the reference's thread-segment semantics
+// (gauss-seidel-variant/sweep.hpp:134-200): rows are cut into segments of
+// `seg` consecutive rows; one thread sweeps one segment in order, using new
+// values only for columns inside its own segment (kept in LDS) and old values
+// elsewhere; the ultra variant adds |a_ij| of columns outside the segment to
+// the divisor.
+constexpr int SEG_NT = 128;
+
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
+__global__ __launch_bounds__(SEG_NT) void csr_sor_segment_kernel(
+    int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const V* __restrict__ va, const T* __restrict__ xin, T* __restrict__ xout,
+    const T* __restrict__ b, T omega, int seg) {
+  extern __shared__ unsigned char smem[];
+  T* fresh = reinterpret_cast<T*>(smem) + int64_t(threadIdx.x) * seg;
+  const int64_t s = blockIdx.x * int64_t(SEG_NT) + threadIdx.x;
+  const int64_t s0 = s * seg;
+  if (s0 >= n) return;
+  const int64_t s1 = min(s0 + seg, n);
+  const T m1w = T(1) - omega;
+  for (int64_t t = 0; t < s1 - s0; ++t) {
+    const int64_t i = FWD ? s0 + t : s1 - 1 - t;
+    const T xi = HAS_X ? xin[i] : T(0);
+    T acc = b[i], d = T(1), div = T(0);
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+      const int64_t j = ci[k];
+      const T aij = T(va[k]);
+      if (j == i) {
+        d = aij;
+        if (ULTRA) acc = fma(-aij, xi, acc);
+        continue;
+      }
+      const bool inside = j >= s0 && j < s1;
+      if (ULTRA) div += inside ? T(0) : (aij < T(0) ? -aij : aij);
+      const bool use_new = inside && (FWD ? j < i : j > i);
+      const T xj = use_new ? fresh[j - s0] : (HAS_X ? xin[j] : T(0));
+      acc = fma(-aij, xj, acc);
+    }
+    T xn;
+    if (ULTRA) {
+      const T ad = d < T(0) ? -d : d;
+      xn = xi + (omega * acc) / (ad + T(0.5) * div);
+    } else {
+      const T quot = acc / d;
+      xn = fma(omega, quot, m1w * xi);
+    }
+    fresh[i - s0] = xn;
+    xout[i] = xn;
+  }
+}
+
+template <typename T, typename V, bool FWD, bool ULTRA>
+void launch_segment(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+                    int seg, hipStream_t st) {
+  const int64_t nseg = (a.n + seg - 1) / seg;
+  const int64_t blocks = (nseg + SEG_NT - 1) / SEG_NT;
+  const size_t sh = size_t(SEG_NT) * size_t(seg) * sizeof(T);
+  if (x_in) {
+    hipLaunchKernelGGL((csr_sor_segment_kernel<T, V, FWD, ULTRA, true>), dim3(blocks),
+                       dim3(SEG_NT), sh, st, a.n, a.rowptr, a.cols, static_cast<const V*>(a.vals),
+                       static_cast<const T*>(x_in), static_cast<T*>(x_out),
+                       static_cast<const T*>(b), T(omega), seg);
+  } else {
+    hipLaunchKernelGGL((csr_sor_segment_kernel<T, V, FWD, ULTRA, false>), dim3(blocks),
+                       dim3(SEG_NT), sh, st, a.n, a.rowptr, a.cols, static_cast<const V*>(a.vals),
+                       static_cast<const T*>(x_in), static_cast<T*>(x_out),
+                       static_cast<const T*>(b), T(omega), seg);
+  }
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, typename V>
+void dispatch_segment(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
+                      double omega, bool fwd, bool ultra, int seg, hipStream_t st) {
+  if (fwd) {
+    if (ultra) launch_segment<T, V, true, true>(a, x_in, x_out, b, omega, seg, st);
+    else launch_segment<T, V, true, false>(a, x_in, x_out, b, omega, seg, st);
+  } else {
+    if (ultra) launch_segment<T, V, false, true>(a, x_in, x_out, b, omega, seg, st);
+    else launch_segment<T, V, false, false>(a, x_in, x_out, b, omega, seg, st);
+  }
+}
+
 template <typename T, typename V, int MODE>
 void launch_apply(const CsrDesc& a, const void* x, const void* b, void* y, double omega,
                   double* partials, hipStream_t st) {
@@ -203,6 +286,14 @@ void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y
   (void)0;
   if (a.f32) dispatch_apply<float, float>(a, mode, x, b, y, omega, partials, st);
   else dispatch_apply<double, double>(a, mode, x, b, y, omega, partials, st);
+}
+
+void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
+                       double omega, bool forward, bool ultra, int seg, hipStream_t st) {
+  LGPU_REQUIRE(x_in != x_out, "SOR sweep needs distinct input and output vectors");
+  LGPU_REQUIRE(seg >= 1 && seg <= kMaxSegmentRows, "segment length out of range");
+  if (a.f32) dispatch_segment<float, float>(a, x_in, x_out, b, omega, forward, ultra, seg, st);
+  else dispatch_segment<double, double>(a, x_in, x_out, b, omega, forward, ultra, seg, st);
 }
 
 void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,

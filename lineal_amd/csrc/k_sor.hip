@@ -7,13 +7,22 @@
 // (8I + a, 8J + b) and at step t computes z = t - a - b.  Its -x and -y
 // neighbours were computed by lanes (a-1, b) and (a, b-1) one step earlier and
 // arrive by cross-lane shuffles; its -z neighbour is its own previous value.
-// Lines on the column border read the new values of the upstream columns
-// (I-1, J) and (I, J-1) from memory after waiting on their progress counters.
-// Columns are claimed through an atomic ticket in diagonal (I + J) order, so a
-// column only ever waits on columns already claimed by running wavefronts:
-// every wait ends.  The backward sweep is the same schedule in reflected
-// coordinates.  Each cell performs exactly the oracle's operations in the
-// oracle's order, so the result equals one-segment lexicographic SOR.
+// The face weights are shared the same way: the -x / -y / -z weight of a cell
+// is the +x / +y / +z weight its predecessor computed one step earlier (the
+// harmonic mean is symmetric bit for bit), so each cell evaluates three faces.
+//
+// Steps run in chunks of K.  At the start of a chunk every lane loads the
+// coefficient, right-hand-side and old-value windows of its own line for the
+// chunk's K cells (lanes on the column border also load the neighbouring
+// column's lines), so the K steps run from registers and shuffles with one
+// memory round trip per chunk.  Border lines then read the new values of the
+// upstream columns (I-1, J) and (I, J-1) after waiting on their progress
+// counters.  Columns are claimed through an atomic ticket in diagonal (I + J)
+// order, so a column only waits on columns already claimed by running
+// wavefronts: every wait ends (waits are also bounded and report a timeout).
+// The backward sweep is the same schedule in reflected coordinates.  Each cell
+// performs exactly the oracle's operations in the oracle's order, so the
+// result equals one-segment lexicographic SOR bit for bit.
 #include "device.hpp"
 
 namespace lgpu {
@@ -21,7 +30,7 @@ namespace lgpu {
 namespace {
 
 constexpr int CB = 8;        // lines per column side
-constexpr int KCH = 16;      // steps between progress publications
+constexpr int K = 8;         // steps per chunk (one progress publication each)
 constexpr int SPIN_LIMIT = 1 << 24;
 
 __device__ __forceinline__ void wait_progress(int* state, int64_t col, int need) {
@@ -37,13 +46,19 @@ __device__ __forceinline__ void wait_progress(int* state, int64_t col, int need)
   }
 }
 
-template <typename T, int K, bool FWD, bool ULTRA, bool HAS_X>
+template <typename R>
+__device__ __forceinline__ R shfl_raw(R v, int src) {
+  if constexpr (sizeof(R) == 1) return R(__shfl(int(v), src, 64));
+  else return __shfl(v, src, 64);
+}
+
+template <typename T, int KIND, bool FWD, bool ULTRA, bool HAS_X>
 __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __restrict__ xin,
                                                     T* __restrict__ xout,
                                                     const T* __restrict__ bv, T omega,
                                                     int* state, const uint32_t* order,
                                                     int64_t cy) {
-  using CI = CellInfo<T, K>;
+  using CI = CellInfo<T, KIND>;
   using Raw = typename CI::Raw;
   const int lane = threadIdx.x;
   const int a = lane / CB, b = lane % CB;
@@ -52,17 +67,55 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   __syncthreads();
   const int64_t col = order[s_ticket];
   const int64_t I = col / cy, J = col % cy;
-  const int64_t lx = I * CB + a, ly = J * CB + b;
-  const bool line = lx < s.nx && ly < s.ny;
-  const int64_t x = FWD ? lx : s.nx - 1 - lx;
-  const int64_t y = FWD ? ly : s.ny - 1 - ly;
-  const int64_t nyz = s.ny * s.nz;
-  const int64_t nsteps = s.nz + 2 * (CB - 1);
+  const int64_t nx = s.nx, ny = s.ny, nz = s.nz, nyz = ny * nz;
+  const int64_t lx = I * CB + a, ly = J * CB + b;  // logical line coordinates
+  const bool line = lx < nx && ly < ny;
+  // physical coordinates and the physical offsets of the logical +x / +y / +z steps
+  const int64_t x = FWD ? lx : nx - 1 - lx;
+  const int64_t y = FWD ? ly : ny - 1 - ly;
+  const int64_t sx = FWD ? nyz : -nyz, sy = FWD ? nz : -nz, sz = FWD ? 1 : -1;
+  const bool has_sx = lx + 1 < nx, has_px = lx > 0;  // logical successor / predecessor lines
+  const bool has_sy = ly + 1 < ny, has_py = ly > 0;
+  // start of the line, clamped into the box for lanes outside it
+  const int64_t cx_ = FWD ? min(lx, nx - 1) : nx - 1 - min(lx, nx - 1);
+  const int64_t cy_ = FWD ? min(ly, ny - 1) : ny - 1 - min(ly, ny - 1);
+  const int64_t lbase = (cx_ * ny + cy_) * nz;
+  const int64_t nsteps = nz + 2 * (CB - 1);
   const T m1w = T(1) - omega;
+  const T qx = T(s.q[0]), qy = T(s.q[1]), qz = T(s.q[2]);
+  const T two_q0 = T(2) * qx;
   int* progress = state + 2;
-  T prev = T(0);
-  for (int64_t t0 = 0; t0 < nsteps; t0 += KCH) {
-    const int64_t tend = min(t0 + KCH, nsteps);
+  // physical cell index of logical z on this line (clamped into the line)
+  auto cell = [&](int64_t lz) {
+    const int64_t c = lz < 0 ? 0 : (lz >= nz ? nz - 1 : lz);
+    return lbase + (FWD ? c : nz - 1 - c);
+  };
+  T prev = T(0), wpx_prev = T(0), wpy_prev = T(0), wpz_prev = T(0);
+  for (int64_t t0 = 0; t0 < nsteps; t0 += K) {
+    const int64_t z0 = t0 - a - b;
+    // ---- chunk windows (independent of the recurrence; one round trip) ----
+    Raw cw[K + 2], ecx[K + 2], ecy[K + 2], ucx[K + 2], ucy[K + 2];
+    T xw[K + 1], exx[K + 1], exy[K + 1], bw[K];
+#pragma unroll
+    for (int j = 0; j < K + 2; ++j) {
+      const int64_t i = cell(z0 - 1 + j);
+      cw[j] = CI::load(s, i);
+      ecx[j] = (a == CB - 1 && has_sx) ? CI::load(s, i + sx) : Raw{};
+      ecy[j] = (b == CB - 1 && has_sy) ? CI::load(s, i + sy) : Raw{};
+      ucx[j] = (a == 0 && has_px) ? CI::load(s, i - sx) : Raw{};
+      ucy[j] = (b == 0 && has_py) ? CI::load(s, i - sy) : Raw{};
+    }
+#pragma unroll
+    for (int j = 0; j < K + 1; ++j) {
+      const int64_t i = cell(z0 + j);
+      xw[j] = HAS_X ? xin[i] : T(0);
+      exx[j] = (HAS_X && a == CB - 1 && has_sx) ? xin[i + sx] : T(0);
+      exy[j] = (HAS_X && b == CB - 1 && has_sy) ? xin[i + sy] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) bw[j] = bv[cell(z0 + j)];
+    // ---- wait for the upstream columns, then read their new border values ----
+    const int64_t tend = min(t0 + K, nsteps);
     const int need = int(min(tend + CB - 1, nsteps));
     if (lane == 0) {
       if (I > 0) wait_progress(state, (I - 1) * cy + J, need);
@@ -70,68 +123,99 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __builtin_amdgcn_wave_barrier();
-    for (int64_t t = t0; t < tend; ++t) {
-      // new values of the logical -x / -y predecessors, computed one step ago
-      const T nbx = __shfl(prev, lane >= CB ? lane - CB : lane, 64);
-      const T nby = __shfl(prev, lane >= 1 ? lane - 1 : lane, 64);
-      const int64_t lz = t - a - b;
-      if (line && lz >= 0 && lz < s.nz) {
-        const int64_t z = FWD ? lz : s.nz - 1 - lz;
-        const int64_t i = (x * s.ny + y) * s.nz + z;
-        const bool hmx = x > 0, hpx = x + 1 < s.nx;
-        const bool hmy = y > 0, hpy = y + 1 < s.ny;
-        const bool hmz = z > 0, hpz = z + 1 < s.nz;
-        const Raw cc = CI::load(s, i);
-        const Raw cmx = hmx ? CI::load(s, i - nyz) : Raw{};
-        const Raw cpx = hpx ? CI::load(s, i + nyz) : Raw{};
-        const Raw cmy = hmy ? CI::load(s, i - s.nz) : Raw{};
-        const Raw cpy = hpy ? CI::load(s, i + s.nz) : Raw{};
-        const Raw cmz = hmz ? CI::load(s, i - 1) : Raw{};
-        const Raw cpz = hpz ? CI::load(s, i + 1) : Raw{};
-        const Row<T> r = eval_row<T, K>(s, x, y, z, cc, cmx, cmy, cmz, cpz, cpy, cpx);
-        // predecessor (new) values in logical coordinates
-        T newx, newy;
-        if (a > 0) newx = nbx;
-        else newx = lx > 0 ? xout[FWD ? i - nyz : i + nyz] : T(0);
-        if (b > 0) newy = nby;
-        else newy = ly > 0 ? xout[FWD ? i - s.nz : i + s.nz] : T(0);
-        const T newz = lz > 0 ? prev : T(0);
-        // successor (old) values
-        T oldx = T(0), oldy = T(0), oldz = T(0), xi = T(0);
-        if (HAS_X) {
-          xi = xin[i];
-          if (FWD) {
-            if (hpx) oldx = xin[i + nyz];
-            if (hpy) oldy = xin[i + s.nz];
-            if (hpz) oldz = xin[i + 1];
-          } else {
-            if (hmx) oldx = xin[i - nyz];
-            if (hmy) oldy = xin[i - s.nz];
-            if (hmz) oldz = xin[i - 1];
-          }
-        }
-        const T vmx = FWD ? newx : oldx, vpx = FWD ? oldx : newx;
-        const T vmy = FWD ? newy : oldy, vpy = FWD ? oldy : newy;
-        const T vmz = FWD ? newz : oldz, vpz = FWD ? oldz : newz;
-        // accum = b_i - sum_{j != i} a_ij x~_j, fnmadd in iterate order (a_ij = -w)
-        T acc = bv[i];
-        acc = fma(r.wmx, vmx, acc);
-        acc = fma(r.wmy, vmy, acc);
-        acc = fma(r.wmz, vmz, acc);
-        acc = fma(r.wpz, vpz, acc);
-        acc = fma(r.wpy, vpy, acc);
-        acc = fma(r.wpx, vpx, acc);
-        T xn;
-        if (ULTRA) {
-          acc = fma(-r.d, xi, acc);
-          const T ad = r.d < T(0) ? -r.d : r.d;
-          xn = xi + (omega * acc) / (ad + T(0.5) * T(0));
-        } else {
-          const T quot = acc / r.d;
-          xn = fma(omega, quot, m1w * xi);
-        }
-        xout[i] = xn;
+    T nwx[K], nwy[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int64_t i = cell(z0 + j);
+      const int64_t lz = z0 + j;
+      const bool ok = line && lz >= 0 && lz < nz;
+      nwx[j] = (ok && a == 0 && has_px) ? xout[i - sx] : T(0);
+      nwy[j] = (ok && b == 0 && has_py) ? xout[i - sy] : T(0);
+    }
+    // ---- K steps from registers ----
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t lz = z0 + k;
+      const bool valid = line && lz >= 0 && lz < nz && t0 + k < nsteps;
+      // successor data of the neighbouring lanes (same logical z)
+      Raw csx = shfl_raw(cw[k + 2], lane + CB < 64 ? lane + CB : lane);
+      Raw csy = shfl_raw(cw[k + 2], lane + 1 < 64 ? lane + 1 : lane);
+      T xsx = __shfl(xw[k + 1], lane + CB < 64 ? lane + CB : lane, 64);
+      T xsy = __shfl(xw[k + 1], lane + 1 < 64 ? lane + 1 : lane, 64);
+      // predecessor results of the neighbouring lanes (one step ago)
+      T npx = __shfl(prev, lane >= CB ? lane - CB : lane, 64);
+      T npy = __shfl(prev, lane >= 1 ? lane - 1 : lane, 64);
+      T wpx_in = __shfl(wpx_prev, lane >= CB ? lane - CB : lane, 64);
+      T wpy_in = __shfl(wpy_prev, lane >= 1 ? lane - 1 : lane, 64);
+      if (a == CB - 1) {
+        csx = ecx[k + 1];
+        xsx = exx[k];
+      }
+      if (b == CB - 1) {
+        csy = ecy[k + 1];
+        xsy = exy[k];
+      }
+      const Raw cc = cw[k + 1];
+      // face weights: successor faces computed here, predecessor faces shared
+      const bool has_sz = lz + 1 < nz, has_pz = lz > 0;
+      const T w_sx = has_sx ? CI::conn(s, cc, csx, qx, 0) : T(0);
+      const T w_sy = has_sy ? CI::conn(s, cc, csy, qy, 1) : T(0);
+      const T w_sz = has_sz ? CI::conn(s, cc, cw[k + 2], qz, 2) : T(0);
+      T w_px, w_py;
+      if (a == 0) w_px = has_px ? CI::conn(s, cc, ucx[k + 1], qx, 0) : T(0);
+      else w_px = wpx_in;
+      if (b == 0) w_py = has_py ? CI::conn(s, cc, ucy[k + 1], qy, 1) : T(0);
+      else w_py = wpy_in;
+      const T w_pz = has_pz ? wpz_prev : T(0);
+      // predecessor (new) and successor (old) values
+      const T n_px = a == 0 ? nwx[k] : npx;
+      const T n_py = b == 0 ? nwy[k] : npy;
+      const T n_pz = has_pz ? prev : T(0);
+      const T o_sx = has_sx ? xsx : T(0);
+      const T o_sy = has_sy ? xsy : T(0);
+      const T o_sz = has_sz ? xw[k + 1] : T(0);
+      // physical orientation: forward, predecessors are -x/-y/-z
+      const T wmx = FWD ? w_px : w_sx, wpx = FWD ? w_sx : w_px;
+      const T wmy = FWD ? w_py : w_sy, wpy = FWD ? w_sy : w_py;
+      const T wmz = FWD ? w_pz : w_sz, wpz = FWD ? w_sz : w_pz;
+      const T vmx = FWD ? n_px : o_sx, vpx = FWD ? o_sx : n_px;
+      const T vmy = FWD ? n_py : o_sy, vpy = FWD ? o_sy : n_py;
+      const T vmz = FWD ? n_pz : o_sz, vpz = FWD ? o_sz : n_pz;
+      // diagonal in the reference's visiting order (algorithm.hpp:106-226)
+      const T ci = CI::value(s, cc);
+      T d = T(0);
+      if (x == 0) d += two_q0 * ci;
+      if (x + 1 == nx) d += two_q0 * ci;
+      d += wmx;
+      d += wmy;
+      d += wmz;
+      d += wpz;
+      d += wpy;
+      d += wpx;
+      if (s.zero_to_one && d == T(0)) d = T(1);
+      const T xi = xw[k];
+      T acc = bw[k];
+      acc = fma(wmx, vmx, acc);
+      acc = fma(wmy, vmy, acc);
+      acc = fma(wmz, vmz, acc);
+      acc = fma(wpz, vpz, acc);
+      acc = fma(wpy, vpy, acc);
+      acc = fma(wpx, vpx, acc);
+      T xn;
+      if (ULTRA) {
+        acc = fma(-d, xi, acc);
+        const T ad = d < T(0) ? -d : d;
+        xn = xi + (omega * acc) / (ad + T(0.5) * T(0));
+      } else {
+        const T quot = acc / d;
+        xn = fma(omega, quot, m1w * xi);
+      }
+      if (valid) {
+        xout[cell(lz)] = xn;
         prev = xn;
+        wpx_prev = w_sx;
+        wpy_prev = w_sy;
+        wpz_prev = w_sz;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -141,32 +225,32 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   }
 }
 
-template <typename T, int K, bool FWD, bool ULTRA>
+template <typename T, int KIND, bool FWD, bool ULTRA>
 void launch4(const StencilDesc& s, const void* x_in, void* x_out, const void* b, double omega,
              const SorSched& sched, hipStream_t st) {
   const int64_t cy = (s.ny + CB - 1) / CB;
   const dim3 grid(unsigned(sched.ncols)), block(64);
   if (x_in) {
-    hipLaunchKernelGGL((sor_wavefront<T, K, FWD, ULTRA, true>), grid, block, 0, st, s,
+    hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, true>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
                        static_cast<const T*>(b), T(omega), sched.state, sched.order, cy);
   } else {
-    hipLaunchKernelGGL((sor_wavefront<T, K, FWD, ULTRA, false>), grid, block, 0, st, s,
+    hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, false>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
                        static_cast<const T*>(b), T(omega), sched.state, sched.order, cy);
   }
   LGPU_HIP_CHECK(hipGetLastError());
 }
 
-template <typename T, int K>
+template <typename T, int KIND>
 void launch2(const StencilDesc& s, const void* x_in, void* x_out, const void* b, double omega,
              bool fwd, bool ultra, const SorSched& sched, hipStream_t st) {
   if (fwd) {
-    if (ultra) launch4<T, K, true, true>(s, x_in, x_out, b, omega, sched, st);
-    else launch4<T, K, true, false>(s, x_in, x_out, b, omega, sched, st);
+    if (ultra) launch4<T, KIND, true, true>(s, x_in, x_out, b, omega, sched, st);
+    else launch4<T, KIND, true, false>(s, x_in, x_out, b, omega, sched, st);
   } else {
-    if (ultra) launch4<T, K, false, true>(s, x_in, x_out, b, omega, sched, st);
-    else launch4<T, K, false, false>(s, x_in, x_out, b, omega, sched, st);
+    if (ultra) launch4<T, KIND, false, true>(s, x_in, x_out, b, omega, sched, st);
+    else launch4<T, KIND, false, false>(s, x_in, x_out, b, omega, sched, st);
   }
 }
 

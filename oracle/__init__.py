@@ -119,6 +119,7 @@ def lib():
             "or_amg_matrix": (cp, [hp, C.c_int]),
             "or_amg_cholesky": (cp, [hp]),
             "or_amg_stats": (None, [hp, C.c_int, i64p]),
+            "or_amg_set_coarse_segment": (None, [hp, C.c_int64]),
             "or_cholesky_solve": (None, [cp, f64p, f64p]),
             "or_cholesky_decompose": (C.c_int64, [cp, i64p, i64p, f64p]),
             "or_vcycle_f64": (None, [hp, f64p, f64p, C.c_int]),
@@ -294,10 +295,11 @@ def default_params(coarse_f32=False):
 
 
 class Hierarchy:
-    def __init__(self, A, params: AmgParams | None = None, coarse_f32=False):
+    def __init__(self, A, params: AmgParams | None = None, coarse_f32=False, coarse_segment=0):
         self.A = A  # keep the finest operator alive
         self.params = params or default_params(coarse_f32)
         self.h = lib().or_amg_setup(C.byref(A.mat), C.byref(self.params))
+        lib().or_amg_set_coarse_segment(self.h, int(coarse_segment))
 
     def __del__(self):
         if getattr(self, "h", None):

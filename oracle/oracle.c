@@ -44,7 +44,24 @@ struct or_hierarchy {
   or_csr mats[32];       /* mats[l] for l >= 1 */
   int64_t stats[32][5];
   or_csr chol;
+  int64_t coarse_seg; /* rows per smoother segment on levels >= 1 (0: one segment) */
 };
+
+/* Thread segments of level l's smoother (parallel/execution-policy/adaptive.hpp
+ * segments rows into contiguous ranges; gauss-seidel-variant/sweep.hpp:153-169
+ * uses new values only inside the row's own segment). */
+static int64_t* or_level_segments(const or_hierarchy* h, int l, int* nseg) {
+  const int64_t n = h->size[l];
+  const int64_t len = (l == 0 || h->coarse_seg <= 0) ? n : h->coarse_seg;
+  const int64_t k = (n + len - 1) / len;
+  int64_t* seg = (int64_t*)malloc(sizeof(int64_t) * (size_t)(k + 1));
+  for (int64_t s = 0; s < k; ++s) seg[s] = s * len;
+  seg[k] = n;
+  *nseg = (int)k;
+  return seg;
+}
+
+void or_amg_set_coarse_segment(or_hierarchy* h, int64_t rows) { h->coarse_seg = rows; }
 
 /* discretization/box-equidistant.hpp:62-79: h_d = L_d / n_d,
  * q_d = (prod_{e != d} h_e) / h_d with the product left-folded from 1 */

@@ -132,6 +132,9 @@ const or_csr* or_amg_cholesky(const or_hierarchy* h);
  * merged, isolated */
 void or_amg_stats(const or_hierarchy* h, int level, int64_t out[5]);
 
+/* rows per thread segment of the smoothers on levels >= 1 (0: one segment) */
+void or_amg_set_coarse_segment(or_hierarchy* h, int64_t rows);
+
 /* coarsest solve x = (L L^T)^{-1} b in f64 (direct/cholesky/solver.hpp:42-51) */
 void or_cholesky_solve(const or_csr* lower, const double* b, double* x);
 /* generic lower Cholesky (direct/cholesky/decomposition.hpp:44-99): output arrays

@@ -100,3 +100,8 @@ def test_csr_kernels(dims, dtype):
                 out = G.sor(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype), la.DenseVector(b),
                             1.0, forward, ultra)
                 same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.0), forward, ultra))
+                for seg in (1, 7, 32, 64):
+                    out = G.sor(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype),
+                                la.DenseVector(b), 1.1, forward, ultra, segment_rows=seg)
+                    bounds = list(range(0, n, seg)) + [n]
+                    same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.1), forward, ultra, bounds))
