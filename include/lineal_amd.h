@@ -142,7 +142,7 @@ typedef struct {
                                    smoothers and level operators, FP64 coarsest solve */
   int ultra;                    /* SOR variant of the smoothers */
   int64_t coarse_segment_rows;  /* coarse-level (CSR) smoothers: rows per thread segment
-                                   (1..64, default 32); 0 sweeps each coarse level in one
+                                   (1..64, default 64); 0 sweeps each coarse level in one
                                    segment (lexicographic). The finest level is always
                                    swept lexicographically. */
 } lgpu_amg_params;

@@ -454,7 +454,7 @@ class MultiGridSolver:
     """
 
     def __init__(self, wright: HierarchyWrightParams | None = None, precision=np.float64, ultra=False,
-                 coarse_segment_rows=32):
+                 coarse_segment_rows=64):
         self.wright = wright or HierarchyWrightParams()
         self.precision = precision
         self.ultra = ultra

@@ -816,7 +816,7 @@ extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {
   p->max_aggregate_size = 9;
   p->precision = LGPU_F64;
   p->ultra = 0;
-  p->coarse_segment_rows = 32;
+  p->coarse_segment_rows = 64;
 }
 
 extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out) {

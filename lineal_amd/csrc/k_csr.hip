@@ -190,6 +190,119 @@ __global__ __launch_bounds__(SEG_NT) void csr_sor_segment_kernel(
   }
 }
 
+// The same thread-segment SOR with segments of exactly 64 rows, one wavefront
+// per segment and one lane per row.  The segment's entries (columns, values
+// and the old values they refer to) are staged in LDS with coalesced loads;
+// then every lane walks its row in column order and stops at the first entry
+// that needs a new value from a row of its own segment that is not finished
+// yet.  A ballot publishes the finished rows after every pass, so the passes
+// resolve the segment's dependency chain while each row still accumulates in
+// exactly the sequential order of the reference.
+constexpr int WSEG = 64;
+constexpr int WSEG_WAVES = 2;
+constexpr int WSEG_CAP = 1024;  // staged entries per segment
+
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
+__global__ __launch_bounds__(WSEG * WSEG_WAVES) void csr_sor_wave_kernel(
+    int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const V* __restrict__ va, const T* __restrict__ xin, T* __restrict__ xout,
+    const T* __restrict__ b, T omega) {
+  __shared__ int32_t s_col[WSEG_WAVES][WSEG_CAP];
+  __shared__ V s_val[WSEG_WAVES][WSEG_CAP];
+  __shared__ T s_old[WSEG_WAVES][WSEG_CAP];
+  __shared__ T s_new[WSEG_WAVES][WSEG];
+  const int lane = threadIdx.x % WSEG, w = threadIdx.x / WSEG;
+  const int64_t s0 = (blockIdx.x * int64_t(WSEG_WAVES) + w) * WSEG;
+  if (s0 >= n) return;
+  const int64_t s1 = min(s0 + WSEG, n);
+  const int64_t e0 = rp[s0], e1 = rp[s1];
+  const bool staged = e1 - e0 <= WSEG_CAP;
+  if (staged) {
+    for (int64_t k = e0 + lane; k < e1; k += WSEG) {
+      const int32_t j = ci[k];
+      s_col[w][k - e0] = j;
+      s_val[w][k - e0] = va[k];
+      s_old[w][k - e0] = HAS_X ? xin[j] : T(0);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int64_t i = s0 + lane;
+  bool finished = i >= s1;
+  int64_t k = 0, kend = 0;
+  T acc = T(0), d = T(1), div = T(0), xi = T(0);
+  if (!finished) {
+    k = rp[i] - e0;
+    kend = rp[i + 1] - e0;
+    acc = b[i];
+    xi = HAS_X ? xin[i] : T(0);
+  }
+  const T m1w = T(1) - omega;
+  uint64_t done = __ballot(finished);
+  while (done != ~uint64_t(0)) {
+    if (!finished) {
+      while (k < kend) {
+        const int64_t j = staged ? s_col[w][k] : ci[e0 + k];
+        const T a = T(staged ? s_val[w][k] : va[e0 + k]);
+        if (j == i) {
+          d = a;
+          if (ULTRA) acc = fma(-a, xi, acc);
+          ++k;
+          continue;
+        }
+        const bool inside = j >= s0 && j < s1;
+        T xj;
+        if (inside && (FWD ? j < i : j > i)) {
+          if (!((done >> (j - s0)) & 1u)) break;  // wait for that row
+          xj = s_new[w][j - s0];
+        } else {
+          xj = staged ? s_old[w][k] : (HAS_X ? xin[j] : T(0));
+          if (ULTRA && !inside) div += a < T(0) ? -a : a;
+        }
+        acc = fma(-a, xj, acc);
+        ++k;
+      }
+      if (k == kend) {
+        T xn;
+        if (ULTRA) {
+          const T ad = d < T(0) ? -d : d;
+          xn = xi + (omega * acc) / (ad + T(0.5) * div);
+        } else {
+          const T quot = acc / d;
+          xn = fma(omega, quot, m1w * xi);
+        }
+        s_new[w][lane] = xn;
+        xout[i] = xn;
+        finished = true;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    done = __ballot(finished);
+  }
+}
+
+template <typename T, typename V, bool FWD, bool ULTRA>
+void launch_wave_segment(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
+                         double omega, hipStream_t st) {
+  const int64_t nseg = (a.n + WSEG - 1) / WSEG;
+  const int64_t blocks = (nseg + WSEG_WAVES - 1) / WSEG_WAVES;
+  if (x_in) {
+    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, true>), dim3(blocks),
+                       dim3(WSEG * WSEG_WAVES), 0, st, a.n, a.rowptr, a.cols,
+                       static_cast<const V*>(a.vals), static_cast<const T*>(x_in),
+                       static_cast<T*>(x_out), static_cast<const T*>(b), T(omega));
+  } else {
+    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, false>), dim3(blocks),
+                       dim3(WSEG * WSEG_WAVES), 0, st, a.n, a.rowptr, a.cols,
+                       static_cast<const V*>(a.vals), static_cast<const T*>(x_in),
+                       static_cast<T*>(x_out), static_cast<const T*>(b), T(omega));
+  }
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T, typename V, bool FWD, bool ULTRA>
 void launch_segment(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
                     int seg, hipStream_t st) {
@@ -213,6 +326,16 @@ void launch_segment(const CsrDesc& a, const void* x_in, void* x_out, const void*
 template <typename T, typename V>
 void dispatch_segment(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
                       double omega, bool fwd, bool ultra, int seg, hipStream_t st) {
+  if (seg == WSEG) {
+    if (fwd) {
+      if (ultra) launch_wave_segment<T, V, true, true>(a, x_in, x_out, b, omega, st);
+      else launch_wave_segment<T, V, true, false>(a, x_in, x_out, b, omega, st);
+    } else {
+      if (ultra) launch_wave_segment<T, V, false, true>(a, x_in, x_out, b, omega, st);
+      else launch_wave_segment<T, V, false, false>(a, x_in, x_out, b, omega, st);
+    }
+    return;
+  }
   if (fwd) {
     if (ultra) launch_segment<T, V, true, true>(a, x_in, x_out, b, omega, seg, st);
     else launch_segment<T, V, true, false>(a, x_in, x_out, b, omega, seg, st);

@@ -31,7 +31,7 @@ CASES = [((4, 9, 7), "lookup"), ((24, 20, 16), "values"), ((32, 32, 32), "values
 @pytest.mark.parametrize("f32", [False, True])
 def test_hierarchy_equals_oracle(dims, kind, f32):
     A, S = pair(dims, kind)
-    H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=32)
+    H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=64)
     M = la.MultiGridSolver(precision=np.float32 if f32 else np.float64).instantiate(A)
     assert M.level_num == H.level_num
     for l in range(H.level_num):
@@ -51,7 +51,7 @@ def test_hierarchy_equals_oracle(dims, kind, f32):
 def test_vcycle_equals_oracle(dims, kind, ultra):
     A, S = pair(dims, kind)
     for f32 in (False, True):
-        H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=32)
+        H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=64)
         M = la.MultiGridSolver(precision=np.float32 if f32 else np.float64, ultra=ultra).instantiate(A)
         r = problems.uniform(S.N, 5)
         if f32:
@@ -96,7 +96,7 @@ def test_reference_pcg_amg_thresholds():
 def test_pcg_amg_matches_oracle(dims, kind, precision):
     A, S = pair(dims, kind)
     f32 = precision == np.float32
-    H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=32)
+    H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=64)
     M = la.MultiGridSolver(precision=precision).instantiate(A)
     b = S.rhs()
     x_ref, it_ref, hist_ref = O.pcg(S, b, H, precon=2 if f32 else 1, rel_tol=1e-8, max_it=200)
@@ -114,7 +114,7 @@ def test_pcg_amg_matches_oracle(dims, kind, precision):
 @pytest.mark.parametrize("dims,kind", CASES[1:3])
 def test_bicgstab_amg_matches_oracle(dims, kind):
     A, S = pair(dims, kind)
-    H = O.Hierarchy(S, coarse_segment=32)
+    H = O.Hierarchy(S, coarse_segment=64)
     M = la.MultiGridSolver().instantiate(A)
     b = S.rhs()
     x_ref, it_ref, _ = O.pbicgstab(S, b, H, precon=1, rel_tol=1e-8, max_it=200)
