@@ -26,6 +26,7 @@ otherwise.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 from dataclasses import dataclass
@@ -33,7 +34,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblineal_amd.so")
+# LINEAL_AMD_LIBRARY selects another build of the same ABI (e.g. a profiling build)
+LIB_PATH = os.environ.get("LINEAL_AMD_LIBRARY") or os.path.join(_HERE, "liblineal_amd.so")
 
 F64, F32 = 0, 1
 OK, INVALID_ARG, OUT_OF_MEMORY, HIP_ERROR, BREAKDOWN, NONFINITE, NOT_CONVERGED, NOT_SUPPORTED, SYNC_TIMEOUT = range(9)
@@ -127,6 +129,15 @@ SIGNATURES = {
 }
 
 _lib = None
+_finalizing = False
+
+
+@atexit.register
+def _mark_finalizing():
+    # objects collected during interpreter shutdown skip their destroy calls
+    # (the process exit releases the device memory)
+    global _finalizing
+    _finalizing = True
 
 
 def lib():
@@ -180,9 +191,9 @@ class Context:
         return ms.value
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not _finalizing:
             lib().lgpu_ctx_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -252,7 +263,7 @@ class DenseVector:
 
     def __del__(self):
         try:
-            if getattr(self, "h", None):
+            if getattr(self, "h", None) and not _finalizing:
                 lib().lgpu_vec_destroy(self.h)
                 self.h = None
         except Exception:
@@ -304,7 +315,7 @@ class _Matrix:
 
     def __del__(self):
         try:
-            if getattr(self, "h", None):
+            if getattr(self, "h", None) and not _finalizing:
                 lib().lgpu_op_destroy(self.h)
                 self.h = None
         except Exception:
@@ -512,7 +523,7 @@ class MultiGridInstance:
 
     def __del__(self):
         try:
-            if getattr(self, "h", None):
+            if getattr(self, "h", None) and not _finalizing:
                 lib().lgpu_amg_destroy(self.h)
                 self.h = None
         except Exception:

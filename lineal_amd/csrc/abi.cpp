@@ -116,7 +116,7 @@ struct lgpu_op_s {
   DevBuf c64, c32, mat;
   std::vector<double> host_coeff;
   std::vector<uint8_t> host_mat;
-  DevBuf sor_state, sor_order;
+  DevBuf sor_state, sor_order, sor_mailbox;
   int64_t sor_cols = 0;
   // csr
   CsrDev csr;
@@ -198,6 +198,7 @@ void init_sor_sched(lgpu_op op) {
   const size_t ints = sor_sched_ints(op->desc);
   op->sor_state.alloc(ints * sizeof(int));
   LGPU_HIP_CHECK(hipMemset(op->sor_state.ptr, 0, ints * sizeof(int)));
+  op->sor_mailbox.alloc(sor_mailbox_bytes(op->desc));
 }
 
 // Host evaluation of the stencil rows in f64 in iterate order (-x,-y,-z,+z,+y,+x,
@@ -304,6 +305,7 @@ void op_sor(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, 
     s.state = op->sor_state.as<int>();
     s.order = op->sor_order.as<uint32_t>();
     s.ncols = op->sor_cols;
+    s.mailbox = op->sor_mailbox.ptr;
     stencil_sor(op->desc, f32, x_in, x_out, b, omega, fwd, ultra, s, st);
   }
 }
@@ -514,12 +516,11 @@ extern "C" int lgpu_vec_wrap(lgpu_ctx ctx, void* device_ptr, int64_t n, int dtyp
   API_END
 }
 
+// Destroying an object does not touch its context (which may already be
+// gone): hipFree waits for the device work that still uses the memory.
 extern "C" int lgpu_vec_destroy(lgpu_vec v) {
   API_BEGIN
-  if (v) {
-    (void)hipStreamSynchronize(v->ctx->stream);
-    delete v;
-  }
+  delete v;
   API_END
 }
 
@@ -710,10 +711,7 @@ extern "C" int lgpu_csr_from_stencil(lgpu_op stencil, int dtype, lgpu_op* out) {
 
 extern "C" int lgpu_op_destroy(lgpu_op op) {
   API_BEGIN
-  if (op) {
-    (void)hipStreamSynchronize(op->ctx->stream);
-    delete op;
-  }
+  delete op;
   API_END
 }
 
@@ -891,10 +889,7 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
 
 extern "C" int lgpu_amg_destroy(lgpu_amg amg) {
   API_BEGIN
-  if (amg) {
-    (void)hipStreamSynchronize(amg->ctx->stream);
-    delete amg;
-  }
+  delete amg;
   API_END
 }
 

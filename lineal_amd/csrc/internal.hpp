@@ -117,11 +117,13 @@ void stencil_rhs(const StencilDesc& s, bool f32, void* b, hipStream_t st);
 
 // lexicographic SOR sweep on the stencil (tile-column wavefront)
 struct SorSched {
-  int* state = nullptr;        // [0] ticket, [1] error, [2..] column progress
+  int* state = nullptr;        // [0] ticket, [1] error (sticky)
   const uint32_t* order = nullptr;  // ticket -> column id (forward topological order)
   int64_t ncols = 0;
+  void* mailbox = nullptr;     // border-line values passed between columns
 };
 size_t sor_sched_ints(const StencilDesc& s);
+size_t sor_mailbox_bytes(const StencilDesc& s);
 void sor_sched_order(const StencilDesc& s, std::vector<uint32_t>& order);
 void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, const void* b,
                  double omega, bool forward, bool ultra, const SorSched& sched, hipStream_t st);

@@ -15,11 +15,14 @@
 // coefficient, right-hand-side and old-value windows of its own line for the
 // chunk's K cells (lanes on the column border also load the neighbouring
 // column's lines), so the K steps run from registers and shuffles with one
-// memory round trip per chunk.  Border lines then read the new values of the
-// upstream columns (I-1, J) and (I, J-1) after waiting on their progress
-// counters.  Columns are claimed through an atomic ticket in diagonal (I + J)
-// order, so a column only waits on columns already claimed by running
-// wavefronts: every wait ends (waits are also bounded and report a timeout).
+// memory round trip per chunk.  The lines on a column's downstream borders
+// publish every new value into a mailbox (a word initialised to a sentinel NaN
+// and written once with a single-copy-atomic store); the border lines of the
+// downstream columns (I+1, J) and (I, J+1) poll those words, so no memory
+// fence and no separate progress flag is needed.  Columns are claimed through
+// an atomic ticket in diagonal (I + J) order, so a column only waits on
+// columns already claimed by running wavefronts: every wait ends (waits are
+// also bounded and report a timeout).
 // The backward sweep is the same schedule in reflected coordinates.  Each cell
 // performs exactly the oracle's operations in the oracle's order, so the
 // result equals one-segment lexicographic SOR bit for bit.
@@ -30,20 +33,58 @@ namespace lgpu {
 namespace {
 
 constexpr int CB = 8;        // lines per column side
-constexpr int K = 8;         // steps per chunk (one progress publication each)
+// steps per chunk (one window load each): 16 for f32, 8 for f64 (register budget)
+template <typename T>
+constexpr int kChunk = sizeof(T) == 4 ? 16 : 8;
 constexpr int SPIN_LIMIT = 1 << 24;
 
-__device__ __forceinline__ void wait_progress(int* state, int64_t col, int need) {
-  int* progress = state + 2;
+// Mailbox words start as a NaN with a payload that no arithmetic produces
+// (arithmetic NaNs are the canonical quiet NaN or carry an input's payload);
+// a word is published with one single-copy-atomic store, so a reader that
+// sees anything else sees the final value and needs no fence.
+template <typename T>
+struct Sentinel;
+template <>
+struct Sentinel<float> {
+  using Bits = uint32_t;
+  static constexpr Bits bits = 0x7FBADBADu;
+};
+template <>
+struct Sentinel<double> {
+  using Bits = uint64_t;
+  static constexpr Bits bits = 0x7FF8BADBADBADBADull;
+};
+
+template <typename T>
+__device__ __forceinline__ bool is_sentinel(T v) {
+  return __builtin_bit_cast(typename Sentinel<T>::Bits, v) == Sentinel<T>::bits;
+}
+
+// bounded wait for a mailbox word; an expired wait sets the sticky error word
+// and every later wait returns at once, so the sweep always ends
+template <typename T>
+__device__ __forceinline__ T mailbox_poll(const T* p, int* state) {
+  T v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   int it = 0;
-  while (__hip_atomic_load(progress + col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-    if (__hip_atomic_load(state + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  while (is_sentinel(v)) {
+    if (__hip_atomic_load(state + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
     __builtin_amdgcn_s_sleep(1);
     if (++it > SPIN_LIMIT) {
       __hip_atomic_store(state + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return;
+      break;
     }
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  return v;
+}
+
+template <typename T>
+__global__ void mailbox_reset(T* box, int64_t n) {
+  using Bits = typename Sentinel<T>::Bits;
+  Bits* p = reinterpret_cast<Bits*>(box);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = Sentinel<T>::bits;
 }
 
 template <typename R>
@@ -57,9 +98,10 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
                                                     T* __restrict__ xout,
                                                     const T* __restrict__ bv, T omega,
                                                     int* state, const uint32_t* order,
-                                                    int64_t cy) {
+                                                    int64_t cy, T* mbox) {
   using CI = CellInfo<T, KIND>;
   using Raw = typename CI::Raw;
+  constexpr int K = kChunk<T>;
   const int lane = threadIdx.x;
   const int a = lane / CB, b = lane % CB;
   __shared__ int s_ticket;
@@ -84,7 +126,13 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   const T m1w = T(1) - omega;
   const T qx = T(s.q[0]), qy = T(s.q[1]), qz = T(s.q[2]);
   const T two_q0 = T(2) * qx;
-  int* progress = state + 2;
+  // mailboxes: [X part: column, b, z] the a = 7 lines; [Y part: column, a, z] the b = 7 lines
+  const int64_t ncols = ((nx + CB - 1) / CB) * cy;
+  T* const ybox = mbox + ncols * CB * nz;
+  T* const out_x = mbox + (col * CB + b) * nz;
+  T* const out_y = ybox + (col * CB + a) * nz;
+  const T* const in_x = mbox + (((I > 0 ? I - 1 : 0) * cy + J) * CB + b) * nz;
+  const T* const in_y = ybox + ((I * cy + (J > 0 ? J - 1 : 0)) * CB + a) * nz;
   // physical cell index of logical z on this line (clamped into the line)
   auto cell = [&](int64_t lz) {
     const int64_t c = lz < 0 ? 0 : (lz >= nz ? nz - 1 : lz);
@@ -114,29 +162,25 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) bw[j] = bv[cell(z0 + j)];
-    // ---- wait for the upstream columns, then read their new border values ----
-    const int64_t tend = min(t0 + K, nsteps);
-    const int need = int(min(tend + CB - 1, nsteps));
-    if (lane == 0) {
-      if (I > 0) wait_progress(state, (I - 1) * cy + J, need);
-      if (J > 0) wait_progress(state, I * cy + J - 1, need);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __builtin_amdgcn_wave_barrier();
+    // ---- new border values of the upstream columns: prefetched from their
+    // mailboxes now, re-polled at the step that needs them if still unset ----
     T nwx[K], nwy[K];
+    const bool take_x = line && a == 0 && has_px;
+    const bool take_y = line && b == 0 && has_py;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const int64_t i = cell(z0 + j);
       const int64_t lz = z0 + j;
-      const bool ok = line && lz >= 0 && lz < nz;
-      nwx[j] = (ok && a == 0 && has_px) ? xout[i - sx] : T(0);
-      nwy[j] = (ok && b == 0 && has_py) ? xout[i - sy] : T(0);
+      const bool ok = lz >= 0 && lz < nz;
+      nwx[j] = (take_x && ok) ? __hip_atomic_load(in_x + lz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : T(0);
+      nwy[j] = (take_y && ok) ? __hip_atomic_load(in_y + lz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : T(0);
     }
     // ---- K steps from registers ----
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t lz = z0 + k;
       const bool valid = line && lz >= 0 && lz < nz && t0 + k < nsteps;
+      if (valid && take_x && is_sentinel(nwx[k])) nwx[k] = mailbox_poll(in_x + lz, state);
+      if (valid && take_y && is_sentinel(nwy[k])) nwy[k] = mailbox_poll(in_y + lz, state);
       // successor data of the neighbouring lanes (same logical z)
       Raw csx = shfl_raw(cw[k + 2], lane + CB < 64 ? lane + CB : lane);
       Raw csy = shfl_raw(cw[k + 2], lane + 1 < 64 ? lane + 1 : lane);
@@ -212,16 +256,15 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
       }
       if (valid) {
         xout[cell(lz)] = xn;
+        // border lines publish their values to the downstream columns
+        if (a == CB - 1) __hip_atomic_store(out_x + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (b == CB - 1) __hip_atomic_store(out_y + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         prev = xn;
         wpx_prev = w_sx;
         wpy_prev = w_sy;
         wpz_prev = w_sz;
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0)
-      __hip_atomic_store(progress + col, int(tend), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -233,11 +276,13 @@ void launch4(const StencilDesc& s, const void* x_in, void* x_out, const void* b,
   if (x_in) {
     hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, true>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched.state, sched.order, cy);
+                       static_cast<const T*>(b), T(omega), sched.state, sched.order, cy,
+                       static_cast<T*>(sched.mailbox));
   } else {
     hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, false>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched.state, sched.order, cy);
+                       static_cast<const T*>(b), T(omega), sched.state, sched.order, cy,
+                       static_cast<T*>(sched.mailbox));
   }
   LGPU_HIP_CHECK(hipGetLastError());
 }
@@ -256,9 +301,13 @@ void launch2(const StencilDesc& s, const void* x_in, void* x_out, const void* b,
 
 }  // namespace
 
-size_t sor_sched_ints(const StencilDesc& s) {
+size_t sor_sched_ints(const StencilDesc&) { return 2; }
+
+// two mailbox lines (the a = 7 and b = 7 border lines) of 8 z-lines per column,
+// sized for f64 words
+size_t sor_mailbox_bytes(const StencilDesc& s) {
   const int64_t cx = (s.nx + CB - 1) / CB, cy = (s.ny + CB - 1) / CB;
-  return size_t(2 + cx * cy);
+  return size_t(2 * cx * cy * CB * s.nz) * sizeof(double);
 }
 
 // tickets in diagonal order: columns sorted by I + J, then by I
@@ -279,7 +328,12 @@ void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, 
   // reset the ticket and the progress counters; state[1] (the error word) is
   // sticky until the host clears it
   vec_fill_int(sched.state, 1, 0, st);
-  vec_fill_int(sched.state + 2, sched.ncols, 0, st);
+  const int64_t words = int64_t(sor_mailbox_bytes(s) / sizeof(double));
+  const int64_t fw = f32 ? 2 * words : words;
+  const dim3 rg(unsigned(std::min<int64_t>((fw + 255) / 256, 4096))), rb(256);
+  if (f32) hipLaunchKernelGGL(mailbox_reset<float>, rg, rb, 0, st, static_cast<float*>(sched.mailbox), fw);
+  else hipLaunchKernelGGL(mailbox_reset<double>, rg, rb, 0, st, static_cast<double*>(sched.mailbox), fw);
+  LGPU_HIP_CHECK(hipGetLastError());
   if (f32) {
     if (s.kind == 0) launch2<float, 0>(s, x_in, x_out, b, omega, forward, ultra, sched, st);
     else launch2<float, 1>(s, x_in, x_out, b, omega, forward, ultra, sched, st);
