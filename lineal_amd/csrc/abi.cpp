@@ -601,6 +601,17 @@ extern "C" int lgpu_stencil_values_create(lgpu_ctx ctx, const lgpu_box* box, con
   op->desc.kind = 0;
   const int64_t n = op->rows();
   op->host_coeff.assign(coeff, coeff + n);
+  {
+    bool f64ok = true, f32ok = true;
+    for (int64_t i = 0; i < n; ++i) {
+      const double c = coeff[i];
+      const float c32 = float(c);
+      f64ok = f64ok && ((c == 0.0 && !std::signbit(c)) || (c >= 0x1p-100 && c <= 0x1p100));
+      f32ok = f32ok && ((c32 == 0.0f && !std::signbit(c32)) || (c32 >= 0x1p-24f && c32 <= 0x1p24f));
+    }
+    op->desc.fast_div64 = f64ok;
+    op->desc.fast_div32 = f32ok;
+  }
   op->c64.alloc(size_t(n) * sizeof(double));
   LGPU_HIP_CHECK(hipMemcpy(op->c64.ptr, coeff, size_t(n) * sizeof(double), hipMemcpyHostToDevice));
   op->desc.c64 = op->c64.as<double>();

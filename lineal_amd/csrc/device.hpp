@@ -15,6 +15,54 @@ __device__ __forceinline__ T hmean(T a, T b) {
   return p != T(0) ? (p + p) / (a + b) : T(0);
 }
 
+// n / d rounded to nearest, for operands that need no range scaling: a
+// reciprocal seed refined by Newton-Raphson steps, then Markstein's remainder
+// correction(s) (P. Markstein, IBM J. Res. Dev. 34(1), 1990).  These are the
+// operations of gfx9's IEEE division sequence once its v_div_scale steps leave
+// the operands unchanged (exponent gap below 768 / 96, no denormal operand,
+// quotient or reciprocal) and its v_div_fixup has no special case to patch,
+// so in that range the result equals IEEE division bit for bit.  Callers
+// guarantee the range: f64 numerators and denominators within [2^-300, 2^300],
+// f32 within [2^-50, 2^50].
+__device__ __forceinline__ double div_rn_scaled_free(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  const double q = n * r;
+  const double rem = fma(-d, q, n);
+  return fma(rem, r, q);
+}
+__device__ __forceinline__ float div_rn_scaled_free(float n, float d) {
+  const float r0 = __builtin_amdgcn_rcpf(d);
+  const float e = fmaf(-d, r0, 1.0f);
+  const float r = fmaf(e, r0, r0);
+  const float q0 = n * r;
+  const float rem0 = fmaf(-d, q0, n);
+  const float q1 = fmaf(rem0, r, q0);
+  const float rem1 = fmaf(-d, q1, n);
+  return fmaf(rem1, r, q1);
+}
+
+// Harmonic mean evaluated without a branch.  FAST: both coefficients are +0
+// or lie in [2^-100, 2^100] (f64) / [2^-24, 2^24] (f32), see
+// StencilDesc::fast_div64 / fast_div32.  Then ab = 0 exactly when a or b is 0,
+// a + b is 0 only when both are, and clamping the denominator to the range's
+// lower end changes nothing else: 0 / max(a + b, lo) is the +0 the reference
+// returns for ab = 0, and every other quotient is the IEEE one.
+template <typename T, bool FAST>
+__device__ __forceinline__ T hmean_sel(T a, T b) {
+  const T p = a * b;
+  if constexpr (FAST) {
+    constexpr T lo = sizeof(T) == 8 ? T(0x1p-100) : T(0x1p-24);
+    return div_rn_scaled_free(p + p, fmax(a + b, lo));
+  } else {
+    const T q = (p + p) / (a + b);
+    return p != T(0) ? q : T(0);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ const T* coeff_ptr(const StencilDesc& s);
 template <>

@@ -86,6 +86,10 @@ struct StencilDesc {
   double mc[4];          // kind 1 material coefficients
   double tab64[3][16];   // kind 1 connection tables H(C[m],C[m'])*q_d in f64
   float tab32[3][16];    //                               in f32
+  // kind 0: every coefficient is +0 or within [2^-100, 2^100] (f64 values) /
+  // [2^-24, 2^24] (their f32 roundings), so harmonic means may use the
+  // scaling-free division
+  int fast_div64, fast_div32;
 };
 
 // kernel modes of the matrix-free stencil pass

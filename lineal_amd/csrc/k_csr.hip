@@ -41,8 +41,23 @@ __global__ __launch_bounds__(NT) void csr_kernel(int64_t n, const int64_t* __res
       const T quot = acc / d;
       y[i] = fma(omega, quot, (T(1) - omega) * (x ? x[i] : T(0)));
     } else {
+      // entries in groups of 8: the index, value and vector loads of a group
+      // are independent, so they are in flight together; the sum still runs
+      // in column order
       T sum = T(0);
-      for (int64_t k = k0; k < k1; ++k) sum = fma(T(va[k]), x[ci[k]], sum);
+      for (int64_t k = k0; k < k1; k += 8) {
+        T av[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const bool ok = k + u < k1;
+          const int32_t j = ok ? ci[k + u] : 0;
+          av[u] = ok ? T(va[k + u]) : T(0);
+          xv[u] = ok ? x[j] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k + u < k1) sum = fma(av[u], xv[u], sum);
+      }
       if constexpr (MODE == SM_SPMV) {
         y[i] = sum;
       } else if constexpr (MODE == SM_SPMV_DOT) {

@@ -13,7 +13,7 @@ from lineal_amd import problems
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(4, 9, 7), (1, 1, 1), (3, 1, 70), (13, 17, 5), (33, 17, 65), (40, 40, 40)]
+SHAPES = [(4, 9, 7), (1, 1, 1), (3, 1, 70), (13, 17, 5), (33, 17, 65), (40, 40, 40), (6, 21, 130)]
 
 
 def make_pair(dims, kind, seed=1, sigma=1.0, p_start=1.0, p_end=0.0, mat_coeffs=(0.0, 1.0, 2.0, 3.0)):
@@ -77,6 +77,43 @@ def test_sor_wavefront(dims, kind, dtype, forward, ultra):
                     omega, forward, ultra)
         ref = O.sor(S, x if with_x else None, b, dtype(omega), forward, ultra)
         same(out.numpy(), ref)
+
+
+def special_coeffs(dims, which, seed=5):
+    """Lognormal coefficients with 30% of the cells replaced by range edges.
+
+    "edges": +0 and the ends of the scaling-free division range (f64 and f32);
+    "wide": values outside that range (exact IEEE division path);
+    "negzero": -0 and +0 cells (a -0 coefficient also selects the exact path).
+    """
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    c = np.exp(rng.normal(0.0, 1.0, n))
+    vals = {"edges": [0.0, 2.0 ** -100, 2.0 ** 100, 2.0 ** -24, 2.0 ** 24],
+            "wide": [0.0, 2.0 ** -200, 2.0 ** 60, 1e-300],
+            "negzero": [-0.0, 0.0]}[which]
+    m = rng.random(n) < 0.3
+    c[m] = np.asarray(vals)[rng.integers(0, len(vals), int(m.sum()))]
+    return c
+
+
+@pytest.mark.parametrize("dims", [(13, 17, 5), (33, 17, 65), (5, 22, 130)])
+@pytest.mark.parametrize("which", ["edges", "wide", "negzero"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_values_coefficient_ranges(dims, which, dtype):
+    c = special_coeffs(dims, which)
+    for zero_to_one in (True, False):
+        box = la.BoxInfo(dims, p_start=1.0, p_end=0.0, zero_to_one=zero_to_one)
+        A = la.StencilMatrix.values(box, c)
+        S = O.Stencil(dims, coeff=c, p_start=1.0, p_end=0.0, zero_to_one=zero_to_one)
+        n = S.N
+        x = problems.uniform(n, 2, dtype)
+        b = problems.uniform(n, 3, dtype)
+        xd = la.DenseVector(x)
+        same(A.spmv(xd, la.DenseVector(n, dtype)).numpy(), O.spmv(S, x))
+        same(A.residual(la.DenseVector(b), xd, la.DenseVector(n, dtype)).numpy(), O.residual(S, b, x))
+        same(A.jacobi(xd, la.DenseVector(n, dtype), la.DenseVector(b), 0.7).numpy(),
+             O.jacobi(S, x, b, dtype(0.7)))
 
 
 @pytest.mark.parametrize("dims", [(4, 9, 7), (13, 17, 5), (33, 17, 65)])
