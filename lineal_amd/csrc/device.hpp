@@ -174,6 +174,22 @@ __device__ __forceinline__ double block_sum(double v) {
   return t;
 }
 
+// Block sum for any block of up to 1024 threads (whole wavefronts); thread 0
+// receives the total.
+__device__ __forceinline__ double block_sum_dyn(double v) {
+  __shared__ double red[16];
+  v = wave_sum(v);
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0) {
+    const int nw = int(blockDim.x / 64);
+    for (int w = 0; w < nw; ++w) t += red[w];
+  }
+  return t;
+}
+
 // Map the hardware dispatch index to a logical tile index such that each of
 // the 8 XCDs (which receive workgroups round robin) processes a contiguous
 // range of logical tiles and so shares halo data through its own L2.
