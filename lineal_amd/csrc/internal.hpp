@@ -108,6 +108,7 @@ struct CsrDesc {
   const int32_t* cols;
   const void* vals;    // f64 or f32
   int f32;
+  int64_t seg_entries;  // most entries in any 64-row segment (rows 64s .. 64s+63)
 };
 
 // ---------------------------------------------------------------------------

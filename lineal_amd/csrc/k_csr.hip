@@ -214,71 +214,161 @@ __global__ __launch_bounds__(SEG_NT) void csr_sor_segment_kernel(
 // resolve the segment's dependency chain while each row still accumulates in
 // exactly the sequential order of the reference.
 constexpr int WSEG = 64;
-constexpr int WSEG_WAVES = 2;
-constexpr int WSEG_CAP = 1024;  // staged entries per segment
+static_assert(WSEG == kMaxSegmentRows, "wave segments are the longest segments");
+// LDS budget of one wave-segment block: what the device grants a block
+size_t wave_lds_budget() {
+  static const size_t budget = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return size_t(64 * 1024);
+    return size_t(v);
+  }();
+  return budget;
+}
 
-template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
-__global__ __launch_bounds__(WSEG * WSEG_WAVES) void csr_sor_wave_kernel(
+// One entry of a staged segment: column, value and the column's old value,
+// read with one LDS access.
+template <typename T, typename V>
+struct alignas(sizeof(V) >= sizeof(T) ? 2 * sizeof(V) : 2 * sizeof(T)) Staged {
+  int32_t col;
+  T old;
+  V val;
+};
+
+template <typename T, typename V>
+size_t wave_lds_bytes(int64_t entries) {
+  return size_t(entries) * sizeof(Staged<T, V>) + WSEG * sizeof(T);
+}
+
+// One wavefront per segment of 64 rows.  The segment's entries (and the old
+// values of their columns) are staged in LDS with coalesced loads; lane r owns
+// row s0 + r.  In column order a row's entries split into
+//   forward:  [other segments' rows before s0: old] [s0..i-1: new] [i] [after i: old]
+//   backward: [before i: old] [i] [i+1..s1-1: new] [s1 and after: old]
+// so each lane first applies its leading old-value entries, then the rounds of
+// the wave loop let a lane apply its new-value entries once the rows they name
+// are finished, and a lane whose new-value entries are all applied runs
+// through its trailing old-value entries without further checks.  Every row
+// performs the oracle's operations in its order (sweep.hpp:134-200 with the
+// segment's own new values).
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, bool STAGED>
+__global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
     int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
     const V* __restrict__ va, const T* __restrict__ xin, T* __restrict__ xout,
-    const T* __restrict__ b, T omega) {
-  __shared__ int32_t s_col[WSEG_WAVES][WSEG_CAP];
-  __shared__ V s_val[WSEG_WAVES][WSEG_CAP];
-  __shared__ T s_old[WSEG_WAVES][WSEG_CAP];
-  __shared__ T s_new[WSEG_WAVES][WSEG];
-  const int lane = threadIdx.x % WSEG, w = threadIdx.x / WSEG;
-  const int64_t s0 = (blockIdx.x * int64_t(WSEG_WAVES) + w) * WSEG;
-  if (s0 >= n) return;
+    const T* __restrict__ b, T omega, int64_t cap) {
+  using S = Staged<T, V>;
+  extern __shared__ double wave_lds[];
+  S* const st = reinterpret_cast<S*>(wave_lds);
+  T* const s_new = reinterpret_cast<T*>(st + (STAGED ? cap : 0));
+  const int lane = threadIdx.x;
+  const int64_t s0 = blockIdx.x * int64_t(WSEG);
   const int64_t s1 = min(s0 + WSEG, n);
   const int64_t e0 = rp[s0], e1 = rp[s1];
-  const bool staged = e1 - e0 <= WSEG_CAP;
-  if (staged) {
+  if constexpr (STAGED) {
     for (int64_t k = e0 + lane; k < e1; k += WSEG) {
-      const int32_t j = ci[k];
-      s_col[w][k - e0] = j;
-      s_val[w][k - e0] = va[k];
-      s_old[w][k - e0] = HAS_X ? xin[j] : T(0);
+      S e;
+      e.col = ci[k];
+      e.val = va[k];
+      e.old = HAS_X ? xin[e.col] : T(0);
+      st[k - e0] = e;
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __syncthreads();
+  auto entry = [&](int64_t k) {
+    if constexpr (STAGED) {
+      return st[k];
+    } else {
+      S e;
+      e.col = ci[e0 + k];
+      e.val = va[e0 + k];
+      e.old = HAS_X ? xin[e.col] : T(0);
+      return e;
+    }
+  };
   const int64_t i = s0 + lane;
   bool finished = i >= s1;
   int64_t k = 0, kend = 0;
   T acc = T(0), d = T(1), div = T(0), xi = T(0);
+  // one old-value entry (ULTRA: entries outside the segment also add |a_ij|)
+  auto apply_old = [&](const S& e) {
+    const T a = T(e.val);
+    if constexpr (ULTRA) {
+      if (e.col < s0 || e.col >= s1) div += a < T(0) ? -a : a;
+    }
+    acc = fma(-a, e.old, acc);
+  };
+  auto apply_diag = [&](const S& e) {
+    d = T(e.val);
+    if constexpr (ULTRA) acc = fma(-d, xi, acc);
+  };
+  // old-value entries while the column stays below `lim`
+  auto run_old = [&](int64_t lim) {
+    while (k < kend) {
+      const S e = entry(k);
+      if (e.col >= lim) break;
+      apply_old(e);
+      ++k;
+    }
+  };
   if (!finished) {
     k = rp[i] - e0;
     kend = rp[i + 1] - e0;
     acc = b[i];
     xi = HAS_X ? xin[i] : T(0);
+    if constexpr (FWD) {
+      run_old(s0);
+    } else {
+      run_old(i);
+      if (k < kend) {
+        const S e = entry(k);
+        if (e.col == i) {
+          apply_diag(e);
+          ++k;
+        }
+      }
+    }
   }
   const T m1w = T(1) - omega;
   uint64_t done = __ballot(finished);
   while (done != ~uint64_t(0)) {
     if (!finished) {
+      // new-value entries: columns in [s0, i) forward, (i, s1) backward
+      const int64_t lim = FWD ? i : s1;
+      bool ready = true;
       while (k < kend) {
-        const int64_t j = staged ? s_col[w][k] : ci[e0 + k];
-        const T a = T(staged ? s_val[w][k] : va[e0 + k]);
-        if (j == i) {
-          d = a;
-          if (ULTRA) acc = fma(-a, xi, acc);
-          ++k;
-          continue;
+        const S e = entry(k);
+        if (e.col >= lim) break;
+        if (!((done >> (e.col - s0)) & 1u)) {
+          ready = false;  // wait for that row
+          break;
         }
-        const bool inside = j >= s0 && j < s1;
-        T xj;
-        if (inside && (FWD ? j < i : j > i)) {
-          if (!((done >> (j - s0)) & 1u)) break;  // wait for that row
-          xj = s_new[w][j - s0];
-        } else {
-          xj = staged ? s_old[w][k] : (HAS_X ? xin[j] : T(0));
-          if (ULTRA && !inside) div += a < T(0) ? -a : a;
-        }
-        acc = fma(-a, xj, acc);
+        acc = fma(-T(e.val), s_new[e.col - s0], acc);
         ++k;
       }
-      if (k == kend) {
+      if (ready) {
+        if constexpr (FWD) {
+          if (k < kend) {
+            const S e = entry(k);
+            if (e.col == i) {
+              apply_diag(e);
+              ++k;
+            }
+          }
+        }
+        // trailing old-value entries, four reads in flight
+        while (k + 4 <= kend) {
+          const S ea = entry(k), eb = entry(k + 1), ec = entry(k + 2), ed = entry(k + 3);
+          apply_old(ea);
+          apply_old(eb);
+          apply_old(ec);
+          apply_old(ed);
+          k += 4;
+        }
+        while (k < kend) {
+          apply_old(entry(k));
+          ++k;
+        }
         T xn;
         if (ULTRA) {
           const T ad = d < T(0) ? -d : d;
@@ -287,7 +377,7 @@ __global__ __launch_bounds__(WSEG * WSEG_WAVES) void csr_sor_wave_kernel(
           const T quot = acc / d;
           xn = fma(omega, quot, m1w * xi);
         }
-        s_new[w][lane] = xn;
+        s_new[lane] = xn;
         xout[i] = xn;
         finished = true;
       }
@@ -299,23 +389,33 @@ __global__ __launch_bounds__(WSEG * WSEG_WAVES) void csr_sor_wave_kernel(
   }
 }
 
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
+void launch_wave_segment2(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
+                          double omega, hipStream_t st) {
+  const int64_t nseg = (a.n + WSEG - 1) / WSEG;
+  const size_t staged = wave_lds_bytes<T, V>(a.seg_entries);
+  const auto xi = static_cast<const T*>(x_in);
+  const auto xo = static_cast<T*>(x_out);
+  const auto bp = static_cast<const T*>(b);
+  const auto vp = static_cast<const V*>(a.vals);
+  if (staged <= wave_lds_budget()) {
+    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, HAS_X, true>), dim3(nseg),
+                       dim3(WSEG), staged, st, a.n, a.rowptr, a.cols, vp, xi, xo, bp, T(omega),
+                       a.seg_entries);
+  } else {
+    const size_t direct = wave_lds_bytes<T, V>(0);
+    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, HAS_X, false>), dim3(nseg),
+                       dim3(WSEG), direct, st, a.n, a.rowptr, a.cols, vp, xi, xo, bp, T(omega),
+                       int64_t(0));
+  }
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T, typename V, bool FWD, bool ULTRA>
 void launch_wave_segment(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
                          double omega, hipStream_t st) {
-  const int64_t nseg = (a.n + WSEG - 1) / WSEG;
-  const int64_t blocks = (nseg + WSEG_WAVES - 1) / WSEG_WAVES;
-  if (x_in) {
-    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, true>), dim3(blocks),
-                       dim3(WSEG * WSEG_WAVES), 0, st, a.n, a.rowptr, a.cols,
-                       static_cast<const V*>(a.vals), static_cast<const T*>(x_in),
-                       static_cast<T*>(x_out), static_cast<const T*>(b), T(omega));
-  } else {
-    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, false>), dim3(blocks),
-                       dim3(WSEG * WSEG_WAVES), 0, st, a.n, a.rowptr, a.cols,
-                       static_cast<const V*>(a.vals), static_cast<const T*>(x_in),
-                       static_cast<T*>(x_out), static_cast<const T*>(b), T(omega));
-  }
-  LGPU_HIP_CHECK(hipGetLastError());
+  if (x_in) launch_wave_segment2<T, V, FWD, ULTRA, true>(a, x_in, x_out, b, omega, st);
+  else launch_wave_segment2<T, V, FWD, ULTRA, false>(a, x_in, x_out, b, omega, st);
 }
 
 template <typename T, typename V, bool FWD, bool ULTRA>
