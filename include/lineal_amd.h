@@ -80,6 +80,11 @@ typedef struct {
   int zero_to_one;     /* ZeroToOne diagonal handling (1) or ZeroToZero (0) */
   double p_start;      /* solution at the x = 0 border */
   double p_end;        /* solution at the x = nx border */
+  double velocity_x;   /* first-order upwind advection along +x (>= 0; 0: pure
+                          diffusion).  Not in the reference: each cell's +x face
+                          carries the outflow v*h_y*h_z (diagonal, after the
+                          diffusion entries), the -x coupling gains -v*h_y*h_z and
+                          the x = 0 rhs the inflow v*h_y*h_z*p_start. */
 } lgpu_box;
 
 /* AdjacentStencilMatrix over StationaryDiffusionValuesValuator
@@ -177,9 +182,11 @@ typedef struct {
 
 /* run_iterative_solver (linear-solver/runner.hpp:17-84) over
  * ConjugateGradientsSolver (conjugate-gradients/solver.hpp:50-115), optionally
- * AMG-preconditioned.  x holds the initial guess and receives the solution
- * (f64).  res_hist (max_it + 1 entries) may be NULL.  Returns LGPU_OK on
- * convergence, LGPU_NOT_CONVERGED, LGPU_BREAKDOWN or LGPU_NONFINITE. */
+ * AMG-preconditioned.  The preconditioner may be built from another operator
+ * of the same size and context (e.g. the diffusion part of an advection
+ * operator).  x holds the initial guess and receives the solution (f64).
+ * res_hist (max_it + 1 entries) may be NULL.  Returns LGPU_OK on convergence,
+ * LGPU_NOT_CONVERGED, LGPU_BREAKDOWN or LGPU_NONFINITE. */
 int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
                   const lgpu_krylov_params* params, int* iterations, double* res_hist);
 /* BiCgStabSolver (bicgstab/solver.hpp:51-151), right-preconditioned. */

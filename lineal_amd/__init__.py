@@ -62,6 +62,7 @@ class _Box(C.Structure):
         ("zero_to_one", C.c_int),
         ("p_start", C.c_double),
         ("p_end", C.c_double),
+        ("velocity_x", C.c_double),
     ]
 
 
@@ -331,6 +332,8 @@ class BoxInfo:
     zero_to_one: bool = True
     p_start: float = 1.0
     p_end: float = 0.0
+    # first-order upwind advection along +x (not in the reference)
+    velocity_x: float = 0.0
 
     def _struct(self):
         b = _Box()
@@ -339,6 +342,7 @@ class BoxInfo:
             b.len[d] = float(self.lengths[d]) if self.lengths is not None else float(self.dims[d])
         b.zero_to_one = int(self.zero_to_one)
         b.p_start, b.p_end = float(self.p_start), float(self.p_end)
+        b.velocity_x = float(self.velocity_x)
         return b
 
 

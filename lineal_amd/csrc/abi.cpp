@@ -194,6 +194,7 @@ void fill_desc(lgpu_op op, const lgpu_box* box) {
   d.zero_to_one = box->zero_to_one;
   d.p_start = box->p_start;
   d.p_end = box->p_end;
+  d.adv = box->velocity_x == 0.0 ? 0.0 : box->velocity_x * h[1] * h[2];
 }
 
 void init_sor_sched(lgpu_op op) {
@@ -209,15 +210,16 @@ void init_sor_sched(lgpu_op op) {
 }
 
 // Host evaluation of the stencil rows in f64 in iterate order (-x,-y,-z,+z,+y,+x,
-// diagonal last), tensor/dynamic/stencil/matrix/algorithm.hpp:91-227; the same
-// operations as the device's eval_row.
+// diagonal last), tensor/dynamic/stencil/matrix/algorithm.hpp:91-227, plus the
+// upwind advection terms (the -x coupling and the diagonal); the same
+// operations as the device's row evaluation.
 void stencil_host_rows(const lgpu_op op, setup::RowMatrix& m) {
   const StencilDesc& d = op->desc;
   const int64_t nx = d.nx, ny = d.ny, nz = d.nz, n = nx * ny * nz, nyz = ny * nz;
   m = setup::RowMatrix{};
   m.n = n;
   m.f32 = false;
-  m.symmetric = true;
+  m.symmetric = d.adv == 0.0;
   m.rowptr.resize(size_t(n + 1));
   m.rowptr[0] = 0;
   for (int64_t i = 0; i < n; ++i) {
@@ -247,7 +249,7 @@ void stencil_host_rows(const lgpu_op op, setup::RowMatrix& m) {
         if (pos[ax] > 0) {
           const double w = conn(i, i - off[ax], ax);
           m.cols[size_t(k)] = int32_t(i - off[ax]);
-          m.vals[size_t(k++)] = -w;
+          m.vals[size_t(k++)] = ax == 0 ? -(w + d.adv) : -w;
           diag += w;
         }
       for (int ax = 2; ax >= 0; --ax) {
@@ -259,6 +261,7 @@ void stencil_host_rows(const lgpu_op op, setup::RowMatrix& m) {
           diag += w;
         }
       }
+      diag += d.adv;
       if (d.zero_to_one && diag == 0.0) diag = 1.0;
       m.cols[size_t(k)] = int32_t(i);
       m.vals[size_t(k)] = diag;
@@ -362,8 +365,12 @@ void coarse_sweep(lgpu_amg amg, CsrDev& a, const void* x_in, void* x_out, const 
   else csr_sweep(a, x_in, x_out, b, omega, fwd, ultra, amg->ctx->stream);
 }
 
-// One V-cycle in precision T from zero: z = M r (multigrid/solver.hpp:56-176)
-void vcycle(lgpu_amg amg, const void* r, void* z) {
+// One V-cycle in precision T from zero: z = M r (multigrid/solver.hpp:56-176).
+// `fine` smooths and forms the residual on the finest level: the hierarchy's
+// own finest operator, or a Krylov operator of the same size (the upwind
+// advection operator over the hierarchy of its diffusion part, so that the
+// forward sweep follows the flow).
+void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine) {
   const bool f32 = amg->f32;
   hipStream_t st = amg->ctx->stream;
   const int L = int(amg->lv.size());
@@ -374,8 +381,8 @@ void vcycle(lgpu_amg amg, const void* r, void* z) {
   for (int l = 0; l + 1 < L; ++l) {
     AmgLevel& cur = amg->lv[size_t(l)];
     if (l == 0) {
-      op_sor(amg->finest, f32, nullptr, xs(0), bs(0), relax, true, ultra);
-      op_apply(amg->finest, SM_RESID, f32, xs(0), bs(0), cur.tmp.ptr, 0.0, nullptr);
+      op_sor(fine, f32, nullptr, xs(0), bs(0), relax, true, ultra);
+      op_apply(fine, SM_RESID, f32, xs(0), bs(0), cur.tmp.ptr, 0.0, nullptr);
     } else {
       coarse_sweep(amg, cur.a, nullptr, xs(l), bs(l), relax, true, ultra);
       csr_apply(cur.a.desc(), SM_RESID, xs(l), bs(l), cur.tmp.ptr, 0.0, nullptr, st);
@@ -392,26 +399,26 @@ void vcycle(lgpu_amg amg, const void* r, void* z) {
     AmgLevel& cur = amg->lv[size_t(l)];
     prolong_add(xs(l), amg->lv[size_t(l + 1)].sol.ptr, f32, cur.agg.as<uint32_t>(), cur.n,
                 inv_relax, cur.aux.ptr, st);
-    if (l == 0) op_sor(amg->finest, f32, cur.aux.ptr, xs(0), bs(0), relax, false, ultra);
+    if (l == 0) op_sor(fine, f32, cur.aux.ptr, xs(0), bs(0), relax, false, ultra);
     else coarse_sweep(amg, cur.a, cur.aux.ptr, xs(l), bs(l), relax, false, ultra);
   }
 }
 
 // z (f64) = M r (f64)
-void precondition(lgpu_amg amg, const double* r, double* z) {
+void precondition(lgpu_amg amg, const double* r, double* z, lgpu_op fine) {
   if (!amg->f32) {
-    vcycle(amg, r, z);
+    vcycle(amg, r, z, fine);
     return;
   }
   const int64_t n = amg->lv[0].n;
   hipStream_t st = amg->ctx->stream;
   vec_convert(r, false, amg->r32.ptr, true, n, st);
-  vcycle(amg, amg->r32.ptr, amg->z32.ptr);
+  vcycle(amg, amg->r32.ptr, amg->z32.ptr, fine);
   vec_convert(amg->z32.ptr, true, z, false, n, st);
 }
 
-bool amg_sync_error(lgpu_amg amg) {
-  bool e = op_sync_error(amg->finest);
+bool amg_sync_error(lgpu_amg amg, lgpu_op fine) {
+  bool e = op_sync_error(amg->finest) || (fine != amg->finest && op_sync_error(fine));
   for (size_t l = 1; l < amg->lv.size(); ++l)
     e = e || read_int(amg->ctx->stream, amg->lv[l].a.sched.as<int>() + 1) != 0;
   return e;
@@ -592,6 +599,8 @@ void check_box(const lgpu_box* box) {
     LGPU_REQUIRE(box->len[k] > 0, "box lengths must be positive");
   }
   LGPU_REQUIRE(box->n[0] * box->n[1] * box->n[2] < (int64_t(1) << 31), "box has more than 2^31 cells");
+  LGPU_REQUIRE(std::isfinite(box->velocity_x) && box->velocity_x >= 0.0,
+               "velocity_x must be finite and non-negative");
 }
 
 }  // namespace
@@ -960,9 +969,10 @@ extern "C" int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z) {
   LGPU_REQUIRE(r->ptr != z->ptr, "r and z must not alias");
   if (r->f32 || z->f32) {
     LGPU_REQUIRE(r->f32 && z->f32 && amg->f32, "f32 vectors need an f32 hierarchy");
-    vcycle(amg, r->ptr, z->ptr);
+    vcycle(amg, r->ptr, z->ptr, amg->finest);
   } else {
-    precondition(amg, static_cast<const double*>(r->ptr), static_cast<double*>(z->ptr));
+    precondition(amg, static_cast<const double*>(r->ptr), static_cast<double*>(z->ptr),
+                 amg->finest);
   }
   API_END
 }
@@ -983,7 +993,7 @@ struct Work {
 bool rere(int iteration, int every) { return every > 0 && iteration > 0 && iteration % every == 0; }
 
 int finish(lgpu_op op, lgpu_amg amg, int status) {
-  if (op_sync_error(op) || (amg && amg_sync_error(amg))) {
+  if (op_sync_error(op) || (amg && amg_sync_error(amg, op))) {
     set_error("a wavefront kernel's bounded wait expired");
     return LGPU_SYNC_TIMEOUT;
   }
@@ -1000,7 +1010,8 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
   check_vec(b, n, "b");
   check_vec(x, n, "x");
   LGPU_REQUIRE(!b->f32 && !x->f32, "the Krylov vectors are f64");
-  LGPU_REQUIRE(!precon || precon->finest == op, "the preconditioner belongs to another operator");
+  LGPU_REQUIRE(!precon || (precon->ctx == op->ctx && precon->finest->rows() == n),
+               "the preconditioner must share the operator's context and size");
   lgpu_ctx ctx = op->ctx;
   hipStream_t st = ctx->stream;
   double* scal = ctx->scal.as<double>();
@@ -1012,7 +1023,7 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
   residual_norm(op, bv, xv, r, SC_RR);
   int cur = SC_RHO0, nxt = SC_RHO1;
   if (precon) {
-    precondition(precon, r, p);
+    precondition(precon, r, p, op);
     dot_into(ctx, p, r, false, n, cur);
   } else {
     LGPU_HIP_CHECK(hipMemcpyAsync(p, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1039,7 +1050,7 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
       reduce_partials(pd, nd, scal + SC_RR, st);
     }
     if (precon) {
-      precondition(precon, r, q);
+      precondition(precon, r, q, op);
       dot_into(ctx, r, q, false, n, nxt);
       xpby_scal(p, q, scal, nxt, cur, n, st);
     } else {
@@ -1068,7 +1079,8 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
   check_vec(b, n, "b");
   check_vec(x, n, "x");
   LGPU_REQUIRE(!b->f32 && !x->f32, "the Krylov vectors are f64");
-  LGPU_REQUIRE(!precon || precon->finest == op, "the preconditioner belongs to another operator");
+  LGPU_REQUIRE(!precon || (precon->ctx == op->ctx && precon->finest->rows() == n),
+               "the preconditioner must share the operator's context and size");
   lgpu_ctx ctx = op->ctx;
   hipStream_t st = ctx->stream;
   double* scal = ctx->scal.as<double>();
@@ -1092,7 +1104,7 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
     // bicgstab/solver.hpp:71-151
     double* y = p;
     if (precon) {
-      precondition(precon, p, px);
+      precondition(precon, p, px, op);
       y = px;
     }
     op_apply(op, SM_SPMV, false, y, nullptr, v, 0.0, nullptr);
@@ -1103,7 +1115,7 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
     reduce_partials(pd, nd, scal + SC_RR, st);
     double* z = r;
     if (precon) {
-      precondition(precon, r, px);
+      precondition(precon, r, px, op);
       z = px;
     }
     op_apply(op, SM_SPMV, false, z, nullptr, t, 0.0, nullptr);

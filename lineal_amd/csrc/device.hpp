@@ -107,50 +107,6 @@ __device__ __forceinline__ float CellInfo<float, 1>::conn(const StencilDesc& s, 
   return s.tab32[d][a * 4 + b];
 }
 
-// Face weights and diagonal of one row in the order of the reference's valued,
-// unordered iterate (tensor/dynamic/stencil/matrix/algorithm.hpp:91-227):
-// diag = 0 (+2 q0 c at x=0) (+2 q0 c at x=nx-1) + w(-x) + w(-y) + w(-z) + w(+z)
-// + w(+y) + w(+x); ZeroToOne replaces a zero diagonal by 1.  Absent faces have
-// weight +0, which leaves every sum unchanged.
-template <typename T>
-struct Row {
-  T wmx, wmy, wmz, wpz, wpy, wpx, d;
-};
-
-template <typename T, int K>
-__device__ __forceinline__ Row<T> eval_row(const StencilDesc& s, int64_t x, int64_t y, int64_t z,
-                                           typename CellInfo<T, K>::Raw cc,
-                                           typename CellInfo<T, K>::Raw cmx,
-                                           typename CellInfo<T, K>::Raw cmy,
-                                           typename CellInfo<T, K>::Raw cmz,
-                                           typename CellInfo<T, K>::Raw cpz,
-                                           typename CellInfo<T, K>::Raw cpy,
-                                           typename CellInfo<T, K>::Raw cpx) {
-  using CI = CellInfo<T, K>;
-  const T q0 = T(s.q[0]), q1 = T(s.q[1]), q2 = T(s.q[2]);
-  const T ci = CI::value(s, cc);
-  Row<T> r;
-  T d = T(0);
-  const T b = (T(2) * q0) * ci;
-  if (x == 0) d += b;
-  if (x + 1 == s.nx) d += b;
-  r.wmx = x > 0 ? CI::conn(s, cc, cmx, q0, 0) : T(0);
-  r.wmy = y > 0 ? CI::conn(s, cc, cmy, q1, 1) : T(0);
-  r.wmz = z > 0 ? CI::conn(s, cc, cmz, q2, 2) : T(0);
-  r.wpz = z + 1 < s.nz ? CI::conn(s, cc, cpz, q2, 2) : T(0);
-  r.wpy = y + 1 < s.ny ? CI::conn(s, cc, cpy, q1, 1) : T(0);
-  r.wpx = x + 1 < s.nx ? CI::conn(s, cc, cpx, q0, 0) : T(0);
-  d += r.wmx;
-  d += r.wmy;
-  d += r.wmz;
-  d += r.wpz;
-  d += r.wpy;
-  d += r.wpx;
-  if (s.zero_to_one && d == T(0)) d = T(1);
-  r.d = d;
-  return r;
-}
-
 // Wave-level sum of a double (64 lanes).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

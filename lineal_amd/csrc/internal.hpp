@@ -80,6 +80,7 @@ struct StencilDesc {
   int kind;              // 0: per-cell coefficients, 1: u8 material lookup
   int zero_to_one;
   double p_start, p_end;
+  double adv;            // upwind advection v * h_y * h_z along +x (0: none)
   const double* c64;     // kind 0
   const float* c32;      // kind 0, f32 copy of c64
   const uint8_t* mat;    // kind 1

@@ -126,6 +126,7 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   const T m1w = T(1) - omega;
   const T qx = T(s.q[0]), qy = T(s.q[1]), qz = T(s.q[2]);
   const T two_q0 = T(2) * qx;
+  const T adv = T(s.adv);  // upwind advection: -x coupling and diagonal
   // mailboxes: [X part: column, b, z] the a = 7 lines; [Y part: column, a, z] the b = 7 lines
   const int64_t ncols = ((nx + CB - 1) / CB) * cy;
   T* const ybox = mbox + ncols * CB * nz;
@@ -236,10 +237,11 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
       d += wpz;
       d += wpy;
       d += wpx;
+      d += adv;
       if (s.zero_to_one && d == T(0)) d = T(1);
       const T xi = xw[k];
       T acc = bw[k];
-      acc = fma(wmx, vmx, acc);
+      acc = fma(x > 0 ? wmx + adv : T(0), vmx, acc);
       acc = fma(wmy, vmy, acc);
       acc = fma(wmz, vmz, acc);
       acc = fma(wpz, vpz, acc);

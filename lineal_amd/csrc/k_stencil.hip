@@ -251,6 +251,7 @@ __global__ __launch_bounds__(WZ * kMaxWaves) void stencil_kernel(StencilDesc s,
   const int y0 = (by * g.nwy + wy) * YR;
   const T q0 = T(s.q[0]), q1 = T(s.q[1]), q2 = T(s.q[2]);
   const T bnd = T(2) * q0;
+  const T adv = T(s.adv);
   const T m1w = T(1) - omega;
   const E* const cbase = C::ptr(s);
   const uint32_t pbe = uint32_t(nyz) * SE, pbt = uint32_t(nyz) * ST;  // plane bytes
@@ -459,6 +460,8 @@ __global__ __launch_bounds__(WZ * kMaxWaves) void stencil_kernel(StencilDesc s,
           if (k < YR - 1) xpy = rin[kp] ? xpy : T(0);
         }
         const T wmx = dir > 0 ? wbh[k] : wah, wpx = dir > 0 ? wah : wbh[k];
+        // -x coupling: face weight plus the upwind inflow (no -x neighbour at x = 0)
+        const T cmx = xlo ? T(0) : wmx + adv;
         const T xmx = dir > 0 ? xbh[k] : xah, xpx = dir > 0 ? xah : xbh[k];
         const T wmz = shift<kWaveShr1>(read_lane(fh, k), wpz);
         const T wmy = gy[k], wpy = gy[k + 1];
@@ -473,6 +476,7 @@ __global__ __launch_bounds__(WZ * kMaxWaves) void stencil_kernel(StencilDesc s,
         d += wpz;
         d += wpy;
         d += wpx;
+        d += adv;
         if (s.zero_to_one && d == T(0)) d = T(1);
         // ZL: rows and lanes outside the box store out of range (dropped) and
         // contribute exact zeros to the partial sums
@@ -481,7 +485,7 @@ __global__ __launch_bounds__(WZ * kMaxWaves) void stencil_kernel(StencilDesc s,
         if constexpr (MODE == SM_JACOBI) {
           // jacobi/sweep.hpp:66-87: accum = b - sum a_ij x_j (fnmadd with a_ij = -w)
           T acc = pb[k];
-          acc = fma(wmx, xmx, acc);
+          acc = fma(cmx, xmx, acc);
           acc = fma(wmy, xmy, acc);
           acc = fma(wmz, xmz, acc);
           acc = fma(wpz, xpz, acc);
@@ -492,7 +496,7 @@ __global__ __launch_bounds__(WZ * kMaxWaves) void stencil_kernel(StencilDesc s,
         } else {
           // matrix-vector-product.hpp:71-82: sum = fmadd(a_ij, x_j, sum) in iterate order
           T sum = T(0);
-          sum = fma(-wmx, xmx, sum);
+          sum = fma(-cmx, xmx, sum);
           sum = fma(-wmy, xmy, sum);
           sum = fma(-wmz, xmz, sum);
           sum = fma(-wpz, xpz, sum);
@@ -563,6 +567,7 @@ __global__ void rhs_kernel(StencilDesc s, T* b) {
   T sum = T(0);
   if (x == 0) sum += ((T(2) * q0) * ci) * T(s.p_start);
   if (x + 1 == s.nx) sum += ((T(2) * q0) * ci) * T(s.p_end);
+  if (x == 0) sum += T(s.adv) * T(s.p_start);
   b[i] = sum;
 }
 

@@ -34,6 +34,7 @@ class StencilStruct(C.Structure):
         ("mat_coeffs", C.c_double * 4),
         ("p_start", C.c_double),
         ("p_end", C.c_double),
+        ("velocity", C.c_double),
     ]
 
 
@@ -155,7 +156,7 @@ class Stencil:
     """Matrix-free diffusion stencil (values or lookup valuator)."""
 
     def __init__(self, dims, coeff=None, materials=None, mat_coeffs=(0.0, 1.0, 2.0, 3.0),
-                 lengths=None, zero_to_one=True, p_start=1.0, p_end=0.0):
+                 lengths=None, zero_to_one=True, p_start=1.0, p_end=0.0, velocity=0.0):
         self.dims = tuple(int(d) for d in dims)
         self.N = int(np.prod(self.dims))
         s = StencilStruct()
@@ -164,6 +165,8 @@ class Stencil:
             s.len[d] = float(lengths[d]) if lengths is not None else float(self.dims[d])
         s.zero_to_one = int(bool(zero_to_one))
         s.p_start, s.p_end = float(p_start), float(p_end)
+        s.velocity = float(velocity)
+        self.velocity = float(velocity)
         self.coeff = self.materials = None
         if coeff is not None:
             self.coeff = np.ascontiguousarray(coeff, dtype=np.float64).reshape(-1)
@@ -219,7 +222,7 @@ class Stencil:
         L.or_csr_from_stencil(C.byref(self.s), int(compress), C.byref(rows), _p(rp, C.c_int64),
                               _p(cols, C.c_int64), _p(vals, C.c_double), _p(rhs, C.c_double),
                               _p(cmap, C.c_int64) if compress else None)
-        m = Csr(rp, cols[:nnz], vals[:nnz], symmetric=True)
+        m = Csr(rp, cols[:nnz], vals[:nnz], symmetric=self.velocity == 0.0)
         return m, rhs, (cmap if compress else None)
 
 

@@ -76,6 +76,12 @@ void or_axis_quotients(const or_stencil* s, double q[3]) {
   }
 }
 
+/* advective face coefficient v * h_y * h_z (0 without advection) */
+static double or_advection(const or_stencil* s) {
+  if (s->velocity == 0.0) return 0.0;
+  return s->velocity * (s->len[1] / (double)s->n[1]) * (s->len[2] / (double)s->n[2]);
+}
+
 #define REAL double
 #define SUF(name) name##_f64
 #define FMA fma
@@ -198,7 +204,7 @@ static void graph_free(graph_t* g) {
 static void graph_build(const or_mat* m, graph_t* g) {
   const int64_t n = m->is_csr ? m->csr->n : m->st->n[0] * m->st->n[1] * m->st->n[2];
   g->n = n;
-  g->symmetric = m->is_csr ? m->csr->symmetric : 1;
+  g->symmetric = m->is_csr ? m->csr->symmetric : m->st->velocity == 0.0;
   g->is_f32 = m->is_csr ? m->csr->is_f32 : 0;
   g->eptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
   g->diag = (double*)malloc(sizeof(double) * (size_t)n);
@@ -645,7 +651,7 @@ static void galerkin(const or_mat* fine, const uint32_t* map, int64_t nc, const 
   out->cols = cc;
   out->vals = cv;
   out->is_f32 = coarse_f32;
-  out->symmetric = fine->is_csr ? fine->csr->symmetric : 1;
+  out->symmetric = fine->is_csr ? fine->csr->symmetric : fine->st->velocity == 0.0;
 }
 
 /* Cholesky-Banachiewicz, linear-solver/direct/cholesky/decomposition.hpp:44-99:
@@ -822,6 +828,20 @@ void or_amg_stats(const or_hierarchy* h, int level, int64_t out[5]) {
 /* ---------------------------------------------------------------------------
  * Krylov drivers */
 
+/* The hierarchy preconditions the Krylov operator `a`: level 0's smoothing
+ * and residual use `a` itself, coarser levels the hierarchy's Galerkin
+ * matrices.  For the operator the hierarchy was built from this is the
+ * reference's V-cycle; for a related operator (diffusion plus advection with
+ * a hierarchy built from the diffusion part) the finest smoother still sees
+ * the operator being solved. */
+static or_hierarchy* precon_view(const or_hierarchy* h, const or_mat* a) {
+  if (!h) return NULL;
+  or_hierarchy* v = (or_hierarchy*)malloc(sizeof(or_hierarchy));
+  *v = *h;
+  v->finest = *a;
+  return v;
+}
+
 static void apply_precon(const or_hierarchy* h, int precon, int ultra, const double* r, double* z,
                          int64_t n) {
   if (precon == 1) {
@@ -848,6 +868,8 @@ int or_pcg(const or_mat* a, const or_hierarchy* h, int precon, int ultra, const 
   double* r = (double*)malloc(sizeof(double) * (size_t)n);
   double* p = (double*)malloc(sizeof(double) * (size_t)n);
   double* q = (double*)malloc(sizeof(double) * (size_t)n);
+  or_hierarchy* hv = precon_view(h, a);
+  h = hv;
   or_residual_f64(a, b, x, r);
   double rr = or_dot_f64(r, r, n), rho;
   if (precon) {
@@ -888,6 +910,7 @@ int or_pcg(const or_mat* a, const or_hierarchy* h, int precon, int ultra, const 
   free(r);
   free(p);
   free(q);
+  free(hv);
   return it;
 }
 
@@ -901,6 +924,8 @@ int or_pbicgstab(const or_mat* a, const or_hierarchy* h, int precon, int ultra,
   double* v = (double*)malloc(sizeof(double) * (size_t)n);
   double* t = (double*)malloc(sizeof(double) * (size_t)n);
   double* y = (double*)malloc(sizeof(double) * (size_t)n);
+  or_hierarchy* hv = precon_view(h, a);
+  h = hv;
   or_residual_f64(a, b, x, r);
   memcpy(r0, r, sizeof(double) * (size_t)n);
   memcpy(p, r, sizeof(double) * (size_t)n);
@@ -939,5 +964,6 @@ int or_pbicgstab(const or_mat* a, const or_hierarchy* h, int precon, int ultra,
   free(v);
   free(t);
   free(y);
+  free(hv);
   return it;
 }

@@ -37,6 +37,11 @@ typedef struct {
   const uint8_t* mat;        /* kind 1, N entries */
   double mat_coeffs[4];      /* kind 1 */
   double p_start, p_end;
+  /* first-order upwind advection with velocity v >= 0 along +x (not in the
+   * reference): every cell's +x face carries the outflow v*a (a = h_y h_z),
+   * the -x coupling gains -v*a, the diagonal +v*a after the diffusion
+   * entries, and the x = 0 rhs +v*a*p_start (inflow of the Dirichlet value) */
+  double velocity;
 } or_stencil;
 
 /* CSR matrix, columns sorted, diagonal stored; values kept as double (an f32

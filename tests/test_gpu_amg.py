@@ -144,3 +144,50 @@ def test_solver_status_zero_rhs():
     x = la.DenseVector(S.N, fill=0.0)
     res = la.ConjugateGradientsSolver(8).solve(A, x, la.DenseVector(S.N, fill=0.0), rel_tol=1e-8, max_it=10)
     assert res.converged and res.iterations == 0
+
+
+def test_bicgstab_advection_closed_form():
+    """Constant coefficient, Pe = v L_x / D = 10: BiCGStab on the upwind
+    operator, preconditioned by the AMG of the diffusion part, reaches the
+    closed-form discrete solution p_i = A + B rho^i, rho = (D + v) / D."""
+    n, D = 32, 1.0
+    v = 10.0 / n
+    dims = (n, 8, 8)
+    c = np.full(n * 64, D)
+    A_adv = la.StencilMatrix.values(la.BoxInfo(dims, velocity_x=v), c)
+    A_diff = la.StencilMatrix.values(la.BoxInfo(dims), c)
+    M = la.MultiGridSolver().instantiate(A_diff)
+    b = A_adv.rhs()
+    x = la.DenseVector(A_adv.row_num, fill=0.0)
+    res = la.BiCgStabSolver(8, M).solve(A_adv, x, b, rel_tol=1e-12, max_it=300)
+    assert res.converged
+    rho = (D + v) / D
+    Mx = np.array([[2 * D + v, 2 * D], [2 * D, (2 * D + v) * rho ** (n - 1)]])
+    a_, b_ = np.linalg.solve(Mx, [(2 * D + v) * 1.0, 0.0])
+    exact = a_ + b_ * rho ** np.arange(n)
+    np.testing.assert_allclose(x.numpy().reshape(n, 64), np.repeat(exact[:, None], 64, axis=1), rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("dims", [(24, 24, 24), (32, 20, 28)])
+def test_bicgstab_advection_two_phase_matches_oracle(dims):
+    """Two-phase lookup field (pore 1, solid 1e-6) with upwind advection at
+    Pe = 10 (v = 10 / L_x): GPU BiCGStab with the diffusion AMG against the
+    oracle's BiCGStab with the same hierarchy."""
+    m = problems.spheres(dims, 3, rmin=2, rmax=4)
+    mc = (1.0, 1e-6, 0.0, 0.0)
+    v = 10.0 / dims[0]
+    A_adv = la.StencilMatrix.lookup(la.BoxInfo(dims, velocity_x=v), m, mc)
+    A_diff = la.StencilMatrix.lookup(la.BoxInfo(dims), m, mc)
+    S_adv = O.Stencil(dims, materials=m, mat_coeffs=mc, velocity=v)
+    S_diff = O.Stencil(dims, materials=m, mat_coeffs=mc)
+    H = O.Hierarchy(S_diff, coarse_segment=64)
+    M = la.MultiGridSolver().instantiate(A_diff)
+    b = S_adv.rhs()
+    x_ref, it_ref, _ = O.pbicgstab(S_adv, b, H, precon=1, rel_tol=1e-10, max_it=300)
+    x = la.DenseVector(S_adv.N, fill=0.0)
+    res = la.BiCgStabSolver(8, M).solve(A_adv, x, la.DenseVector(b), rel_tol=1e-10, max_it=300)
+    assert res.converged
+    assert abs(res.iterations - it_ref) <= 2
+    assert np.abs(x.numpy() - x_ref).max() / np.abs(x_ref).max() <= 1e-6
+    r = O.residual(S_adv, b, x.numpy())
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(b)

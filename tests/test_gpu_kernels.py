@@ -142,3 +142,35 @@ def test_csr_kernels(dims, dtype):
                                 la.DenseVector(b), 1.1, forward, ultra, segment_rows=seg)
                     bounds = list(range(0, n, seg)) + [n]
                     same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.1), forward, ultra, bounds))
+
+
+@pytest.mark.parametrize("dims", [(13, 17, 5), (33, 17, 65), (6, 21, 130)])
+@pytest.mark.parametrize("kind", ["values", "lookup"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_advection_kernels(dims, kind, dtype):
+    """First-order upwind advection along +x: every pass stays exact."""
+    v = 0.37
+    box = la.BoxInfo(dims, p_start=1.0, p_end=0.0, velocity_x=v)
+    if kind == "values":
+        c = problems.lognormal(dims, 1.0, 1)
+        A = la.StencilMatrix.values(box, c)
+        S = O.Stencil(dims, coeff=c, p_start=1.0, p_end=0.0, velocity=v)
+    else:
+        m = problems.diagonal_lines(dims)
+        mc = (0.0, 1.0, 2.0, 3.0)
+        A = la.StencilMatrix.lookup(box, m, mc)
+        S = O.Stencil(dims, materials=m, mat_coeffs=mc, p_start=1.0, p_end=0.0, velocity=v)
+    n = S.N
+    x = problems.uniform(n, 2, dtype)
+    b = problems.uniform(n, 3, dtype)
+    xd = la.DenseVector(x)
+    same(A.spmv(xd, la.DenseVector(n, dtype)).numpy(), O.spmv(S, x))
+    same(A.rhs(dtype).numpy(), S.rhs(dtype))
+    same(A.residual(la.DenseVector(b), xd, la.DenseVector(n, dtype)).numpy(), O.residual(S, b, x))
+    same(A.jacobi(xd, la.DenseVector(n, dtype), la.DenseVector(b), 0.7).numpy(), O.jacobi(S, x, b, dtype(0.7)))
+    for forward in (True, False):
+        for ultra in (False, True):
+            for with_x in (True, False):
+                out = A.sor(xd if with_x else None, la.DenseVector(n, dtype), la.DenseVector(b), 1.1,
+                            forward, ultra)
+                same(out.numpy(), O.sor(S, x if with_x else None, b, dtype(1.1), forward, ultra))

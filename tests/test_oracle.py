@@ -484,3 +484,75 @@ def test_mixed_precision_vcycle_close_to_fp64():
     x32, it32, _ = O.pcg(S, b, H, precon=2, rel_tol=1e-10)
     assert abs(it64 - it32) <= 2
     assert np.abs(x64 - x32).max() <= 1e-8 * np.abs(x64).max()
+
+
+# ---------------------------------------------------------------------------
+# upwind advection (not in the reference; pinned by a dense restatement and the
+# closed-form discrete solution)
+
+
+def dense_advection(dims, coeff, velocity, lengths=None, zero_to_one=True, p_start=1.0):
+    """Diffusion operator plus first-order upwind advection along +x: every
+    cell's +x face carries the outflow v*a (a = h_y h_z) and the -x coupling the
+    inflow -v*a; the x = 0 rhs gains v*a*p_start."""
+    nx, ny, nz = dims
+    L = lengths or dims
+    h = [L[d] / dims[d] for d in range(3)]
+    a = velocity * h[1] * h[2]
+    A = dense_operator(dims, coeff, lengths, zero_to_one=False)
+    N = nx * ny * nz
+    nyz = ny * nz
+    for i in range(N):
+        A[i, i] += a
+        if i >= nyz:
+            A[i, i - nyz] -= a
+        if zero_to_one and A[i, i] == 0:
+            A[i, i] = 1.0
+    return A, a * p_start
+
+
+@pytest.mark.parametrize("dims,lengths", [((4, 9, 7), None), ((5, 3, 4), (1.0, 2.0, 7.5))])
+def test_advection_rows_equal_dense_assembly(dims, lengths):
+    S0, c = lognormal_stencil(dims, lengths=lengths)
+    v = 0.37
+    S = O.Stencil(dims, coeff=c, lengths=lengths, velocity=v)
+    A, inflow = dense_advection(dims, c, v, lengths)
+    for i in range(S.N):
+        cols, vals = S.row(i)
+        row = np.zeros(S.N)
+        row[cols] = vals
+        np.testing.assert_allclose(row, A[i], rtol=1e-15, atol=1e-15)
+    assert not np.allclose(A, A.T)
+    b0 = S0.rhs()
+    b = S.rhs()
+    nyz = dims[1] * dims[2]
+    np.testing.assert_allclose(b[:nyz], b0[:nyz] + inflow, rtol=1e-15)
+    np.testing.assert_array_equal(b[nyz:], b0[nyz:])
+
+
+@pytest.mark.parametrize("velocity,coef", [(0.4, 1.0), (2.5, 0.3), (10.0 / 24.0, 1.0)])
+def test_advection_closed_form_one_dimensional(velocity, coef):
+    """Constant coefficient D, constant v: interior rows
+    -(D+v) p_{i-1} + (2D+v) p_i - D p_{i+1} = 0 give p_i = A + B rho^i with
+    rho = (D+v)/D (the cell Peclet series); the x = 0 row
+    (3D+v) p_0 - D p_1 = (2D+v) p_start and the x = n-1 row
+    -(D+v) p_{n-2} + (3D+v) p_{n-1} = 2D p_end fix A and B."""
+    n, ps, pe = 24, 1.0, 0.0
+    dims = (n, 3, 2)
+    c = np.full(n * 6, coef)
+    S = O.Stencil(dims, coeff=c, velocity=velocity, p_start=ps, p_end=pe)
+    D, v = coef, velocity
+    rho = (D + v) / D
+    M = np.array([[2 * D + v, 2 * D], [2 * D, (2 * D + v) * rho ** (n - 1)]])
+    Acoef, Bcoef = np.linalg.solve(M, [(2 * D + v) * ps, 2 * D * pe])
+    exact = Acoef + Bcoef * rho ** np.arange(n)
+    # the operator: a dense solve of the oracle's CSR equals the closed form
+    C, rhs, _ = S.to_csr()
+    dense = np.zeros((S.N, S.N))
+    for i in range(S.N):
+        dense[i, C.cols[C.rowptr[i]:C.rowptr[i + 1]]] = C.vals[C.rowptr[i]:C.rowptr[i + 1]]
+    p = np.linalg.solve(dense, S.rhs())
+    np.testing.assert_allclose(p.reshape(n, 6), np.repeat(exact[:, None], 6, axis=1), rtol=1e-11, atol=1e-13)
+    # the oracle's SpMV agrees: A p_exact reproduces the rhs
+    r = O.residual(S, S.rhs(), np.repeat(exact[:, None], 6, axis=1).reshape(-1))
+    assert np.abs(r).max() <= 1e-12 * np.abs(S.rhs()).max()
