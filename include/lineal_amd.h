@@ -193,6 +193,34 @@ int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
 int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
                         const lgpu_krylov_params* params, int* iterations, double* res_hist);
 
+/* ---- adjoint of the solve (no counterpart in the reference) ---------------
+ * Objective: the total outlet flux of a per-cell coefficient stencil without
+ * advection,
+ *   J(p, c) = sum over the x = nx-1 cells of (2 q_0 c_i)(p_i - p_end),
+ * built from the reference's Dirichlet border summands
+ * (tensor/dynamic/stencil/valuator/stationary-diffusion-values.hpp:124-138),
+ * the nearest reference functional being EffDiffCalc's mid-plane flux
+ * (component-wise/sink/effective-diffusion.hpp:228-335).  Parameters: the
+ * per-cell log coefficients theta_i = log c_i.  All vectors are f64. */
+
+/* *flux = J(p, c) */
+int lgpu_outlet_flux(lgpu_op op, lgpu_vec p, double* flux);
+/* g = dJ/dp: 2 q_0 c_i on the x = nx-1 cells, 0 elsewhere */
+int lgpu_outlet_flux_adjoint_rhs(lgpu_op op, lgpu_vec g);
+/* grad = dJ/dtheta given the forward solution p (A p = b) and the adjoint
+ * solution lambda (A lambda = dJ/dp), one fused pass over the faces:
+ *   grad_k = c_k ([x = nx-1] 2 q_0 (p_k - p_end)
+ *                 + lambda_k 2 q_0 ([x = 0](p_start - p_k) + [x = nx-1](p_end - p_k)))
+ *          - sum over faces (k, j) H(c_k, c_j) q_d c_j / (c_k + c_j) (lambda_k - lambda_j)(p_k - p_j) */
+int lgpu_outlet_flux_gradient(lgpu_op op, lgpu_vec p, lgpu_vec lambda, lgpu_vec grad);
+/* Forward solve A p = b, adjoint solve A lambda = dJ/dp with the same
+ * (optional) AMG preconditioner and CG parameters, then the gradient.  p and
+ * lambda hold the initial guesses and receive the solutions; *flux receives
+ * J(p, c); iterations[0] / iterations[1] the forward / adjoint CG counts (may
+ * be NULL).  Returns the first non-OK status of the two solves. */
+int lgpu_adjoint_grad(lgpu_op op, lgpu_amg precon, const lgpu_krylov_params* params, lgpu_vec p,
+                      lgpu_vec lambda, lgpu_vec grad, double* flux, int* iterations);
+
 #ifdef __cplusplus
 }
 #endif

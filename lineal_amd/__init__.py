@@ -127,6 +127,11 @@ SIGNATURES = {
     "lgpu_amg_vcycle": (C.c_int, [_vp, _vp, _vp]),
     "lgpu_cg_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
     "lgpu_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
+    "lgpu_outlet_flux": (C.c_int, [_vp, _vp, _f64p]),
+    "lgpu_outlet_flux_adjoint_rhs": (C.c_int, [_vp, _vp]),
+    "lgpu_outlet_flux_gradient": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "lgpu_adjoint_grad": (C.c_int, [_vp, _vp, C.POINTER(_KrylovParams), _vp, _vp, _vp, _f64p,
+                                    C.POINTER(C.c_int)]),
 }
 
 _lib = None
@@ -581,3 +586,52 @@ class BiCgStabSolver(_Krylov):
 def run_iterative_solver(solver: _Krylov, lhs, sol, rhs, rel_tol=1e-8, max_it=1000) -> SolveResult:
     """run_iterative_solver (linear-solver/runner.hpp:17-84) with a relative-residual stop."""
     return solver.solve(lhs, sol, rhs, rel_tol, max_it)
+
+
+# ---------------------------------------------------------------------------
+# adjoint of the solve (not in the reference)
+
+
+def outlet_flux(A: StencilMatrix, p: DenseVector) -> float:
+    """J = sum over the x = nx-1 cells of (2 q_0 c_i)(p_i - p_end)."""
+    out = C.c_double()
+    _check(lib().lgpu_outlet_flux(A.h, p.h, C.byref(out)))
+    return out.value
+
+
+def outlet_flux_adjoint_rhs(A: StencilMatrix, g: DenseVector) -> DenseVector:
+    _check(lib().lgpu_outlet_flux_adjoint_rhs(A.h, g.h))
+    return g
+
+
+def outlet_flux_gradient(A: StencilMatrix, p: DenseVector, lam: DenseVector,
+                         grad: DenseVector) -> DenseVector:
+    """grad = dJ/dlog c from the forward solution p and the adjoint solution lam."""
+    _check(lib().lgpu_outlet_flux_gradient(A.h, p.h, lam.h, grad.h))
+    return grad
+
+
+@dataclass
+class AdjointResult:
+    status: int
+    flux: float
+    forward_iterations: int
+    adjoint_iterations: int
+
+    @property
+    def converged(self):
+        return self.status == OK
+
+
+def adjoint_grad(A: StencilMatrix, precon: MultiGridInstance | None, p: DenseVector,
+                 lam: DenseVector, grad: DenseVector, rel_tol=1e-10, max_it=1000,
+                 rere_every=8) -> AdjointResult:
+    """Forward CG solve A p = b, adjoint CG solve A lam = dJ/dp with the same
+    preconditioner, and the fused gradient pass (lgpu_adjoint_grad)."""
+    prm = _KrylovParams(float(rel_tol), int(max_it), int(rere_every))
+    flux = C.c_double()
+    its = (C.c_int * 2)()
+    st = lib().lgpu_adjoint_grad(A.h, _vh(precon), C.byref(prm), p.h, lam.h, grad.h,
+                                 C.byref(flux), its)
+    _check(st, allow=(NOT_CONVERGED, BREAKDOWN, NONFINITE))
+    return AdjointResult(st, flux.value, its[0], its[1])

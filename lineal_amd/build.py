@@ -19,11 +19,13 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblineal_amd.so")
 ARCH = os.environ.get("LINEAL_AMD_ARCH", "gfx950")
 
-SOURCES = ["abi.cpp", "setup.cpp", "k_stencil.hip", "k_sor.hip", "k_csr.hip", "k_blas.hip"]
+SOURCES = ["abi.cpp", "setup.cpp", "k_stencil.hip", "k_sor.hip", "k_csr.hip", "k_blas.hip",
+           "k_adjoint.hip"]
 
 # -ffp-contract=off: only the fma() calls written in the sources are fused, so
-# every kernel performs exactly the oracle's operations.  Packed FP32 ops are
-# disabled (see the build notes in DESIGN.md).
+# every kernel performs exactly the oracle's operations.  Packed FP32 ops
+# (v_pk_fma_f32 and relatives) are disabled: some of their forms have given
+# sporadic wrong results on MI355X.
 FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",

@@ -290,8 +290,9 @@ void op_apply(lgpu_op op, int mode, bool f32, const void* x, const void* b, void
   }
 }
 
+// partial sums written by an f64 reduction pass of the operator
 int64_t op_partials(lgpu_op op) {
-  return op->is_csr ? csr_partials(op->csr.desc()) : stencil_partials(op->desc);
+  return op->is_csr ? csr_partials(op->csr.desc()) : stencil_partials(op->desc, false);
 }
 
 void csr_sweep(CsrDev& a, const void* x_in, void* x_out, const void* b, double omega, bool fwd,
@@ -1147,5 +1148,103 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
   }
   if (iterations) *iterations = it;
   return finish(op, precon, status);
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the solve: outlet flux and its gradient w.r.t. log coefficients
+
+namespace {
+
+void check_adjoint_op(lgpu_op op) {
+  LGPU_REQUIRE(op, "NULL operator");
+  LGPU_REQUIRE(!op->is_csr && op->desc.kind == 0,
+               "the outlet-flux adjoint needs a per-cell coefficient stencil operator");
+  LGPU_REQUIRE(op->desc.adv == 0.0, "the outlet-flux adjoint needs a symmetric operator (no advection)");
+}
+
+void check_f64(lgpu_vec v, int64_t n, const char* what) {
+  check_vec(v, n, what);
+  LGPU_REQUIRE(!v->f32, std::string(what) + " must be f64");
+}
+
+double outlet_flux_value(lgpu_op op, const double* p) {
+  lgpu_ctx ctx = op->ctx;
+  const int64_t np = outlet_flux_partials(op->desc);
+  double* part = ctx->partials_for(np);
+  outlet_flux(op->desc, p, part, ctx->stream);
+  reduce_partials(part, np, ctx->scal.as<double>() + SC_FLUX, ctx->stream);
+  return read_scalar(ctx, SC_FLUX);
+}
+
+}  // namespace
+
+extern "C" int lgpu_outlet_flux(lgpu_op op, lgpu_vec p, double* flux) {
+  API_BEGIN
+  check_adjoint_op(op);
+  LGPU_REQUIRE(flux, "NULL flux");
+  check_f64(p, op->rows(), "p");
+  *flux = outlet_flux_value(op, static_cast<const double*>(p->ptr));
+  API_END
+}
+
+extern "C" int lgpu_outlet_flux_adjoint_rhs(lgpu_op op, lgpu_vec g) {
+  API_BEGIN
+  check_adjoint_op(op);
+  check_f64(g, op->rows(), "g");
+  outlet_flux_adjoint_rhs(op->desc, static_cast<double*>(g->ptr), op->ctx->stream);
+  API_END
+}
+
+extern "C" int lgpu_outlet_flux_gradient(lgpu_op op, lgpu_vec p, lgpu_vec lambda, lgpu_vec grad) {
+  API_BEGIN
+  check_adjoint_op(op);
+  const int64_t n = op->rows();
+  check_f64(p, n, "p");
+  check_f64(lambda, n, "lambda");
+  check_f64(grad, n, "grad");
+  LGPU_REQUIRE(grad->ptr != p->ptr && grad->ptr != lambda->ptr, "grad must not alias p or lambda");
+  outlet_flux_gradient(op->desc, static_cast<const double*>(p->ptr),
+                       static_cast<const double*>(lambda->ptr), static_cast<double*>(grad->ptr),
+                       op->ctx->stream);
+  API_END
+}
+
+extern "C" int lgpu_adjoint_grad(lgpu_op op, lgpu_amg precon, const lgpu_krylov_params* params,
+                                 lgpu_vec p, lgpu_vec lambda, lgpu_vec grad, double* flux,
+                                 int* iterations) {
+  API_BEGIN
+  check_adjoint_op(op);
+  LGPU_REQUIRE(params, "NULL params");
+  const int64_t n = op->rows();
+  check_f64(p, n, "p");
+  check_f64(lambda, n, "lambda");
+  check_f64(grad, n, "grad");
+  LGPU_REQUIRE(p->ptr != lambda->ptr && grad->ptr != p->ptr && grad->ptr != lambda->ptr,
+               "p, lambda and grad must not alias");
+  lgpu_ctx ctx = op->ctx;
+  lgpu_vec_s rhs_v{};
+  DevBuf rhs(size_t(n) * sizeof(double));
+  rhs_v.ctx = ctx;
+  rhs_v.n = n;
+  rhs_v.f32 = false;
+  rhs_v.ptr = rhs.ptr;
+  int it[2] = {0, 0};
+  stencil_rhs(op->desc, false, rhs.ptr, ctx->stream);
+  int status = lgpu_cg_solve(op, precon, &rhs_v, p, params, &it[0], nullptr);
+  if (status == LGPU_OK || status == LGPU_NOT_CONVERGED) {
+    outlet_flux_adjoint_rhs(op->desc, rhs.as<double>(), ctx->stream);
+    const int s2 = lgpu_cg_solve(op, precon, &rhs_v, lambda, params, &it[1], nullptr);
+    if (status == LGPU_OK) status = s2;
+    outlet_flux_gradient(op->desc, static_cast<const double*>(p->ptr),
+                         static_cast<const double*>(lambda->ptr), static_cast<double*>(grad->ptr),
+                         ctx->stream);
+    if (flux) *flux = outlet_flux_value(op, static_cast<const double*>(p->ptr));
+  }
+  if (iterations) {
+    iterations[0] = it[0];
+    iterations[1] = it[1];
+  }
+  return status;
   API_END
 }

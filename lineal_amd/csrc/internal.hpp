@@ -115,8 +115,8 @@ struct CsrDesc {
 // ---------------------------------------------------------------------------
 // kernel launchers (stream ordered)
 
-// number of partial sums written by stencil reductions
-int64_t stencil_partials(const StencilDesc& s);
+// number of partial sums a stencil reduction pass of that precision writes
+int64_t stencil_partials(const StencilDesc& s, bool f32);
 void stencil_apply(const StencilDesc& s, int mode, bool f32, const void* x, const void* b,
                    void* y, double omega, double* partials, hipStream_t st);
 void stencil_rhs(const StencilDesc& s, bool f32, void* b, hipStream_t st);
@@ -144,6 +144,14 @@ void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, dou
 constexpr int kMaxSegmentRows = 64;
 void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
                        double omega, bool forward, bool ultra, int seg, hipStream_t st);
+
+// outlet flux J = sum_{x = nx-1} (2 q_0 c_i)(p_i - p_end), its adjoint
+// right-hand side dJ/dp and the gradient dJ/dlog c (per-cell coefficients, f64)
+int64_t outlet_flux_partials(const StencilDesc& s);
+void outlet_flux(const StencilDesc& s, const double* p, double* partials, hipStream_t st);
+void outlet_flux_adjoint_rhs(const StencilDesc& s, double* g, hipStream_t st);
+void outlet_flux_gradient(const StencilDesc& s, const double* p, const double* lam, double* grad,
+                          hipStream_t st);
 
 // vector kernels
 void vec_fill(void* x, int64_t n, bool f32, double v, hipStream_t st);
@@ -176,6 +184,7 @@ enum : int {
   SC_TT = 6,      // <t, t>
   SC_ALPHA = 7,
   SC_OMEGA = 8,
+  SC_FLUX = 9,    // outlet flux
   SC_COUNT = 16,
 };
 // x += (num/den) p, r -= (num/den) q, partial += r^2

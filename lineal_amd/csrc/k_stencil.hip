@@ -638,8 +638,9 @@ void dispatch_mode(const StencilDesc& s, int mode, const void* x, const void* b,
 
 }  // namespace
 
-int64_t stencil_partials(const StencilDesc& s) {
-  return std::max(shape_of<float>(s).blocks(), shape_of<double>(s).blocks());
+int64_t stencil_partials(const StencilDesc& s, bool f32) {
+  // every block of the pass writes exactly one partial
+  return f32 ? shape_of<float>(s).blocks() : shape_of<double>(s).blocks();
 }
 
 void stencil_apply(const StencilDesc& s, int mode, bool f32, const void* x, const void* b,

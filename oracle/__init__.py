@@ -127,6 +127,9 @@ def lib():
             "or_vcycle_f32": (None, [hp, f32p, f32p, C.c_int]),
             "or_pcg": (C.c_int, [mp, hp, C.c_int, C.c_int, f64p, f64p, C.c_double, C.c_int, f64p]),
             "or_pbicgstab": (C.c_int, [mp, hp, C.c_int, C.c_int, f64p, f64p, C.c_double, C.c_int, f64p]),
+            "or_outlet_flux": (C.c_double, [sp, f64p]),
+            "or_outlet_flux_adjoint_rhs": (None, [sp, f64p]),
+            "or_outlet_flux_gradient": (None, [sp, f64p, f64p, f64p]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -377,3 +380,46 @@ def pbicgstab(A, b, hierarchy=None, precon=0, ultra=False, rel_tol=1e-8, max_it=
     it = lib().or_pbicgstab(C.byref(A.mat), hierarchy.h if hierarchy else None, precon,
                             int(ultra), _pf(b), _pf(x), rel_tol, max_it, _pf(hist))
     return x, it, hist[: it + 1]
+
+
+def _require_values(S):
+    if S.coeff is None or S.velocity != 0.0:
+        raise ValueError("the outlet-flux adjoint needs per-cell coefficients and no advection")
+
+
+def outlet_flux(S, p):
+    """J = sum over the x = nx-1 cells of (2 q_0 c_i)(p_i - p_end)."""
+    _require_values(S)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    return float(lib().or_outlet_flux(C.byref(S.s), _pf(p)))
+
+
+def outlet_flux_adjoint_rhs(S):
+    """dJ/dp: 2 q_0 c_i on the outlet cells, 0 elsewhere."""
+    _require_values(S)
+    g = np.zeros(S.N)
+    lib().or_outlet_flux_adjoint_rhs(C.byref(S.s), _pf(g))
+    return g
+
+
+def outlet_flux_gradient(S, p, lam):
+    """dJ/dlog c from the forward solution p and the adjoint solution lam."""
+    _require_values(S)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    g = np.zeros(S.N)
+    lib().or_outlet_flux_gradient(C.byref(S.s), _pf(p), _pf(lam), _pf(g))
+    return g
+
+
+def adjoint_outlet_flux(S, hierarchy=None, precon=1, rel_tol=1e-10, max_it=1000):
+    """FP64 forward and adjoint PCG solves plus the gradient: (J, grad, p, lam,
+    forward iterations, adjoint iterations)."""
+    _require_values(S)
+    b = S.rhs()
+    p, it_f, _ = pcg(S, b, hierarchy, precon=precon if hierarchy else 0, rel_tol=rel_tol,
+                     max_it=max_it)
+    g = outlet_flux_adjoint_rhs(S)
+    lam, it_a, _ = pcg(S, g, hierarchy, precon=precon if hierarchy else 0, rel_tol=rel_tol,
+                       max_it=max_it)
+    return outlet_flux(S, p), outlet_flux_gradient(S, p, lam), p, lam, it_f, it_a

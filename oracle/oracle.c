@@ -967,3 +967,71 @@ int or_pbicgstab(const or_mat* a, const or_hierarchy* h, int precon, int ultra,
   free(hv);
   return it;
 }
+
+/* ---------------------------------------------------------------------------
+ * Outlet flux and its adjoint gradient (not in the reference).
+ *
+ * J(p, c) = sum over the cells with x = nx-1 of (2 q_0 c_i) (p_i - p_end): the
+ * flux leaving through the outlet's Dirichlet half cells, built from the
+ * reference's border summands 2 q_0 c_i (valuator/stationary-diffusion-values.hpp:124-138).
+ * With A(c) p = b(c) and A^T lambda = dJ/dp (A symmetric: g_i = [x = nx-1] 2 q_0 c_i),
+ *   dJ/dlog c_k = c_k [ dJ/dc_k + lambda^T (db/dc_k - (dA/dc_k) p) ]
+ *              = c_k ( [x = nx-1] 2 q_0 (p_k - p_end)
+ *                    + lambda_k 2 q_0 ([x = 0](p_start - p_k) + [x = nx-1](p_end - p_k)) )
+ *              - sum over faces (k, j): H(c_k, c_j) q_d c_j / (c_k + c_j)
+ *                                      (lambda_k - lambda_j)(p_k - p_j),
+ * using c_k dH(c_k, c_j)/dc_k = H(c_k, c_j) c_j / (c_k + c_j).  Per-cell
+ * coefficient stencils (kind 0) without advection only. */
+
+double or_outlet_flux(const or_stencil* s, const double* p) {
+  double q[3];
+  or_axis_quotients(s, q);
+  const int64_t plane = s->n[1] * s->n[2], base = (s->n[0] - 1) * plane;
+  double sum = 0;
+  for (int64_t t = 0; t < plane; ++t) {
+    const int64_t i = base + t;
+    sum += ((2.0 * q[0]) * s->coeff[i]) * (p[i] - s->p_end);
+  }
+  return sum;
+}
+
+void or_outlet_flux_adjoint_rhs(const or_stencil* s, double* g) {
+  double q[3];
+  or_axis_quotients(s, q);
+  const int64_t plane = s->n[1] * s->n[2], n = s->n[0] * plane, base = n - plane;
+  for (int64_t i = 0; i < n; ++i) g[i] = i >= base ? (2.0 * q[0]) * s->coeff[i] : 0.0;
+}
+
+void or_outlet_flux_gradient(const or_stencil* s, const double* p, const double* lam,
+                             double* grad) {
+  double q[3];
+  or_axis_quotients(s, q);
+  const int64_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  const int64_t off[3] = {ny * nz, nz, 1};
+  const double b2 = 2.0 * q[0];
+  for (int64_t x = 0; x < nx; ++x)
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t z = 0; z < nz; ++z) {
+        const int64_t k = (x * ny + y) * nz + z;
+        const int64_t pos[3] = {x, y, z};
+        const double ck = s->coeff[k], pk = p[k], lk = lam[k];
+        double acc = 0;
+        if (x + 1 == nx) acc = b2 * (pk - s->p_end);
+        if (x == 0) acc += lk * (b2 * (s->p_start - pk));
+        if (x + 1 == nx) acc += lk * (b2 * (s->p_end - pk));
+        acc = ck * acc;
+        /* faces in the row order -x, -y, -z, +z, +y, +x */
+        for (int e = 0; e < 6; ++e) {
+          const int d = e < 3 ? e : 5 - e;
+          const int up = e >= 3;
+          if (up ? pos[d] + 1 >= s->n[d] : pos[d] == 0) continue;
+          const int64_t j = up ? k + off[d] : k - off[d];
+          const double cj = s->coeff[j];
+          const double w = or_harmonic_mean_f64(ck, cj) * q[d];
+          if (w == 0.0) continue;
+          const double t = w * (cj / (ck + cj));
+          acc -= t * ((lk - lam[j]) * (pk - p[j]));
+        }
+        grad[k] = acc;
+      }
+}

@@ -160,6 +160,15 @@ int or_pcg(const or_mat* a, const or_hierarchy* h, int precon, int ultra, const 
 int or_pbicgstab(const or_mat* a, const or_hierarchy* h, int precon, int ultra,
                  const double* b, double* x, double rel_tol, int max_it, double* res_hist);
 
+/* Outlet flux J = sum_{x = nx-1} (2 q_0 c_i)(p_i - p_end) of a per-cell
+ * coefficient stencil, the adjoint right-hand side dJ/dp, and the gradient
+ * dJ/dlog c given the forward solution p and the adjoint solution lambda
+ * (not in the reference; derivation in oracle.c). */
+double or_outlet_flux(const or_stencil* s, const double* p);
+void or_outlet_flux_adjoint_rhs(const or_stencil* s, double* g);
+void or_outlet_flux_gradient(const or_stencil* s, const double* p, const double* lam,
+                             double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -191,3 +191,18 @@ def test_bicgstab_advection_two_phase_matches_oracle(dims):
     assert np.abs(x.numpy() - x_ref).max() / np.abs(x_ref).max() <= 1e-6
     r = O.residual(S_adv, b, x.numpy())
     assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("dims", [(256, 256, 256), (256, 128, 256)])
+def test_cg_residual_history_full_size(dims):
+    """Unpreconditioned CG at full size against the oracle: every reduction
+    (<p, A p> fused into the stencil pass, ||r||^2 fused into the update) must
+    cover exactly the partial sums its own pass wrote.  At these shapes the
+    f32 and f64 stencil passes use different block counts."""
+    A, S = pair(dims, "values")
+    b = S.rhs()
+    x = la.DenseVector(S.N, fill=0.0)
+    res = la.ConjugateGradientsSolver(8).solve(A, x, la.DenseVector(b), rel_tol=1e-30, max_it=10)
+    _, it, hist = O.pcg(S, b, None, precon=0, rel_tol=1e-30, max_it=10)
+    assert res.iterations == it == 10
+    np.testing.assert_allclose(res.residuals, hist, rtol=1e-9)

@@ -556,3 +556,95 @@ def test_advection_closed_form_one_dimensional(velocity, coef):
     # the oracle's SpMV agrees: A p_exact reproduces the rhs
     r = O.residual(S, S.rhs(), np.repeat(exact[:, None], 6, axis=1).reshape(-1))
     assert np.abs(r).max() <= 1e-12 * np.abs(S.rhs()).max()
+
+
+# ---------------------------------------------------------------------------
+# outlet flux and its adjoint gradient (not in the reference)
+
+
+def dense_rhs(dims, coeff, lengths, p_start, p_end):
+    nx, ny, nz = dims
+    h = [lengths[d] / dims[d] for d in range(3)]
+    q0 = h[1] * h[2] / h[0]
+    b = np.zeros(nx * ny * nz)
+    plane = ny * nz
+    b[:plane] += 2 * q0 * coeff[:plane] * p_start
+    b[-plane:] += 2 * q0 * coeff[-plane:] * p_end
+    return b, q0
+
+
+def dense_flux(dims, coeff, lengths, p_start, p_end):
+    A = dense_operator(dims, coeff, lengths)
+    b, q0 = dense_rhs(dims, coeff, lengths, p_start, p_end)
+    p = np.linalg.solve(A, b)
+    plane = dims[1] * dims[2]
+    return float(np.sum(2 * q0 * coeff[-plane:] * (p[-plane:] - p_end))), p, A, q0
+
+
+@pytest.mark.parametrize("dims,lengths,ps,pe", [((9, 1, 1), (9, 1, 1), 1.0, 0.0),
+                                                ((7, 3, 2), (3.5, 1.5, 4.0), 2.0, 0.5)])
+def test_outlet_flux_gradient_closed_form_columns(dims, lengths, ps, pe):
+    """A field constant across y and z: every column is a series of
+    resistances 1/c_i (the half faces of H sum to 1/(2c_i) + 1/(2c_j)), so
+    J = n_y n_z q_0 dp / R with R = sum_i 1/c_i and, per cell,
+    dJ/dlog c_k = J_col^2 / (q_0 dp c_k) with J_col = q_0 dp / R."""
+    nx, ny, nz = dims
+    col = problems.lognormal((nx, 1, 1), 1.0, 4)
+    c = np.repeat(col, ny * nz)
+    S = O.Stencil(dims, coeff=c, lengths=lengths, p_start=ps, p_end=pe)
+    A = dense_operator(dims, c, lengths)
+    p = np.linalg.solve(A, S.rhs())
+    lam = np.linalg.solve(A, O.outlet_flux_adjoint_rhs(S))
+    q0 = S.quotients()[0]
+    dp = ps - pe
+    R = np.sum(1.0 / col)
+    j_col = q0 * dp / R
+    assert O.outlet_flux(S, p) == pytest.approx(ny * nz * j_col, rel=1e-13)
+    expect = np.repeat(j_col ** 2 / (q0 * dp * col), ny * nz)
+    np.testing.assert_allclose(O.outlet_flux_gradient(S, p, lam), expect, rtol=1e-11)
+
+
+def test_outlet_flux_adjoint_rhs_is_the_flux_derivative():
+    dims, lengths = (4, 3, 5), (2.0, 3.0, 2.5)
+    S, c = lognormal_stencil(dims, lengths=lengths, p_start=1.5, p_end=0.25)
+    g = O.outlet_flux_adjoint_rhs(S)
+    rng = np.random.default_rng(3)
+    p, dp = rng.standard_normal(S.N), rng.standard_normal(S.N)
+    # J is affine in p: J(p + dp) - J(p) = g . dp
+    assert O.outlet_flux(S, p + dp) - O.outlet_flux(S, p) == pytest.approx(g @ dp, rel=1e-12)
+
+
+@pytest.mark.parametrize("dims,lengths", [((4, 3, 5), (2.0, 3.0, 2.5)), ((5, 4, 4), None)])
+def test_outlet_flux_gradient_matches_dense_central_differences(dims, lengths):
+    """Every cell's gradient against central differences of J computed by
+    dense solves (h = 1e-5 in log c); covers the y and z faces, which the
+    column closed form leaves at zero."""
+    L = lengths or dims
+    S, c = lognormal_stencil(dims, lengths=lengths, p_start=1.5, p_end=0.25)
+    J, p, A, _ = dense_flux(dims, c, L, 1.5, 0.25)
+    lam = np.linalg.solve(A.T, O.outlet_flux_adjoint_rhs(S))
+    g = O.outlet_flux_gradient(S, p, lam)
+    assert O.outlet_flux(S, p) == pytest.approx(J, rel=1e-13)
+    h = 1e-5
+    fd = np.zeros(S.N)
+    for k in range(S.N):
+        cp, cm = c.copy(), c.copy()
+        cp[k] *= np.exp(h)
+        cm[k] *= np.exp(-h)
+        fd[k] = (dense_flux(dims, cp, L, 1.5, 0.25)[0] - dense_flux(dims, cm, L, 1.5, 0.25)[0]) / (2 * h)
+    assert np.abs(g - fd).max() <= 1e-7 * np.abs(fd).max()
+    assert np.abs(fd).min() > 0  # every cell influences the flux
+
+
+def test_adjoint_outlet_flux_driver_with_amg():
+    dims = (12, 10, 8)
+    S, c = lognormal_stencil(dims)
+    H = O.Hierarchy(S)
+    J, g, p, lam, itf, ita = O.adjoint_outlet_flux(S, H, rel_tol=1e-12)
+    A = dense_operator(dims, c)
+    p_ref = np.linalg.solve(A, S.rhs())
+    lam_ref = np.linalg.solve(A, O.outlet_flux_adjoint_rhs(S))
+    g_ref = O.outlet_flux_gradient(S, p_ref, lam_ref)
+    assert np.abs(g - g_ref).max() <= 1e-9 * np.abs(g_ref).max()
+    assert J == pytest.approx(O.outlet_flux(S, p_ref), rel=1e-11)
+    assert 0 < itf < 100 and 0 < ita < 100
