@@ -25,6 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the finest-level f32 SOR sweeps at 256^3: 2 x
+# FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc runs (the factor 2 is what a plain
+# copy kernel of known size gives), profiles/stencil_kernels_256_hbm_counters.md
+TRAFFIC_256 = {"stencil_sor_fwd": 688_138_957, "stencil_sor_bwd": 849_919_078}
 
 
 def log(msg):
@@ -117,6 +121,9 @@ def main():
         r = solve(M32)
         log(f"warmup mixed: {r.iterations} it, status {r.status}")
     ctx.synchronize()
+    # every launch of the hot kernel classes is bracketed by HIP events on the
+    # library's stream during the timed solves
+    ctx.kernel_timing(True)
     t0 = time.perf_counter()
     its32 = []
     for _ in range(args.steps):
@@ -125,6 +132,17 @@ def main():
         assert r.converged, r
     ctx.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
+    classes = {}
+    for name, cls in (("stencil_sor_fwd", la.KT_STENCIL_SOR_FWD), ("stencil_sor_bwd", la.KT_STENCIL_SOR_BWD),
+                      ("stencil_pass", la.KT_STENCIL_PASS), ("csr_sor", la.KT_CSR_SOR),
+                      ("csr_pass", la.KT_CSR_PASS)):
+        ms, cnt = ctx.kernel_time(cls)
+        classes[name] = {"ms_total_per_solve": ms / args.steps, "launches_per_solve": cnt / args.steps,
+                         "ms_per_launch": ms / max(cnt, 1)}
+    ctx.kernel_timing(False)
+    for k, v in classes.items():
+        log(f"timed region {k}: {v['ms_total_per_solve']:.1f} ms/solve over {v['launches_per_solve']:.0f} "
+            f"launches, {v['ms_per_launch'] * 1e3:.1f} us each")
     x_mixed = x.numpy()
     log(f"mixed: {t_mixed * 1e3:.1f} ms/solve, iterations {its32}")
 
@@ -167,9 +185,19 @@ def main():
     for k, v in kern.items():
         log(f"{k}: {v['ms']:.3f} ms, {v['GB/s']:.0f} GB/s")
 
-    # the dominant kernel of the mixed solve is the finest-level SOR sweep
-    dom_ms, dom_bytes = t_sor32, sor32_bytes
+    # The dominant kernel of the mixed solve: the finest-level SOR sweep in
+    # f32 (a forward sweep from zero as pre-smoother, a backward sweep from the
+    # prolonged iterate as post-smoother).  Algorithmic bytes per launch: the
+    # f32 coefficient and right-hand side read, x_out written, plus x_in read
+    # by the backward sweep, each cell once.
+    dom = max(("stencil_sor_fwd", "stencil_sor_bwd"), key=lambda k: classes[k]["ms_total_per_solve"])
+    dom_bytes = (12 if dom == "stencil_sor_fwd" else 16) * N
+    dom_ms = classes[dom]["ms_per_launch"]
     achieved = dom_bytes / dom_ms / 1e6
+    # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+    # the same kernels at 256^3 (python -m lineal_amd.profile_kernels), FETCH_SIZE
+    # scaled by the factor a plain copy kernel of known size gives
+    traffic = TRAFFIC_256.get(dom) if n == 256 else None
     result = {
         "metric": "AMG-CG time-to-1e-8 at 256^3, mixed vs FP64 (s); SpMV HBM GB/s vs peak",
         "value": t_mixed,
@@ -198,6 +226,7 @@ def main():
             "setup_seconds_fp64": M64.setup_seconds,
             "levels": [M32.level_size(l) for l in range(M32.level_num)],
             "kernels": kern,
+            "timed_region_kernel_classes": classes,
             "spmv_roofline": {"bound": "hbm", "achieved": spmv_bytes / t_spmv / 1e6, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": spmv_bytes / t_spmv / 1e6 / HBM_PEAK_GBS},
         },
@@ -207,8 +236,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "sor_wavefront<float, values, forward, regular, x_in=0> (finest-level pre-smoother)",
+            "traffic": traffic,
+            "kernel": {"stencil_sor_fwd": "sor_wavefront<float, values, forward, regular, x_in = 0> "
+                                          "(finest-level pre-smoother)",
+                       "stencil_sor_bwd": "sor_wavefront<float, values, backward, regular, x_in> "
+                                          "(finest-level post-smoother)"}[dom],
+            "algorithmic_bytes": dom_bytes,
+            "ms_per_launch": dom_ms,
         },
     }
     if not args.no_cpu_baseline:

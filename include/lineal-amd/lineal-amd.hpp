@@ -1,0 +1,571 @@
+// Header-only C++20 classes over the lineal_amd C ABI.  They mirror the
+// reference's matrix, vector and solver interfaces (paths relative to the
+// reference's include/lineal/) so that code written against those interfaces
+// can switch to the GPU classes:
+//   GpuDenseVector<Real>        DenseVector        tensor/dynamic/dense/vector/shared.hpp:35-362
+//   GpuStencilValueMatrix       AdjacentStencilMatrix over StationaryDiffusionValuesValuator
+//                               tensor/dynamic/stencil/matrix/shared.hpp:35-360,
+//                               tensor/dynamic/stencil/valuator/stationary-diffusion-values.hpp:31-211
+//   GpuStencilLookupMatrix      ... over StationaryDiffusionLookupValuator
+//                               tensor/dynamic/stencil/valuator/stationary-diffusion-lookup.hpp:29-151
+//   GpuCsrMatrix                CsrMatrix          tensor/dynamic/csr/shared.hpp:37-1103
+//   GpuJacobiSolver             JacobiSolver       linear-solver/iterative-stationary/jacobi/solver.hpp:21-39
+//   GpuSorSolver                SorSolver          linear-solver/iterative-stationary/gauss-seidel-variant/solver.hpp:22-44
+//   GpuMultiGridSolver          MultiGridSolver    linear-solver/multigrid/solver.hpp:29-196
+//   GpuConjugateGradients       ConjugateGradientsSolver  linear-solver/conjugate-gradients/solver.hpp:25-221
+//   GpuBiCgStab                 BiCgStabSolver     linear-solver/bicgstab/solver.hpp:26-259
+//   run_iterative_solver        linear-solver/runner.hpp:17-84
+// Errors: every failing C call throws GpuError (the reference throws C++
+// exceptions for structural errors; NaN propagation becomes a status).
+// Objects of one GpuContext are used from one host thread; all work is
+// ordered on the context's stream.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "lineal_amd.h"
+
+namespace lineal::gpu {
+
+class GpuError : public std::runtime_error {
+ public:
+  GpuError(int status, const std::string& what)
+      : std::runtime_error(std::string(lgpu_status_string(status)) + ": " + what), status_(status) {}
+  [[nodiscard]] int status() const { return status_; }
+
+ private:
+  int status_;
+};
+
+inline void check(int status) {
+  if (status != LGPU_OK) throw GpuError(status, lgpu_last_error());
+}
+
+// One device and one HIP stream: the GPU counterpart of the reference Env's
+// execution policy (parallel/execution-policy/adaptive.hpp:146-349).
+class GpuContext {
+ public:
+  explicit GpuContext(int device = 0) {
+    lgpu_ctx h = nullptr;
+    check(lgpu_ctx_create(device, &h));
+    h_.reset(h);
+  }
+  void synchronize() const { check(lgpu_ctx_synchronize(h_.get())); }
+  [[nodiscard]] lgpu_ctx handle() const { return h_.get(); }
+
+ private:
+  struct Del {
+    void operator()(lgpu_ctx c) const { lgpu_ctx_destroy(c); }
+  };
+  std::unique_ptr<lgpu_ctx_s, Del> h_;
+};
+
+template <typename TReal>
+class GpuDenseVector {
+  static_assert(std::is_same_v<TReal, double> || std::is_same_v<TReal, float>);
+
+ public:
+  using Value = TReal;
+  using Size = std::size_t;
+  static constexpr int dtype = std::is_same_v<TReal, double> ? LGPU_F64 : LGPU_F32;
+
+  GpuDenseVector(Size size, const GpuContext& ctx) : ctx_(&ctx) {
+    lgpu_vec h = nullptr;
+    check(lgpu_vec_create(ctx.handle(), int64_t(size), dtype, &h));
+    h_.reset(h);
+  }
+  GpuDenseVector(Size size, Value value, const GpuContext& ctx) : GpuDenseVector(size, ctx) {
+    fill(value);
+  }
+  GpuDenseVector(std::span<const Value> host, const GpuContext& ctx)
+      : GpuDenseVector(host.size(), ctx) {
+    upload(host);
+  }
+  GpuDenseVector(GpuDenseVector&&) noexcept = default;
+  GpuDenseVector& operator=(GpuDenseVector&&) noexcept = default;
+  // copies are deep, as for the reference's owned vectors
+  GpuDenseVector(const GpuDenseVector& o) : GpuDenseVector(o.size(), *o.ctx_) { assign(o); }
+  GpuDenseVector& operator=(const GpuDenseVector& o) {
+    if (this != &o) assign(o);
+    return *this;
+  }
+
+  [[nodiscard]] Size size() const {
+    int64_t n = 0;
+    check(lgpu_vec_size(h_.get(), &n));
+    return Size(n);
+  }
+  void fill(Value v) { check(lgpu_vec_fill(h_.get(), double(v))); }
+  template <typename TOther>
+  void assign(const GpuDenseVector<TOther>& src) {
+    check(lgpu_vec_copy(src.handle(), h_.get()));
+  }
+  void upload(std::span<const Value> host) {
+    check(lgpu_vec_upload(h_.get(), host.data(), int64_t(host.size())));
+  }
+  void download(std::span<Value> host) const {
+    check(lgpu_vec_download(h_.get(), host.data(), int64_t(host.size())));
+  }
+  [[nodiscard]] std::vector<Value> to_host() const {
+    std::vector<Value> v(size());
+    download(v);
+    return v;
+  }
+  // device pointer of the storage
+  [[nodiscard]] Value* data() { return static_cast<Value*>(lgpu_vec_data(h_.get())); }
+  [[nodiscard]] const Value* data() const {
+    return static_cast<const Value*>(lgpu_vec_data(h_.get()));
+  }
+  [[nodiscard]] lgpu_vec handle() const { return h_.get(); }
+  [[nodiscard]] const GpuContext& context() const { return *ctx_; }
+
+ private:
+  struct Del {
+    void operator()(lgpu_vec v) const { lgpu_vec_destroy(v); }
+  };
+  const GpuContext* ctx_;
+  std::unique_ptr<lgpu_vec_s, Del> h_;
+};
+
+// dot<double>(a, b), component-wise/sink/accumulator.hpp:175-190
+template <typename T>
+inline double dot(const GpuDenseVector<T>& a, const GpuDenseVector<T>& b) {
+  double out = 0;
+  check(lgpu_vec_dot(a.handle(), b.handle(), &out));
+  return out;
+}
+
+// Box geometry and Dirichlet data (discretization/box-equidistant.hpp:23-111,
+// tensor/dynamic/stencil/matrix/diffusion.hpp:52-96), flow axis 0.
+struct BoxInfo {
+  std::array<int64_t, 3> axis_sizes;
+  std::array<double, 3> axis_lengths;
+  double solution_start = 1.0;
+  double solution_end = 0.0;
+  bool zero_to_one = true;
+  double velocity_x = 0.0;  // upwind advection along +x (not in the reference)
+
+  explicit BoxInfo(std::array<int64_t, 3> sizes) : axis_sizes(sizes) {
+    for (int d = 0; d < 3; ++d) axis_lengths[size_t(d)] = double(sizes[size_t(d)]);
+  }
+  BoxInfo(std::array<int64_t, 3> sizes, std::array<double, 3> lengths)
+      : axis_sizes(sizes), axis_lengths(lengths) {}
+  [[nodiscard]] int64_t cell_num() const { return axis_sizes[0] * axis_sizes[1] * axis_sizes[2]; }
+  [[nodiscard]] lgpu_box c_box() const {
+    lgpu_box b{};
+    for (int d = 0; d < 3; ++d) {
+      b.n[d] = axis_sizes[size_t(d)];
+      b.len[d] = axis_lengths[size_t(d)];
+    }
+    b.zero_to_one = zero_to_one ? 1 : 0;
+    b.p_start = solution_start;
+    b.p_end = solution_end;
+    b.velocity_x = velocity_x;
+    return b;
+  }
+};
+
+// Common operations of the GPU matrices: the reference's SharedMatrix
+// queries plus the products the solvers use.
+class GpuMatrixBase {
+ public:
+  [[nodiscard]] std::size_t row_num() const {
+    int64_t n = 0;
+    check(lgpu_op_rows(h_.get(), &n));
+    return std::size_t(n);
+  }
+  [[nodiscard]] std::size_t column_num() const { return row_num(); }
+  [[nodiscard]] bool is_symmetric() const { return symmetric_; }
+  [[nodiscard]] lgpu_op handle() const { return h_.get(); }
+  [[nodiscard]] const GpuContext& context() const { return *ctx_; }
+
+  // y = A x, MatrixVectorProductExpr (component-wise/matrix-vector-product.hpp:34-140)
+  template <typename T>
+  void multiply(const GpuDenseVector<T>& x, GpuDenseVector<T>& y) const {
+    check(lgpu_op_spmv(h_.get(), x.handle(), y.handle()));
+  }
+  template <typename T>
+  GpuDenseVector<T> operator*(const GpuDenseVector<T>& x) const {
+    GpuDenseVector<T> y(x.size(), *ctx_);
+    multiply(x, y);
+    return y;
+  }
+  // r = b - A x
+  template <typename T>
+  void residual(const GpuDenseVector<T>& b, const GpuDenseVector<T>& x, GpuDenseVector<T>& r) const {
+    check(lgpu_op_residual(h_.get(), b.handle(), x.handle(), r.handle()));
+  }
+
+ protected:
+  GpuMatrixBase(const GpuContext& ctx, lgpu_op h, bool symmetric)
+      : ctx_(&ctx), h_(h), symmetric_(symmetric) {}
+  struct Del {
+    void operator()(lgpu_op o) const { lgpu_op_destroy(o); }
+  };
+  const GpuContext* ctx_;
+  std::unique_ptr<lgpu_op_s, Del> h_;
+  bool symmetric_;
+};
+
+// Matrix-free stencil with the Dirichlet right-hand side of its valuator
+// (AdjacentStencilVector, tensor/dynamic/stencil/vector/shared.hpp:181-205).
+class GpuStencilMatrixBase : public GpuMatrixBase {
+ public:
+  template <typename T = double>
+  [[nodiscard]] GpuDenseVector<T> rhs() const {
+    GpuDenseVector<T> b(row_num(), *ctx_);
+    check(lgpu_op_rhs(h_.get(), b.handle()));
+    return b;
+  }
+  [[nodiscard]] const BoxInfo& info() const { return box_; }
+
+ protected:
+  GpuStencilMatrixBase(const GpuContext& ctx, lgpu_op h, BoxInfo box)
+      : GpuMatrixBase(ctx, h, box.velocity_x == 0.0), box_(box) {}
+  BoxInfo box_;
+
+  static lgpu_op make_values(const GpuContext& ctx, const BoxInfo& box,
+                             std::span<const double> coeff) {
+    if (std::int64_t(coeff.size()) != box.cell_num())
+      throw GpuError(LGPU_INVALID_ARG, "one coefficient per cell");
+    lgpu_box b = box.c_box();
+    lgpu_op h = nullptr;
+    check(lgpu_stencil_values_create(ctx.handle(), &b, coeff.data(), &h));
+    return h;
+  }
+  static lgpu_op make_lookup(const GpuContext& ctx, const BoxInfo& box,
+                             std::span<const std::uint8_t> materials,
+                             const std::array<double, 4>& coeffs) {
+    if (std::int64_t(materials.size()) != box.cell_num())
+      throw GpuError(LGPU_INVALID_ARG, "one material id per cell");
+    lgpu_box b = box.c_box();
+    lgpu_op h = nullptr;
+    check(lgpu_stencil_lookup_create(ctx.handle(), &b, materials.data(), coeffs.data(), &h));
+    return h;
+  }
+};
+
+class GpuStencilValueMatrix : public GpuStencilMatrixBase {
+ public:
+  GpuStencilValueMatrix(const BoxInfo& box, std::span<const double> coeff, const GpuContext& ctx)
+      : GpuStencilMatrixBase(ctx, make_values(ctx, box, coeff), box) {}
+};
+
+class GpuStencilLookupMatrix : public GpuStencilMatrixBase {
+ public:
+  GpuStencilLookupMatrix(const BoxInfo& box, std::span<const std::uint8_t> materials,
+                         const std::array<double, 4>& material_coeffs, const GpuContext& ctx)
+      : GpuStencilMatrixBase(ctx, make_lookup(ctx, box, materials, material_coeffs), box) {}
+};
+
+// CSR with sorted columns and the diagonal stored; values kept in TReal.
+template <typename TReal = double>
+class GpuCsrMatrix : public GpuMatrixBase {
+ public:
+  using Value = TReal;
+  GpuCsrMatrix(std::span<const int64_t> rowptr, std::span<const int32_t> cols,
+               std::span<const double> vals, bool symmetric, const GpuContext& ctx)
+      : GpuMatrixBase(ctx, make(ctx, rowptr, cols, vals, symmetric), symmetric) {}
+  // csr_from_adjacent_stencil (tensor/dynamic/csr/create-from.hpp:37-239)
+  explicit GpuCsrMatrix(const GpuStencilMatrixBase& stencil)
+      : GpuMatrixBase(stencil.context(), from_stencil(stencil), stencil.is_symmetric()) {}
+
+ private:
+  static constexpr int dtype = std::is_same_v<TReal, double> ? LGPU_F64 : LGPU_F32;
+  static lgpu_op make(const GpuContext& ctx, std::span<const int64_t> rowptr,
+                      std::span<const int32_t> cols, std::span<const double> vals, bool symmetric) {
+    if (rowptr.empty()) throw GpuError(LGPU_INVALID_ARG, "empty row pointer");
+    lgpu_op h = nullptr;
+    check(lgpu_csr_create(ctx.handle(), int64_t(rowptr.size()) - 1, rowptr.data(), cols.data(),
+                          vals.data(), dtype, symmetric ? 1 : 0, &h));
+    return h;
+  }
+  static lgpu_op from_stencil(const GpuStencilMatrixBase& s) {
+    lgpu_op h = nullptr;
+    check(lgpu_csr_from_stencil(s.handle(), dtype, &h));
+    return h;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// stationary iterative solvers: instantiate(lhs) -> Instance with ex-situ
+// apply(sol_in, sol_out, rhs) (broom.hpp:188-244); a null sol_in is the
+// constant-zero initial guess.
+
+enum class SorDirection : bool { BACKWARD = false, FORWARD = true };
+enum class SorVariant : bool { REGULAR = false, ULTRA = true };
+
+template <typename TReal>
+class GpuJacobiSolver {
+ public:
+  explicit GpuJacobiSolver(TReal relax) : relax_(relax) {}
+  class Instance {
+   public:
+    Instance(const GpuMatrixBase& lhs, TReal relax) : lhs_(&lhs), relax_(relax) {}
+    void apply(const GpuDenseVector<TReal>* sol_in, GpuDenseVector<TReal>& sol_out,
+               const GpuDenseVector<TReal>& rhs) const {
+      check(lgpu_op_jacobi(lhs_->handle(), sol_in ? sol_in->handle() : nullptr, sol_out.handle(),
+                           rhs.handle(), double(relax_)));
+    }
+
+   private:
+    const GpuMatrixBase* lhs_;
+    TReal relax_;
+  };
+  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const { return Instance(lhs, relax_); }
+
+ private:
+  TReal relax_;
+};
+
+template <typename TReal, SorDirection tDir = SorDirection::FORWARD,
+          SorVariant tVariant = SorVariant::REGULAR>
+class GpuSorSolver {
+ public:
+  explicit GpuSorSolver(TReal relax) : relax_(relax) {}
+  class Instance {
+   public:
+    Instance(const GpuMatrixBase& lhs, TReal relax) : lhs_(&lhs), relax_(relax) {}
+    // one lexicographic sweep (one thread segment)
+    void apply(const GpuDenseVector<TReal>* sol_in, GpuDenseVector<TReal>& sol_out,
+               const GpuDenseVector<TReal>& rhs) const {
+      check(lgpu_op_sor(lhs_->handle(), sol_in ? sol_in->handle() : nullptr, sol_out.handle(),
+                        rhs.handle(), double(relax_), tDir == SorDirection::FORWARD ? 1 : 0,
+                        tVariant == SorVariant::ULTRA ? 1 : 0));
+    }
+    // the reference's thread-segment semantics with segments of segment_rows
+    // rows (CSR operators)
+    void apply_segmented(const GpuDenseVector<TReal>* sol_in, GpuDenseVector<TReal>& sol_out,
+                         const GpuDenseVector<TReal>& rhs, int64_t segment_rows) const {
+      check(lgpu_op_sor_segmented(lhs_->handle(), sol_in ? sol_in->handle() : nullptr,
+                                  sol_out.handle(), rhs.handle(), double(relax_),
+                                  tDir == SorDirection::FORWARD ? 1 : 0,
+                                  tVariant == SorVariant::ULTRA ? 1 : 0, segment_rows));
+    }
+
+   private:
+    const GpuMatrixBase* lhs_;
+    TReal relax_;
+  };
+  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const { return Instance(lhs, relax_); }
+
+ private:
+  TReal relax_;
+};
+
+// ---------------------------------------------------------------------------
+// aggregation AMG
+
+struct HierarchyWrightParams {
+  int max_level_num = 16;
+  int64_t min_coarsen_size = 32;
+  double min_coarsen_factor = 1.2;
+  double strong_threshold = 1.0 / 3.0;
+  double aggressive_strong_threshold = 1.0 / 3.0;
+  int64_t min_aggregate_size = 6;
+  int64_t max_aggregate_size = 9;
+};
+
+// MultiGridSolver{relax = 1, DefaultCycle{REGULAR, 1}, wright} with SOR(1)
+// pre/post smoothers and a Cholesky coarsest solver.  TLevelReal selects the
+// hierarchy's precision: float gives the mixed-precision preconditioner
+// (FP32 hierarchy, smoothers and level operators; FP64 coarsest solve).
+template <typename TLevelReal = double>
+class GpuMultiGridSolver {
+ public:
+  explicit GpuMultiGridSolver(HierarchyWrightParams params = {},
+                              SorVariant variant = SorVariant::REGULAR,
+                              int64_t coarse_segment_rows = 64)
+      : params_(params), variant_(variant), coarse_segment_rows_(coarse_segment_rows) {}
+
+  class Instance {
+   public:
+    Instance(const GpuMatrixBase& lhs, const GpuMultiGridSolver& s) : lhs_(&lhs) {
+      lgpu_amg_params p;
+      lgpu_amg_default_params(&p);
+      p.max_level_num = s.params_.max_level_num;
+      p.min_coarsen_size = s.params_.min_coarsen_size;
+      p.min_coarsen_factor = s.params_.min_coarsen_factor;
+      p.strong_threshold = s.params_.strong_threshold;
+      p.aggressive_strong_threshold = s.params_.aggressive_strong_threshold;
+      p.min_aggregate_size = s.params_.min_aggregate_size;
+      p.max_aggregate_size = s.params_.max_aggregate_size;
+      p.precision = std::is_same_v<TLevelReal, float> ? LGPU_F32 : LGPU_F64;
+      p.ultra = s.variant_ == SorVariant::ULTRA ? 1 : 0;
+      p.coarse_segment_rows = s.coarse_segment_rows_;
+      lgpu_amg h = nullptr;
+      check(lgpu_amg_create(lhs.handle(), &p, &h));
+      h_.reset(h);
+    }
+    // sol = M rhs, one V-cycle from a constant-zero initial guess
+    // (multigrid/solver.hpp:47-53)
+    void apply(GpuDenseVector<double>& sol, const GpuDenseVector<double>& rhs) const {
+      check(lgpu_amg_vcycle(h_.get(), rhs.handle(), sol.handle()));
+    }
+    [[nodiscard]] int level_num() const {
+      int l = 0;
+      check(lgpu_amg_level_num(h_.get(), &l));
+      return l;
+    }
+    [[nodiscard]] std::size_t level_size(int level) const {
+      int64_t n = 0;
+      check(lgpu_amg_level_size(h_.get(), level, &n));
+      return std::size_t(n);
+    }
+    [[nodiscard]] lgpu_amg handle() const { return h_.get(); }
+    [[nodiscard]] const GpuMatrixBase& lhs() const { return *lhs_; }
+
+   private:
+    struct Del {
+      void operator()(lgpu_amg a) const { lgpu_amg_destroy(a); }
+    };
+    const GpuMatrixBase* lhs_;
+    std::unique_ptr<lgpu_amg_s, Del> h_;
+  };
+  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const { return Instance(lhs, *this); }
+
+ private:
+  HierarchyWrightParams params_;
+  SorVariant variant_;
+  int64_t coarse_segment_rows_;
+};
+
+// ---------------------------------------------------------------------------
+// Krylov solvers: instantiate(lhs) -> Instance; instance.instantiate(sol, rhs)
+// -> SystemInstance whose solve() runs the iteration on the device until the
+// stop rule holds (run_iterative_solver with a relative-residual callback).
+
+struct StopRule {
+  double rel_tol = 1e-8;
+  int max_it = 1000;
+};
+
+struct SolveReport {
+  int status = LGPU_OK;
+  int iterations = 0;
+  std::vector<double> residual_norms;  // ||r_k||_2, k = 0..iterations
+  [[nodiscard]] bool converged() const { return status == LGPU_OK; }
+  [[nodiscard]] double residual_norm() const {
+    return residual_norms.empty() ? 0.0 : residual_norms.back();
+  }
+};
+
+namespace detail {
+using KrylovFn = int (*)(lgpu_op, lgpu_amg, lgpu_vec, lgpu_vec, const lgpu_krylov_params*, int*,
+                         double*);
+
+template <KrylovFn tFn, typename TPrecon>
+class KrylovSolver {
+ public:
+  // ConjugateGradientsSolver{type_tag<Defs>, rere_controller, precon}: the
+  // true residual every `rere_every` iterations (test/aux/iterative.hpp:97-99)
+  explicit KrylovSolver(int rere_every = 8, const TPrecon* precon = nullptr)
+      : rere_every_(rere_every), precon_(precon) {}
+
+  class Instance {
+   public:
+    Instance(const GpuMatrixBase& lhs, const KrylovSolver& s)
+        : lhs_(&lhs), rere_every_(s.rere_every_) {
+      if (s.precon_) precon_.emplace_back(s.precon_->instantiate(lhs));
+    }
+    class SystemInstance {
+     public:
+      SystemInstance(const Instance& inst, GpuDenseVector<double>& sol,
+                     const GpuDenseVector<double>& rhs)
+          : inst_(&inst), sol_(&sol), rhs_(&rhs) {}
+      SolveReport solve(StopRule stop) {
+        lgpu_krylov_params p{stop.rel_tol, stop.max_it, inst_->rere_every_};
+        SolveReport r;
+        r.residual_norms.resize(size_t(stop.max_it) + 1);
+        const int st = tFn(inst_->lhs_->handle(),
+                           inst_->precon_.empty() ? nullptr : inst_->precon_.front().handle(),
+                           rhs_->handle(), sol_->handle(), &p, &r.iterations,
+                           r.residual_norms.data());
+        if (st != LGPU_OK && st != LGPU_NOT_CONVERGED && st != LGPU_BREAKDOWN &&
+            st != LGPU_NONFINITE)
+          check(st);
+        r.status = st;
+        r.residual_norms.resize(size_t(r.iterations) + 1);
+        return r;
+      }
+      [[nodiscard]] const GpuDenseVector<double>& solution() const { return *sol_; }
+
+     private:
+      const Instance* inst_;
+      GpuDenseVector<double>* sol_;
+      const GpuDenseVector<double>* rhs_;
+    };
+    [[nodiscard]] SystemInstance instantiate(GpuDenseVector<double>& sol,
+                                             const GpuDenseVector<double>& rhs) const {
+      return SystemInstance(*this, sol, rhs);
+    }
+
+   private:
+    const GpuMatrixBase* lhs_;
+    int rere_every_;
+    std::vector<typename TPrecon::Instance> precon_;  // zero or one
+  };
+  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const { return Instance(lhs, *this); }
+
+ private:
+  int rere_every_;
+  const TPrecon* precon_;
+};
+}  // namespace detail
+
+template <typename TPrecon = GpuMultiGridSolver<double>>
+using GpuConjugateGradients = detail::KrylovSolver<&lgpu_cg_solve, TPrecon>;
+template <typename TPrecon = GpuMultiGridSolver<double>>
+using GpuBiCgStab = detail::KrylovSolver<&lgpu_bicgstab_solve, TPrecon>;
+
+// run_iterative_solver (linear-solver/runner.hpp:17-84) with the relative
+// residual stop rule of the reference's test callbacks
+// (test/aux/iterative.hpp:25-65).
+template <typename TSolver>
+inline SolveReport run_iterative_solver(const TSolver& solver, const GpuMatrixBase& lhs,
+                                        GpuDenseVector<double>& sol,
+                                        const GpuDenseVector<double>& rhs, StopRule stop) {
+  auto inst = solver.instantiate(lhs);
+  auto sys = inst.instantiate(sol, rhs);
+  return sys.solve(stop);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the solve (not in the reference): outlet flux J and dJ/dlog c
+
+struct AdjointReport {
+  int status = LGPU_OK;
+  double flux = 0.0;
+  int forward_iterations = 0, adjoint_iterations = 0;
+  [[nodiscard]] bool converged() const { return status == LGPU_OK; }
+};
+
+template <typename TPreconInstance>
+inline AdjointReport outlet_flux_gradient(const GpuStencilValueMatrix& lhs,
+                                          const TPreconInstance* precon,
+                                          GpuDenseVector<double>& p, GpuDenseVector<double>& lambda,
+                                          GpuDenseVector<double>& grad, StopRule stop,
+                                          int rere_every = 8) {
+  lgpu_krylov_params prm{stop.rel_tol, stop.max_it, rere_every};
+  AdjointReport r;
+  int its[2] = {0, 0};
+  const int st = lgpu_adjoint_grad(lhs.handle(), precon ? precon->handle() : nullptr, &prm,
+                                   p.handle(), lambda.handle(), grad.handle(), &r.flux, its);
+  if (st != LGPU_OK && st != LGPU_NOT_CONVERGED && st != LGPU_BREAKDOWN && st != LGPU_NONFINITE)
+    check(st);
+  r.status = st;
+  r.forward_iterations = its[0];
+  r.adjoint_iterations = its[1];
+  return r;
+}
+
+}  // namespace lineal::gpu

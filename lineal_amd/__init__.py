@@ -39,6 +39,8 @@ LIB_PATH = os.environ.get("LINEAL_AMD_LIBRARY") or os.path.join(_HERE, "liblinea
 
 F64, F32 = 0, 1
 OK, INVALID_ARG, OUT_OF_MEMORY, HIP_ERROR, BREAKDOWN, NONFINITE, NOT_CONVERGED, NOT_SUPPORTED, SYNC_TIMEOUT = range(9)
+# kernel classes of Context.kernel_time
+KT_STENCIL_SOR_FWD, KT_STENCIL_SOR_BWD, KT_STENCIL_PASS, KT_CSR_SOR, KT_CSR_PASS = range(5)
 
 _vp = C.c_void_p
 _i64 = C.c_int64
@@ -127,6 +129,8 @@ SIGNATURES = {
     "lgpu_amg_vcycle": (C.c_int, [_vp, _vp, _vp]),
     "lgpu_cg_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
     "lgpu_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
+    "lgpu_ctx_kernel_timing": (C.c_int, [_vp, C.c_int]),
+    "lgpu_ctx_kernel_time": (C.c_int, [_vp, C.c_int, _f64p, _i64p]),
     "lgpu_outlet_flux": (C.c_int, [_vp, _vp, _f64p]),
     "lgpu_outlet_flux_adjoint_rhs": (C.c_int, [_vp, _vp]),
     "lgpu_outlet_flux_gradient": (C.c_int, [_vp, _vp, _vp, _vp]),
@@ -195,6 +199,16 @@ class Context:
         ms = C.c_double()
         _check(lib().lgpu_timer_stop(self.h, C.byref(ms)))
         return ms.value
+
+    def kernel_timing(self, enable: bool):
+        """Bracket every launch of each kernel class with HIP events (clears records)."""
+        _check(lib().lgpu_ctx_kernel_timing(self.h, int(bool(enable))))
+
+    def kernel_time(self, kernel_class: int):
+        """(total milliseconds, launches) of a kernel class since timing was enabled."""
+        ms, n = C.c_double(), C.c_int64()
+        _check(lib().lgpu_ctx_kernel_time(self.h, int(kernel_class), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def close(self):
         if getattr(self, "h", None) and not _finalizing:

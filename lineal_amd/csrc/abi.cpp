@@ -57,6 +57,11 @@ struct lgpu_ctx_s {
   int64_t partial_cap = 0;
   DevBuf scal;            // SC_COUNT device scalars
   double* host_scal = nullptr;  // pinned mirror
+  // opt-in timing of kernel classes (lgpu_ctx_kernel_timing): start/stop
+  // event pairs recorded around every launch of the class
+  bool ktime_on = false;
+  std::vector<hipEvent_t> kev[LGPU_KT_COUNT];
+  size_t kused[LGPU_KT_COUNT] = {};
   double* partials_for(int64_t need) {
     need *= 2;  // dual reductions
     if (need > partial_cap) {
@@ -66,6 +71,37 @@ struct lgpu_ctx_s {
     return partials.as<double>();
   }
 };
+
+namespace {
+
+// Brackets one launch of a kernel class with HIP events on the context's
+// stream while kernel timing is enabled.
+struct KTimed {
+  lgpu_ctx ctx;
+  hipEvent_t stop = nullptr;
+  KTimed(lgpu_ctx c, int cls) : ctx(c) {
+    if (!ctx->ktime_on) return;
+    auto& v = ctx->kev[cls];
+    size_t& u = ctx->kused[cls];
+    if (u + 2 > v.size()) {
+      hipEvent_t a = nullptr, b = nullptr;
+      LGPU_HIP_CHECK(hipEventCreate(&a));
+      LGPU_HIP_CHECK(hipEventCreate(&b));
+      v.push_back(a);
+      v.push_back(b);
+    }
+    LGPU_HIP_CHECK(hipEventRecord(v[u], ctx->stream));
+    stop = v[u + 1];
+    u += 2;
+  }
+  ~KTimed() {
+    if (stop) (void)hipEventRecord(stop, ctx->stream);
+  }
+  KTimed(const KTimed&) = delete;
+  KTimed& operator=(const KTimed&) = delete;
+};
+
+}  // namespace
 
 struct lgpu_vec_s {
   lgpu_ctx ctx = nullptr;
@@ -283,9 +319,11 @@ void op_apply(lgpu_op op, int mode, bool f32, const void* x, const void* b, void
   hipStream_t st = op->ctx->stream;
   if (op->is_csr) {
     LGPU_REQUIRE(op->csr.f32 == f32, "CSR value type and vector type differ");
+    KTimed kt(op->ctx, LGPU_KT_CSR_PASS);
     csr_apply(op->csr.desc(), mode, x, b, y, omega, partials, st);
   } else {
     if (f32) op->ensure_f32();
+    KTimed kt(op->ctx, LGPU_KT_STENCIL_PASS);
     stencil_apply(op->desc, mode, f32, x, b, y, omega, partials, st);
   }
 }
@@ -309,6 +347,7 @@ void op_sor(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, 
   hipStream_t st = op->ctx->stream;
   if (op->is_csr) {
     LGPU_REQUIRE(op->csr.f32 == f32, "CSR value type and vector type differ");
+    KTimed kt(op->ctx, LGPU_KT_CSR_SOR);
     csr_sweep(op->csr, x_in, x_out, b, omega, fwd, ultra, st);
   } else {
     if (f32) op->ensure_f32();
@@ -317,6 +356,7 @@ void op_sor(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, 
     s.order = op->sor_order.as<uint32_t>();
     s.ncols = op->sor_cols;
     s.mailbox = op->sor_mailbox.ptr;
+    KTimed kt(op->ctx, fwd ? LGPU_KT_STENCIL_SOR_FWD : LGPU_KT_STENCIL_SOR_BWD);
     stencil_sor(op->desc, f32, x_in, x_out, b, omega, fwd, ultra, s, st);
   }
 }
@@ -362,6 +402,7 @@ void dot_into(lgpu_ctx ctx, const void* a, const void* b, bool f32, int64_t n, i
 void coarse_sweep(lgpu_amg amg, CsrDev& a, const void* x_in, void* x_out, const void* b,
                   double omega, bool fwd, bool ultra) {
   const int64_t seg = amg->params.coarse_segment_rows;
+  KTimed kt(amg->ctx, LGPU_KT_CSR_SOR);
   if (seg > 0) csr_sor_segmented(a.desc(), x_in, x_out, b, omega, fwd, ultra, int(seg), amg->ctx->stream);
   else csr_sweep(a, x_in, x_out, b, omega, fwd, ultra, amg->ctx->stream);
 }
@@ -386,6 +427,7 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine) {
       op_apply(fine, SM_RESID, f32, xs(0), bs(0), cur.tmp.ptr, 0.0, nullptr);
     } else {
       coarse_sweep(amg, cur.a, nullptr, xs(l), bs(l), relax, true, ultra);
+      KTimed kt(amg->ctx, LGPU_KT_CSR_PASS);
       csr_apply(cur.a.desc(), SM_RESID, xs(l), bs(l), cur.tmp.ptr, 0.0, nullptr, st);
     }
     const int64_t nc = amg->lv[size_t(l + 1)].n;
@@ -469,6 +511,8 @@ extern "C" int lgpu_ctx_destroy(lgpu_ctx ctx) {
   if (!ctx) return LGPU_OK;
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);
+  for (auto& v : ctx->kev)
+    for (hipEvent_t e : v) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipStreamDestroy(ctx->stream);
@@ -498,6 +542,34 @@ extern "C" int lgpu_timer_stop(lgpu_ctx ctx, double* ms) {
   float f = 0;
   LGPU_HIP_CHECK(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
   *ms = f;
+  API_END
+}
+
+extern "C" int lgpu_ctx_kernel_timing(lgpu_ctx ctx, int enable) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx, "ctx is NULL");
+  LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  for (size_t& u : ctx->kused) u = 0;
+  ctx->ktime_on = enable != 0;
+  API_END
+}
+
+extern "C" int lgpu_ctx_kernel_time(lgpu_ctx ctx, int kernel_class, double* total_ms,
+                                    int64_t* launches) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && total_ms && launches, "NULL argument");
+  LGPU_REQUIRE(kernel_class >= 0 && kernel_class < LGPU_KT_COUNT, "unknown kernel class");
+  LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  const auto& v = ctx->kev[kernel_class];
+  const size_t u = ctx->kused[kernel_class];
+  double t = 0;
+  for (size_t k = 0; k < u; k += 2) {
+    float f = 0;
+    LGPU_HIP_CHECK(hipEventElapsedTime(&f, v[k], v[k + 1]));
+    t += f;
+  }
+  *total_ms = t;
+  *launches = int64_t(u / 2);
   API_END
 }
 

@@ -8,7 +8,8 @@ Kernels (all on a lognormal sigma=1 values stencil of n^3 cells):
   spmv_f64 / spmv_f32      y = A x
   resid_f32                r = b - A x
   jacobi_f64               one Jacobi sweep (omega = 2/3)
-  sor_f64 / sor_f32_zero   one forward lexicographic SOR sweep (with x_in / from zero)
+  sor_f64 / sor_f32 / sor_f32_zero   one forward lexicographic SOR sweep (with x_in / from zero)
+  sor_f64_bwd / sor_f32_bwd          one backward sweep with x_in (the V-cycle's post-smoother)
   csr_spmv_f64 / csr_spmv_f32   y = A x with the assembled CSR copy
   lookup_spmv_f64          y = A x with the u8 lookup (two-phase spheres) operator
   copy_f64 / copy_f32      a plain vector copy of n^3 entries (counter calibration)
@@ -62,7 +63,8 @@ def main(argv=None):
             dt = np.float32 if "f32" in name else np.float64
             x, y, b = vs(dt, 10), vs(dt, 20), vs(dt, 30)
             zero = name.endswith("zero")
-            fn = lambda k: A.sor(None if zero else x[k % rot], y[k % rot], b[k % rot], 1.0, True, False)
+            fwd = "bwd" not in name
+            fn = lambda k: A.sor(None if zero else x[k % rot], y[k % rot], b[k % rot], 1.0, fwd, False)
         elif name.startswith("csr_spmv"):
             dt = np.float32 if name.endswith("f32") else np.float64
             if dt not in csr:

@@ -1,0 +1,129 @@
+// The reference's stencil test scenarios (test/shared-stencil.hpp:120-239)
+// written against the header-only GPU classes: a 4 x 9 x 7 lookup stencil
+// with the diagonal-lines materials (ids (y + z) % 4, coefficients 0..3,
+// test/aux/materials.hpp:217-233).
+//   known solution 2: PCG-AMG and BiCGStab-AMG reach residual <= 1e-13 and
+//                     |x - 2|_inf <= 1e-13 within 32 iterations
+//   Dirichlet rhs:    PCG-AMG residual |A x - b|_inf <= 2e-14
+//   CSR copy:         CSR SpMV equals the matrix-free SpMV
+//   adjoint:          outlet flux of a constant-coefficient values stencil
+//                     equals n_y n_z c dp / n_x and dJ/dlog c_k = J_col^2 / c_k
+// Prints one line per check; the exit status is the number of failed checks.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "lineal-amd/lineal-amd.hpp"
+
+namespace lg = lineal::gpu;
+
+static int failures = 0;
+
+static void expect(bool ok, const char* what, double value) {
+  std::printf("%s %s (%.3e)\n", ok ? "ok  " : "FAIL", what, value);
+  if (!ok) ++failures;
+}
+
+static double inf_norm_diff(const std::vector<double>& a, double v) {
+  double m = 0;
+  for (double x : a) m = std::fmax(m, std::fabs(x - v));
+  return m;
+}
+
+int main() {
+  try {
+    lg::GpuContext ctx(0);
+    const std::array<int64_t, 3> dims{4, 9, 7};
+    lg::BoxInfo box(dims);
+    std::vector<std::uint8_t> mat(size_t(box.cell_num()));
+    for (int64_t x = 0; x < dims[0]; ++x)
+      for (int64_t y = 0; y < dims[1]; ++y)
+        for (int64_t z = 0; z < dims[2]; ++z)
+          mat[size_t((x * dims[1] + y) * dims[2] + z)] = std::uint8_t((y + z) % 4);
+    lg::GpuStencilLookupMatrix A(box, mat, {0.0, 1.0, 2.0, 3.0}, ctx);
+    const size_t n = A.row_num();
+    expect(A.is_symmetric() && n == 252, "lookup stencil rows", double(n));
+
+    // known solution 2 (test/shared-stencil.hpp:154-190)
+    lg::GpuDenseVector<double> two(n, 2.0, ctx);
+    const auto b = A * two;
+    lg::GpuMultiGridSolver<double> amg;
+    for (int which = 0; which < 2; ++which) {
+      lg::GpuDenseVector<double> x(n, 0.0, ctx);
+      lg::SolveReport r;
+      if (which == 0) {
+        lg::GpuConjugateGradients<> cg(8, &amg);
+        r = lg::run_iterative_solver(cg, A, x, b, {1e-15, 32});
+      } else {
+        lg::GpuBiCgStab<> bicg(8, &amg);
+        r = lg::run_iterative_solver(bicg, A, x, b, {1e-15, 32});
+      }
+      const double err = inf_norm_diff(x.to_host(), 2.0);
+      expect(r.residual_norm() <= 1e-13 && err <= 1e-13,
+             which == 0 ? "pcg_amg known solution 2" : "bicgstab_amg known solution 2", err);
+    }
+
+    // Dirichlet rhs (test/shared-stencil.hpp:192-239)
+    {
+      const auto rhs = A.rhs();
+      lg::GpuDenseVector<double> x(n, 0.0, ctx);
+      lg::GpuConjugateGradients<> cg(8, &amg);
+      const auto r = lg::run_iterative_solver(cg, A, x, rhs, {1e-15, 64});
+      lg::GpuDenseVector<double> res(n, ctx);
+      A.residual(rhs, x, res);
+      const double rinf = inf_norm_diff(res.to_host(), 0.0);
+      expect(rinf <= 2e-14, "pcg_amg dirichlet residual", rinf);
+      (void)r;
+    }
+
+    // CSR copy of the stencil
+    {
+      lg::GpuCsrMatrix<double> C(A);
+      std::vector<double> xv(n);
+      for (size_t i = 0; i < n; ++i) xv[i] = std::sin(double(i));
+      lg::GpuDenseVector<double> x(std::span<const double>(xv), ctx);
+      const auto y1 = (A * x).to_host();
+      const auto y2 = (C * x).to_host();
+      double d = 0;
+      for (size_t i = 0; i < n; ++i) d = std::fmax(d, std::fabs(y1[i] - y2[i]));
+      expect(d <= 1e-14, "csr spmv equals matrix-free spmv", d);
+    }
+
+    // stationary solvers: one forward SOR sweep on A x = A 2 from zero
+    {
+      lg::GpuSorSolver<double> sor(1.0);
+      auto inst = sor.instantiate(A);
+      lg::GpuDenseVector<double> x(n, ctx);
+      inst.apply(nullptr, x, b);
+      lg::GpuJacobiSolver<double> jac(2.0 / 3.0);
+      lg::GpuDenseVector<double> y(n, ctx);
+      jac.instantiate(A).apply(&two, y, b);
+      const double e = inf_norm_diff(y.to_host(), 2.0);
+      expect(e <= 1e-14, "jacobi keeps the exact solution", e);
+    }
+
+    // adjoint on a constant-coefficient values stencil: J = n_y n_z c dp / n_x
+    {
+      const std::array<int64_t, 3> d2{16, 6, 5};
+      lg::BoxInfo b2(d2);
+      std::vector<double> c(size_t(b2.cell_num()), 1.5);
+      lg::GpuStencilValueMatrix V(b2, c, ctx);
+      lg::GpuMultiGridSolver<double> m2;
+      auto mi = m2.instantiate(V);
+      const size_t n2 = V.row_num();
+      lg::GpuDenseVector<double> p(n2, 0.0, ctx), lam(n2, 0.0, ctx), g(n2, ctx);
+      const auto r = lg::outlet_flux_gradient(V, &mi, p, lam, g, {1e-13, 200});
+      const double j = double(d2[1] * d2[2]) * 1.5 / double(d2[0]);
+      expect(r.converged() && std::fabs(r.flux - j) <= 1e-11 * j, "outlet flux closed form",
+             std::fabs(r.flux - j));
+      // column closed form: dJ/dlog c_k = J_col^2 / c_k with J_col = c / n_x
+      const double jc = 1.5 / double(d2[0]);
+      const double e = inf_norm_diff(g.to_host(), jc * jc / 1.5);
+      expect(e <= 1e-11, "outlet flux gradient closed form", e);
+    }
+  } catch (const lg::GpuError& e) {
+    std::printf("FAIL exception: %s\n", e.what());
+    return 100;
+  }
+  return failures;
+}

@@ -41,3 +41,18 @@ def test_invalid_arguments_are_reported_not_raised():
     p = la._AmgParams()
     L.lgpu_amg_default_params(ctypes.byref(p))
     assert (p.max_level_num, p.min_coarsen_size, p.min_aggregate_size, p.max_aggregate_size) == (16, 32, 6, 9)
+
+
+def test_cpp_header_compiles_against_the_c_abi(tmp_path):
+    """The header-only C++ classes build with g++ alone (host code)."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    assert gxx, "g++ is part of the toolchain"
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "lineal-amd/lineal-amd.hpp"\n'
+                   "int main() { lineal::gpu::HierarchyWrightParams p; return int(p.max_level_num) - 16; }\n")
+    res = subprocess.run([gxx, "-std=c++20", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                          f"-I{os.path.join(ROOT, 'include')}", str(src)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
