@@ -847,8 +847,8 @@ static void apply_precon(const or_hierarchy* h, int precon, int ultra, const dou
   if (precon == 1) {
     or_vcycle_f64(h, r, z, ultra);
   } else if (precon == 2) {
-    float* r32 = (float*)malloc(sizeof(float) * (size_t)n);
-    float* z32 = (float*)malloc(sizeof(float) * (size_t)n);
+    float* r32 = (float*)calloc((size_t)n, sizeof(float));
+    float* z32 = (float*)calloc((size_t)n, sizeof(float));
     for (int64_t i = 0; i < n; ++i) r32[i] = (float)r[i];
     or_vcycle_f32(h, r32, z32, ultra);
     for (int64_t i = 0; i < n; ++i) z[i] = (double)z32[i];

@@ -206,3 +206,20 @@ def test_cg_residual_history_full_size(dims):
     _, it, hist = O.pcg(S, b, None, precon=0, rel_tol=1e-30, max_it=10)
     assert res.iterations == it == 10
     np.testing.assert_allclose(res.residuals, hist, rtol=1e-9)
+
+
+@pytest.mark.timeout(600)
+def test_hierarchy_and_vcycle_full_size_256():
+    """The 256^3 lognormal hierarchy (aggregate maps) and one FP64 V-cycle
+    equal the oracle's bit for bit at the benchmark's size."""
+    dims = (256, 256, 256)
+    A, S = pair(dims, "values")
+    M = la.MultiGridSolver(precision=np.float64).instantiate(A)
+    H = O.Hierarchy(S, coarse_segment=64)
+    assert [M.level_size(l) for l in range(M.level_num)] == [H.size(l) for l in range(H.level_num)]
+    for l in range(H.level_num - 1):
+        np.testing.assert_array_equal(M.aggregates(l), H.agg_map(l))
+    r = problems.uniform(S.N, 4)
+    z = la.DenseVector(S.N)
+    M.apply(la.DenseVector(r), z)
+    np.testing.assert_array_equal(z.numpy(), H.vcycle(r))
