@@ -33,9 +33,12 @@ namespace lgpu {
 namespace {
 
 constexpr int CB = 8;        // lines per column side
-// steps per chunk (one window load each): 16 for f32, 8 for f64 (register budget)
+// steps per chunk (one window load and one mailbox prefetch each): shorter
+// chunks let a column follow its upstream columns more closely, longer ones
+// amortise the loads; 8 measured best at 256^3 for f32 (4, 16 and 32 slower)
+// and for f64 (4 slower)
 template <typename T>
-constexpr int kChunk = sizeof(T) == 4 ? 16 : 8;
+constexpr int kChunk = 8;
 constexpr int SPIN_LIMIT = 1 << 24;
 
 // Mailbox words start as a NaN with a payload that no arithmetic produces
@@ -66,6 +69,7 @@ template <typename T>
 __device__ __forceinline__ T mailbox_poll(const T* p, int* state) {
   T v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   int it = 0;
+#pragma clang loop unroll(disable)
   while (is_sentinel(v)) {
     if (__hip_atomic_load(state + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
     __builtin_amdgcn_s_sleep(1);
@@ -93,7 +97,50 @@ __device__ __forceinline__ R shfl_raw(R v, int src) {
   else return __shfl(v, src, 64);
 }
 
-template <typename T, int KIND, bool FWD, bool ULTRA, bool HAS_X>
+// Buffer resources over the planes a column touches: base in scalar
+// registers, 32-bit per-lane byte offsets.  Offsets are clamped into the range
+// (values of cells outside a line are never used by a valid step); masked-off
+// lanes use kOff, which reads 0 and drops a store, so loads and stores need no
+// branches.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t kOff = 0xFFFFFFF0u;
+template <typename E>
+__device__ __forceinline__ Rsrc col_rsrc(const E* base, int64_t elem0, int64_t elems) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<E*>(base + elem0), short(0),
+                                           int(uint32_t(elems * int64_t(sizeof(E)))), 0x00020000);
+}
+template <typename E>
+__device__ __forceinline__ E rload(Rsrc r, uint32_t off) {
+  if constexpr (sizeof(E) == 1) return E(__builtin_amdgcn_raw_buffer_load_b8(r, off, 0u, 0));
+  else if constexpr (sizeof(E) == 4) return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0u, 0));
+  else return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
+}
+template <typename T>
+__device__ __forceinline__ void rstore(T v, Rsrc r, uint32_t off) {
+  if constexpr (sizeof(T) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0u, 0);
+  else {
+    using V2 = unsigned int __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0u, 0);
+  }
+}
+
+// The per-cell array a valuator reads: coefficients or material ids.
+template <typename T, int KIND>
+__device__ __forceinline__ const typename CellInfo<T, KIND>::Raw* cell_ptr(const StencilDesc& s) {
+  if constexpr (KIND == 0) return coeff_ptr<T>(s);
+  else return s.mat;
+}
+
+// Connection value of two cells: harmonic mean of per-cell coefficients
+// (branch-free with FD, whose range the host checked) or the material table.
+template <typename T, int KIND, bool FD>
+__device__ __forceinline__ T conn_of(const StencilDesc& s, typename CellInfo<T, KIND>::Raw a,
+                                     typename CellInfo<T, KIND>::Raw b, T qd, int d) {
+  if constexpr (KIND == 0) return hmean_sel<T, FD>(a, b) * qd;
+  else return CellInfo<T, KIND>::conn(s, a, b, qd, d);
+}
+
+template <typename T, int KIND, bool FWD, bool ULTRA, bool HAS_X, bool FD>
 __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __restrict__ xin,
                                                     T* __restrict__ xout,
                                                     const T* __restrict__ bv, T omega,
@@ -102,6 +149,7 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   using CI = CellInfo<T, KIND>;
   using Raw = typename CI::Raw;
   constexpr int K = kChunk<T>;
+  constexpr uint32_t SR = sizeof(Raw), ST = sizeof(T);
   const int lane = threadIdx.x;
   const int a = lane / CB, b = lane % CB;
   __shared__ int s_ticket;
@@ -112,105 +160,131 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   const int64_t nx = s.nx, ny = s.ny, nz = s.nz, nyz = ny * nz;
   const int64_t lx = I * CB + a, ly = J * CB + b;  // logical line coordinates
   const bool line = lx < nx && ly < ny;
-  // physical coordinates and the physical offsets of the logical +x / +y / +z steps
+  // physical x of the line and the presence of the logical neighbour lines
   const int64_t x = FWD ? lx : nx - 1 - lx;
-  const int64_t y = FWD ? ly : ny - 1 - ly;
-  const int64_t sx = FWD ? nyz : -nyz, sy = FWD ? nz : -nz, sz = FWD ? 1 : -1;
-  const bool has_sx = lx + 1 < nx, has_px = lx > 0;  // logical successor / predecessor lines
+  const bool has_sx = lx + 1 < nx, has_px = lx > 0;
   const bool has_sy = ly + 1 < ny, has_py = ly > 0;
-  // start of the line, clamped into the box for lanes outside it
-  const int64_t cx_ = FWD ? min(lx, nx - 1) : nx - 1 - min(lx, nx - 1);
-  const int64_t cy_ = FWD ? min(ly, ny - 1) : ny - 1 - min(ly, ny - 1);
-  const int64_t lbase = (cx_ * ny + cy_) * nz;
+  // the physical planes of logical x in [I*CB - 1, I*CB + CB]
+  const int64_t l0 = I * CB > 0 ? I * CB - 1 : 0;
+  const int64_t l1 = I * CB + CB < nx - 1 ? I * CB + CB : nx - 1;
+  const int64_t p0 = FWD ? l0 : nx - 1 - l1, p1 = FWD ? l1 : nx - 1 - l0;
+  const int64_t relems = (p1 - p0 + 1) * nyz;
+  const Rsrc rc = col_rsrc(cell_ptr<T, KIND>(s), p0 * nyz, relems);
+  const Rsrc rxi = col_rsrc(xin, p0 * nyz, HAS_X ? relems : 0);
+  const Rsrc rbv = col_rsrc(bv, p0 * nyz, relems);
+  const Rsrc rxo = col_rsrc(static_cast<const T*>(xout), p0 * nyz, relems);
+  // element offset (in the resources) of logical z = 0 on this line (lines
+  // outside the box are clamped into it), and the element steps of logical
+  // +z / +x / +y
+  const int64_t cxl = lx < nx ? lx : nx - 1, cyl = ly < ny ? ly : ny - 1;
+  const int64_t pxl = FWD ? cxl : nx - 1 - cxl, pyl = FWD ? cyl : ny - 1 - cyl;
+  const int32_t dz = FWD ? 1 : -1;
+  const int32_t zorg = int32_t((pxl - p0) * nyz + pyl * nz + (FWD ? 0 : nz - 1));
+  const int32_t hsx = int32_t(FWD ? nyz : -nyz), hsy = int32_t(FWD ? nz : -nz);
+  // halo lines: the +x line for a = 7, the -x line for a = 0 (+y / -y for b)
+  const bool hx_on = a == CB - 1 || a == 0, hy_on = b == CB - 1 || b == 0;
+  const int32_t hxo = a == CB - 1 ? hsx : -hsx, hyo = b == CB - 1 ? hsy : -hsy;
+  // byte offset of element e, clamped into the resource: the clamp also keeps
+  // the compiler from splitting a constant out of the offset into the
+  // instruction's immediate, which the hardware would add without wrapping
+  const int32_t emax = int32_t(relems - 1);
+  auto boff = [emax](int32_t e, uint32_t sz) {
+    return uint32_t(e < 0 ? 0 : (e > emax ? emax : e)) * sz;
+  };
   const int64_t nsteps = nz + 2 * (CB - 1);
   const T m1w = T(1) - omega;
   const T qx = T(s.q[0]), qy = T(s.q[1]), qz = T(s.q[2]);
   const T two_q0 = T(2) * qx;
   const T adv = T(s.adv);  // upwind advection: -x coupling and diagonal
-  // mailboxes: [X part: column, b, z] the a = 7 lines; [Y part: column, a, z] the b = 7 lines
+  // mailboxes: [X part: column, b, z] the a = 7 lines; [Y part: column, a, z]
+  // the b = 7 lines
   const int64_t ncols = ((nx + CB - 1) / CB) * cy;
   T* const ybox = mbox + ncols * CB * nz;
   T* const out_x = mbox + (col * CB + b) * nz;
   T* const out_y = ybox + (col * CB + a) * nz;
   const T* const in_x = mbox + (((I > 0 ? I - 1 : 0) * cy + J) * CB + b) * nz;
   const T* const in_y = ybox + ((I * cy + (J > 0 ? J - 1 : 0)) * CB + a) * nz;
-  // physical cell index of logical z on this line (clamped into the line)
-  auto cell = [&](int64_t lz) {
-    const int64_t c = lz < 0 ? 0 : (lz >= nz ? nz - 1 : lz);
-    return lbase + (FWD ? c : nz - 1 - c);
-  };
+  const bool take_x = line && a == 0 && has_px;
+  const bool take_y = line && b == 0 && has_py;
+  const bool give_x = a == CB - 1, give_y = b == CB - 1;
   T prev = T(0), wpx_prev = T(0), wpy_prev = T(0), wpz_prev = T(0);
   for (int64_t t0 = 0; t0 < nsteps; t0 += K) {
-    const int64_t z0 = t0 - a - b;
-    // ---- chunk windows (independent of the recurrence; one round trip) ----
-    Raw cw[K + 2], ecx[K + 2], ecy[K + 2], ucx[K + 2], ucy[K + 2];
+    const int32_t z0 = int32_t(t0) - a - b;
+    // ---- chunk windows: unconditional loads, one round trip ----
+    Raw cw[K + 2], hx[K + 2], hy[K + 2];
     T xw[K + 1], exx[K + 1], exy[K + 1], bw[K];
 #pragma unroll
     for (int j = 0; j < K + 2; ++j) {
-      const int64_t i = cell(z0 - 1 + j);
-      cw[j] = CI::load(s, i);
-      ecx[j] = (a == CB - 1 && has_sx) ? CI::load(s, i + sx) : Raw{};
-      ecy[j] = (b == CB - 1 && has_sy) ? CI::load(s, i + sy) : Raw{};
-      ucx[j] = (a == 0 && has_px) ? CI::load(s, i - sx) : Raw{};
-      ucy[j] = (b == 0 && has_py) ? CI::load(s, i - sy) : Raw{};
+      const int32_t e = zorg + dz * (z0 - 1 + j);
+      cw[j] = rload<Raw>(rc, boff(e, SR));
+      hx[j] = rload<Raw>(rc, hx_on ? boff(e + hxo, SR) : kOff);
+      hy[j] = rload<Raw>(rc, hy_on ? boff(e + hyo, SR) : kOff);
     }
 #pragma unroll
     for (int j = 0; j < K + 1; ++j) {
-      const int64_t i = cell(z0 + j);
-      xw[j] = HAS_X ? xin[i] : T(0);
-      exx[j] = (HAS_X && a == CB - 1 && has_sx) ? xin[i + sx] : T(0);
-      exy[j] = (HAS_X && b == CB - 1 && has_sy) ? xin[i + sy] : T(0);
+      const int32_t e = zorg + dz * (z0 + j);
+      xw[j] = HAS_X ? rload<T>(rxi, boff(e, ST)) : T(0);
+      exx[j] = HAS_X ? rload<T>(rxi, give_x ? boff(e + hsx, ST) : kOff) : T(0);
+      exy[j] = HAS_X ? rload<T>(rxi, give_y ? boff(e + hsy, ST) : kOff) : T(0);
     }
 #pragma unroll
-    for (int j = 0; j < K; ++j) bw[j] = bv[cell(z0 + j)];
+    for (int j = 0; j < K; ++j) bw[j] = rload<T>(rbv, boff(zorg + dz * (z0 + j), ST));
     // ---- new border values of the upstream columns: prefetched from their
-    // mailboxes now, re-polled at the step that needs them if still unset ----
+    // mailboxes now (every lane reads a word of its upstream line, clamped
+    // into it, and lanes that take none discard it), polled again at the step
+    // that needs them if still unset ----
     T nwx[K], nwy[K];
-    const bool take_x = line && a == 0 && has_px;
-    const bool take_y = line && b == 0 && has_py;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      const int64_t lz = z0 + j;
+      const int32_t lz = z0 + j;
       const bool ok = lz >= 0 && lz < nz;
-      nwx[j] = (take_x && ok) ? __hip_atomic_load(in_x + lz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : T(0);
-      nwy[j] = (take_y && ok) ? __hip_atomic_load(in_y + lz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : T(0);
+      const int32_t cz = lz < 0 ? 0 : (lz >= nz ? int32_t(nz - 1) : lz);
+      const T vx = __hip_atomic_load(in_x + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const T vy = __hip_atomic_load(in_y + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      nwx[j] = take_x && ok ? vx : T(0);
+      nwy[j] = take_y && ok ? vy : T(0);
     }
     // ---- K steps from registers ----
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int64_t lz = z0 + k;
+      const int32_t lz = z0 + k;
       const bool valid = line && lz >= 0 && lz < nz && t0 + k < nsteps;
-      if (valid && take_x && is_sentinel(nwx[k])) nwx[k] = mailbox_poll(in_x + lz, state);
-      if (valid && take_y && is_sentinel(nwy[k])) nwy[k] = mailbox_poll(in_y + lz, state);
+      if (valid && ((take_x && is_sentinel(nwx[k])) || (take_y && is_sentinel(nwy[k])))) {
+        if (take_x) nwx[k] = mailbox_poll(in_x + lz, state);
+        if (take_y) nwy[k] = mailbox_poll(in_y + lz, state);
+      }
       // successor data of the neighbouring lanes (same logical z)
       Raw csx = shfl_raw(cw[k + 2], lane + CB < 64 ? lane + CB : lane);
       Raw csy = shfl_raw(cw[k + 2], lane + 1 < 64 ? lane + 1 : lane);
       T xsx = __shfl(xw[k + 1], lane + CB < 64 ? lane + CB : lane, 64);
       T xsy = __shfl(xw[k + 1], lane + 1 < 64 ? lane + 1 : lane, 64);
       // predecessor results of the neighbouring lanes (one step ago)
-      T npx = __shfl(prev, lane >= CB ? lane - CB : lane, 64);
-      T npy = __shfl(prev, lane >= 1 ? lane - 1 : lane, 64);
-      T wpx_in = __shfl(wpx_prev, lane >= CB ? lane - CB : lane, 64);
-      T wpy_in = __shfl(wpy_prev, lane >= 1 ? lane - 1 : lane, 64);
-      if (a == CB - 1) {
-        csx = ecx[k + 1];
+      const T npx = __shfl(prev, lane >= CB ? lane - CB : lane, 64);
+      const T npy = __shfl(prev, lane >= 1 ? lane - 1 : lane, 64);
+      const T wpx_in = __shfl(wpx_prev, lane >= CB ? lane - CB : lane, 64);
+      const T wpy_in = __shfl(wpy_prev, lane >= 1 ? lane - 1 : lane, 64);
+      if (give_x) {
+        csx = hx[k + 1];
         xsx = exx[k];
       }
-      if (b == CB - 1) {
-        csy = ecy[k + 1];
+      if (give_y) {
+        csy = hy[k + 1];
         xsy = exy[k];
       }
       const Raw cc = cw[k + 1];
       // face weights: successor faces computed here, predecessor faces shared
+      // (computed unconditionally, absent faces selected to 0)
       const bool has_sz = lz + 1 < nz, has_pz = lz > 0;
-      const T w_sx = has_sx ? CI::conn(s, cc, csx, qx, 0) : T(0);
-      const T w_sy = has_sy ? CI::conn(s, cc, csy, qy, 1) : T(0);
-      const T w_sz = has_sz ? CI::conn(s, cc, cw[k + 2], qz, 2) : T(0);
-      T w_px, w_py;
-      if (a == 0) w_px = has_px ? CI::conn(s, cc, ucx[k + 1], qx, 0) : T(0);
-      else w_px = wpx_in;
-      if (b == 0) w_py = has_py ? CI::conn(s, cc, ucy[k + 1], qy, 1) : T(0);
-      else w_py = wpy_in;
+      const T c_sx = conn_of<T, KIND, FD>(s, cc, csx, qx, 0);
+      const T c_sy = conn_of<T, KIND, FD>(s, cc, csy, qy, 1);
+      const T c_sz = conn_of<T, KIND, FD>(s, cc, cw[k + 2], qz, 2);
+      const T c_px = conn_of<T, KIND, FD>(s, cc, hx[k + 1], qx, 0);
+      const T c_py = conn_of<T, KIND, FD>(s, cc, hy[k + 1], qy, 1);
+      const T w_sx = has_sx ? c_sx : T(0);
+      const T w_sy = has_sy ? c_sy : T(0);
+      const T w_sz = has_sz ? c_sz : T(0);
+      const T w_px = a == 0 ? (has_px ? c_px : T(0)) : wpx_in;
+      const T w_py = b == 0 ? (has_py ? c_py : T(0)) : wpy_in;
       const T w_pz = has_pz ? wpz_prev : T(0);
       // predecessor (new) and successor (old) values
       const T n_px = a == 0 ? nwx[k] : npx;
@@ -256,37 +330,45 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
         const T quot = acc / d;
         xn = fma(omega, quot, m1w * xi);
       }
-      if (valid) {
-        xout[cell(lz)] = xn;
-        // border lines publish their values to the downstream columns
-        if (a == CB - 1) __hip_atomic_store(out_x + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (b == CB - 1) __hip_atomic_store(out_y + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        prev = xn;
-        wpx_prev = w_sx;
-        wpy_prev = w_sy;
-        wpz_prev = w_sz;
-      }
+      rstore<T>(xn, rxo, valid ? boff(zorg + dz * lz, ST) : kOff);
+      // border lines publish their values to the downstream columns
+      if (valid && give_x) __hip_atomic_store(out_x + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (valid && give_y) __hip_atomic_store(out_y + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // a lane's values of a step outside its line are read only by steps
+      // outside the box
+      prev = xn;
+      wpx_prev = w_sx;
+      wpy_prev = w_sy;
+      wpz_prev = w_sz;
     }
   }
 }
 
-template <typename T, int KIND, bool FWD, bool ULTRA>
-void launch4(const StencilDesc& s, const void* x_in, void* x_out, const void* b, double omega,
+template <typename T, int KIND, bool FWD, bool ULTRA, bool FD>
+void launch5(const StencilDesc& s, const void* x_in, void* x_out, const void* b, double omega,
              const SorSched& sched, hipStream_t st) {
   const int64_t cy = (s.ny + CB - 1) / CB;
   const dim3 grid(unsigned(sched.ncols)), block(64);
   if (x_in) {
-    hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, true>), grid, block, 0, st, s,
+    hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, true, FD>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
                        static_cast<const T*>(b), T(omega), sched.state, sched.order, cy,
                        static_cast<T*>(sched.mailbox));
   } else {
-    hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, false>), grid, block, 0, st, s,
+    hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, false, FD>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
                        static_cast<const T*>(b), T(omega), sched.state, sched.order, cy,
                        static_cast<T*>(sched.mailbox));
   }
   LGPU_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, int KIND, bool FWD, bool ULTRA>
+void launch4(const StencilDesc& s, const void* x_in, void* x_out, const void* b, double omega,
+             const SorSched& sched, hipStream_t st) {
+  const bool fd = KIND == 0 && (sizeof(T) == 8 ? s.fast_div64 : s.fast_div32);
+  if (fd) launch5<T, KIND, FWD, ULTRA, true>(s, x_in, x_out, b, omega, sched, st);
+  else launch5<T, KIND, FWD, ULTRA, false>(s, x_in, x_out, b, omega, sched, st);
 }
 
 template <typename T, int KIND>
@@ -327,6 +409,9 @@ void sor_sched_order(const StencilDesc& s, std::vector<uint32_t>& order) {
 void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, const void* b,
                  double omega, bool forward, bool ultra, const SorSched& sched, hipStream_t st) {
   LGPU_REQUIRE(x_in != x_out, "SOR sweep needs distinct input and output vectors");
+  // a column's ten planes are addressed with 32-bit byte offsets below 2^31
+  LGPU_REQUIRE(10 * s.ny * s.nz * int64_t(f32 ? 4 : 8) < (int64_t(1) << 31),
+               "SOR sweep: the y-z plane is too large for 32-bit column offsets");
   // reset the ticket and the progress counters; state[1] (the error word) is
   // sticky until the host clears it
   vec_fill_int(sched.state, 1, 0, st);
