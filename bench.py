@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the finest-level f32 SOR sweeps at 256^3: 2 x
 # FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc runs (the factor 2 is what a plain
 # copy kernel of known size gives), profiles/stencil_kernels_256_hbm_counters.md
-TRAFFIC_256 = {"stencil_sor_fwd": 688_138_957, "stencil_sor_bwd": 849_919_078}
+TRAFFIC_256 = {"stencil_sor_fwd": 722_203_853, "stencil_sor_bwd": 952_195_482}
 
 
 def log(msg):
@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--workload", default="lognormal", choices=["lognormal", "two-phase", "mixed512"],
+                    help="lognormal: the headline AMG-CG metric (default); two-phase and mixed512: "
+                         "the other solve configurations, one JSON line each")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this path has no multi-GPU implementation "
@@ -96,6 +99,11 @@ def main():
         from lineal_amd import build as hip_build
 
         hip_build.build()
+
+    if args.workload == "two-phase":
+        return two_phase(args)
+    if args.workload == "mixed512":
+        return mixed512(args)
 
     n = args.n
     dims = (n, n, n)
@@ -249,6 +257,126 @@ def main():
         log("cpu baseline (oracle)")
         result["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.sigma, n, its64[-1])
     print(json.dumps(result), flush=True)
+
+
+def timed_solves(ctx, solve, steps, warmup):
+    for _ in range(warmup):
+        solve()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    res = None
+    for _ in range(steps):
+        res = solve()
+        assert res.converged, res
+    ctx.synchronize()
+    return (time.perf_counter() - t0) / steps, res
+
+
+def two_phase(args):
+    """256^3 two-phase lookup field (u8 ids from seeded random spheres of radius
+    4-12 until 35% solid; coefficients 1.0 pore / 1e-6 solid): FP64 AMG-CG to
+    1e-8, and BiCGStab with first-order upwind advection v = (10 / L_x, 0, 0)
+    (Peclet number v L_x / D_ref = 10 with D_ref = 1) preconditioned by the
+    FP64 AMG of the diffusion operator."""
+    import lineal_amd as la
+    from lineal_amd import problems
+
+    n = args.n
+    dims = (n, n, n)
+    N = n ** 3
+    ctx = la.default_context()
+    log(f"generating {n}^3 spheres (35% solid)")
+    m = problems.spheres(dims, 3)
+    mc = (1.0, 1e-6, 0.0, 0.0)
+    A = la.StencilMatrix.lookup(la.BoxInfo(dims), m, mc, ctx)
+    v = 10.0 / n
+    Aadv = la.StencilMatrix.lookup(la.BoxInfo(dims, velocity_x=v), m, mc, ctx)
+    M = la.MultiGridSolver(precision=np.float64).instantiate(A)
+    log(f"AMG setup {M.setup_seconds:.2f} s, levels {[M.level_size(l) for l in range(M.level_num)]}")
+    b, badv = A.rhs(), Aadv.rhs()
+    x = la.DenseVector(N, ctx=ctx)
+
+    def cg():
+        x.fill(0.0)
+        return la.ConjugateGradientsSolver(8, M).solve(A, x, b, rel_tol=1e-8, max_it=1000)
+
+    def bicg():
+        x.fill(0.0)
+        return la.BiCgStabSolver(8, M).solve(Aadv, x, badv, rel_tol=1e-8, max_it=1000)
+
+    steps, warmup = max(1, min(args.steps, 3)), min(args.warmup, 1)
+    t_cg, r_cg = timed_solves(ctx, cg, steps, warmup)
+    log(f"AMG-CG: {t_cg * 1e3:.1f} ms, {r_cg.iterations} iterations")
+    t_bi, r_bi = timed_solves(ctx, bicg, steps, warmup)
+    r = la.DenseVector(N, ctx=ctx)
+    Aadv.residual(badv, x, r)
+    true_rel = float(np.sqrt(r.dot(r) / badv.dot(badv)))
+    log(f"BiCGStab (Pe 10): {t_bi * 1e3:.1f} ms, {r_bi.iterations} iterations, true rel. residual {true_rel:.2e}")
+    print(json.dumps({
+        "metric": "two-phase 256^3 solve time (s): FP64 AMG-CG to 1e-8; BiCGStab with upwind advection Pe 10",
+        "value": t_cg, "unit": "s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": t_cg * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n}^3 two-phase lookup stencil (spheres r 4-12, 35% solid, 1.0 / 1e-6), "
+                               "FP64 AMG-PCG rel. tol 1e-8; BiCGStab + upwind advection v = 10/L_x",
+                   "cg_seconds": t_cg, "cg_iterations": r_cg.iterations,
+                   "bicgstab_seconds": t_bi, "bicgstab_iterations": r_bi.iterations,
+                   "bicgstab_true_rel_residual": true_rel, "setup_seconds": M.setup_seconds,
+                   "solid_fraction": float(m.mean())},
+    }), flush=True)
+
+
+def mixed512(args):
+    """512^3 lognormal sigma = 2: FP32-hierarchy AMG inside FP64 CG against the
+    plain FP64 AMG-CG, both to 1e-8; iterations, times, the mixed solution's
+    relative difference to the FP64 one and the FP64 solution's true relative
+    residual."""
+    import lineal_amd as la
+    from lineal_amd import problems
+
+    n = 512 if args.n == 256 else args.n
+    sigma = 2.0 if args.sigma == 1.0 else args.sigma
+    dims = (n, n, n)
+    N = n ** 3
+    ctx = la.default_context()
+    log(f"generating {n}^3 lognormal field sigma={sigma}")
+    A = la.StencilMatrix.values(la.BoxInfo(dims), problems.lognormal(dims, sigma, 1), ctx)
+    b = A.rhs()
+    M32 = la.MultiGridSolver(precision=np.float32).instantiate(A)
+    log(f"FP32 AMG setup {M32.setup_seconds:.1f} s, levels {[M32.level_size(l) for l in range(M32.level_num)]}")
+    M64 = la.MultiGridSolver(precision=np.float64).instantiate(A)
+    log(f"FP64 AMG setup {M64.setup_seconds:.1f} s")
+    x = la.DenseVector(N, ctx=ctx)
+
+    def make(M):
+        def solve():
+            x.fill(0.0)
+            return la.ConjugateGradientsSolver(8, M).solve(A, x, b, rel_tol=1e-8, max_it=2000)
+        return solve
+
+    steps, warmup = max(1, min(args.steps, 2)), 0
+    t32, r32 = timed_solves(ctx, make(M32), steps, warmup)
+    x32 = x.numpy()
+    log(f"mixed: {t32 * 1e3:.1f} ms, {r32.iterations} iterations")
+    t64, r64 = timed_solves(ctx, make(M64), steps, warmup)
+    x64 = x.numpy()
+    r = la.DenseVector(N, ctx=ctx)
+    A.residual(b, x, r)
+    true_rel = float(np.sqrt(r.dot(r) / b.dot(b)))
+    rel = float(np.abs(x32 - x64).max() / np.abs(x64).max())
+    log(f"fp64: {t64 * 1e3:.1f} ms, {r64.iterations} iterations; mixed vs fp64 {rel:.2e}; "
+        f"fp64 true rel. residual {true_rel:.2e}")
+    print(json.dumps({
+        "metric": "AMG-CG time-to-1e-8 at 512^3 lognormal sigma=2, mixed vs FP64 (s)",
+        "value": t32, "unit": "s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": t32 * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32/f64", "data": "synthetic",
+        "config": {"workload": f"{n}^3 lognormal sigma={sigma} values stencil, AMG-PCG rel. tol 1e-8",
+                   "mixed_seconds": t32, "fp64_seconds": t64, "mixed_over_fp64": t32 / t64,
+                   "iterations_mixed": r32.iterations, "iterations_fp64": r64.iterations,
+                   "mixed_vs_fp64_rel_error": rel, "fp64_true_rel_residual": true_rel,
+                   "setup_seconds_mixed": M32.setup_seconds, "setup_seconds_fp64": M64.setup_seconds},
+    }), flush=True)
 
 
 if __name__ == "__main__":
