@@ -189,7 +189,8 @@ struct lgpu_amg_s {
   lgpu_amg_params params{};
   setup::Hierarchy host;
   std::vector<AmgLevel> lv;
-  DevBuf chol;
+  DevBuf chol;   // coarsest lower factor (Cholesky, or the unit lower LU factor)
+  DevBuf upper;  // coarsest upper LU factor (nonsymmetric hierarchies)
   double setup_seconds = 0.0;
   bool f32 = false;
   DevBuf r32, z32;  // finest f32 copies for the mixed-precision cycle
@@ -436,7 +437,10 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine) {
   }
   {
     AmgLevel& c = amg->lv[size_t(L - 1)];
-    dense_cholesky_solve(amg->chol.as<double>(), c.n, c.rhs.ptr, c.sol.ptr, f32, st);
+    if (amg->host.lu)
+      dense_lu_solve(amg->chol.as<double>(), amg->upper.as<double>(), c.n, c.rhs.ptr, c.sol.ptr, f32, st);
+    else
+      dense_cholesky_solve(amg->chol.as<double>(), c.n, c.rhs.ptr, c.sol.ptr, f32, st);
   }
   for (int l = L - 2; l >= 0; --l) {
     AmgLevel& cur = amg->lv[size_t(l)];
@@ -947,7 +951,8 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
     stencil_host_rows(finest, rows);
     amg->host = setup::build(rows, sp);
   }
-  LGPU_REQUIRE(amg->host.spd, "coarsest matrix is not positive definite");
+  LGPU_REQUIRE(amg->host.spd, amg->host.lu ? "the coarsest LU factorisation met a zero pivot"
+                                          : "coarsest matrix is not positive definite");
   const setup::Hierarchy& h = amg->host;
   const int L = int(h.sizes.size());
   LGPU_REQUIRE(h.sizes.back() <= 4096, "coarsest level has more than 4096 rows");
@@ -976,6 +981,10 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
   const int64_t nc = h.sizes.back();
   amg->chol.alloc(size_t(nc * nc) * sizeof(double));
   LGPU_HIP_CHECK(hipMemcpy(amg->chol.ptr, h.chol_dense.data(), size_t(nc * nc) * sizeof(double), hipMemcpyHostToDevice));
+  if (h.lu) {
+    amg->upper.alloc(size_t(nc * nc) * sizeof(double));
+    LGPU_HIP_CHECK(hipMemcpy(amg->upper.ptr, h.upper_dense.data(), size_t(nc * nc) * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (amg->f32) {
     finest->ensure_f32();
     amg->r32.alloc(size_t(h.sizes[0]) * sizeof(float));

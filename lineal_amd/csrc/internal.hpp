@@ -171,6 +171,9 @@ void prolong_add(const void* x, const void* xc, bool f32, const uint32_t* agg, i
 // dense lower-triangular Cholesky solve, n <= 4096, f64 factor; rhs/x in f64 or f32
 void dense_cholesky_solve(const double* L, int64_t n, const void* b, void* x, bool f32,
                           hipStream_t st);
+// dense LU solve with unit lower L and upper U (f64 factors), n <= 4096
+void dense_lu_solve(const double* L, const double* U, int64_t n, const void* b, void* x, bool f32,
+                    hipStream_t st);
 
 // Krylov helper kernels operating on device scalars
 // scal layout indices

@@ -184,7 +184,42 @@ __global__ __launch_bounds__(NT) void chol_solve_kernel(const double* L, int64_t
   for (int64_t i = threadIdx.x; i < n; i += NT) x[i] = T(s[i]);
 }
 
+// Dense LU solve with a unit lower factor L and an upper factor U (f64, row
+// major), column by column in one block: forward, x_j is final once reached
+// and is subtracted from the rows below; backward, x_j = s_j / u_jj is
+// subtracted from the rows above (the oracle's order).
+template <typename T>
+__global__ __launch_bounds__(NT) void lu_solve_kernel(const double* L, const double* U, int64_t n,
+                                                      const T* b, T* x) {
+  extern __shared__ double sh[];
+  double* s = sh;  // n entries
+  for (int64_t i = threadIdx.x; i < n; i += NT) s[i] = double(b[i]);
+  __syncthreads();
+  for (int64_t j = 0; j < n; ++j) {
+    const double xj = s[j];
+    for (int64_t i = j + 1 + threadIdx.x; i < n; i += NT) s[i] = s[i] - L[i * n + j] * xj;
+    __syncthreads();
+  }
+  for (int64_t j = n - 1; j >= 0; --j) {
+    const double xj = s[j] / U[j * n + j];
+    __syncthreads();
+    if (threadIdx.x == 0) s[j] = xj;
+    for (int64_t i = threadIdx.x; i < j; i += NT) s[i] = s[i] - U[i * n + j] * xj;
+    __syncthreads();
+  }
+  for (int64_t i = threadIdx.x; i < n; i += NT) x[i] = T(s[i]);
+}
+
 }  // namespace
+
+void dense_lu_solve(const double* L, const double* U, int64_t n, const void* b, void* x, bool f32,
+                    hipStream_t st) {
+  LGPU_REQUIRE(n <= 4096, "coarsest level too large for the dense direct solve");
+  const size_t sh = size_t(n) * sizeof(double);
+  if (f32) hipLaunchKernelGGL(lu_solve_kernel<float>, dim3(1), dim3(NT), sh, st, L, U, n, static_cast<const float*>(b), static_cast<float*>(x));
+  else hipLaunchKernelGGL(lu_solve_kernel<double>, dim3(1), dim3(NT), sh, st, L, U, n, static_cast<const double*>(b), static_cast<double*>(x));
+  LGPU_HIP_CHECK(hipGetLastError());
+}
 
 void vec_fill(void* x, int64_t n, bool f32, double v, hipStream_t st) {
   if (f32) hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(NT), 0, st, static_cast<float*>(x), n, float(v));

@@ -419,6 +419,52 @@ bool cholesky(const RowMatrix& a, RowMatrix& lower) {
   return true;
 }
 
+// Row i starts from its own entries; each finished row k < i that holds a
+// (possibly filled-in) entry eliminates it in ascending k, l_ik = w_k / u_kk,
+// updating the entries of its upper structure (w_j -= l_ik u_kj, or
+// w_j = -(l_ik u_kj) as fill-in); then u_ii = w_i, u_ij = w_j (j > i), l_ii = 1.
+bool lu_dense(const RowMatrix& a, std::vector<double>& lower, std::vector<double>& upper) {
+  const int64_t n = a.n;
+  const size_t nn = size_t(n * n);
+  lower.assign(nn, 0.0);
+  upper.assign(nn, 0.0);
+  std::vector<double> w(static_cast<size_t>(n));
+  std::vector<char> present(static_cast<size_t>(n)), ustruct(nn, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    std::fill(w.begin(), w.end(), 0.0);
+    std::fill(present.begin(), present.end(), 0);
+    for (int64_t p = a.rowptr[size_t(i)]; p < a.rowptr[size_t(i + 1)]; ++p) {
+      w[size_t(a.cols[size_t(p)])] = a.vals[size_t(p)];
+      present[size_t(a.cols[size_t(p)])] = 1;
+    }
+    for (int64_t k = 0; k < i; ++k) {
+      if (!present[size_t(k)]) continue;
+      const double l = w[size_t(k)] / upper[size_t(k * n + k)];
+      lower[size_t(i * n + k)] = l;
+      for (int64_t j = k + 1; j < n; ++j) {
+        if (!ustruct[size_t(k * n + j)]) continue;
+        const double sub = l * upper[size_t(k * n + j)];
+        if (present[size_t(j)]) {
+          w[size_t(j)] -= sub;
+        } else {
+          w[size_t(j)] = -sub;
+          present[size_t(j)] = 1;
+        }
+      }
+    }
+    if (!present[size_t(i)] || w[size_t(i)] == 0.0) return false;
+    lower[size_t(i * n + i)] = 1.0;
+    upper[size_t(i * n + i)] = w[size_t(i)];
+    ustruct[size_t(i * n + i)] = 1;
+    for (int64_t j = i + 1; j < n; ++j) {
+      if (!present[size_t(j)]) continue;
+      upper[size_t(i * n + j)] = w[size_t(j)];
+      ustruct[size_t(i * n + j)] = 1;
+    }
+  }
+  return true;
+}
+
 // HierarchyWrightInstance::create_impl (multigrid/hierarchy/wright.hpp:120-143, 145-309)
 Hierarchy build(const RowMatrix& finest, const Params& p) {
   Hierarchy h;
@@ -446,6 +492,13 @@ Hierarchy build(const RowMatrix& finest, const Params& p) {
     }
     if (stop) break;
     fine = &h.coarse.back();
+  }
+  if (!h.coarse.back().symmetric) {
+    // nonsymmetric hierarchies (BiCgStab) take an LU coarsest solver
+    // (test/aux/iterative.hpp:122-181)
+    h.lu = true;
+    h.spd = lu_dense(h.coarse.back(), h.chol_dense, h.upper_dense);
+    return h;
   }
   RowMatrix lower;
   h.spd = cholesky(h.coarse.back(), lower);

@@ -47,8 +47,11 @@ struct Hierarchy {
   std::vector<int64_t> sizes;     // level sizes, finest first
   std::vector<Level> transfer;    // transfer[l] maps level l to level l+1
   std::vector<RowMatrix> coarse;  // coarse[l] is level l+1's matrix (column order)
-  std::vector<double> chol_dense; // coarsest lower Cholesky factor, dense row major
-  bool spd = true;
+  std::vector<double> chol_dense; // coarsest lower factor, dense row major: Cholesky
+                                  // (symmetric) or the unit lower LU factor
+  std::vector<double> upper_dense;  // coarsest upper LU factor (nonsymmetric), dense row major
+  bool lu = false;                  // nonsymmetric coarsest matrix: LU instead of Cholesky
+  bool spd = true;                  // the coarsest factorisation succeeded
 };
 
 constexpr uint32_t kUnaggregated = 0xFFFFFFFFu;
@@ -59,5 +62,9 @@ Hierarchy build(const RowMatrix& finest, const Params& p);
 // Sparse Cholesky-Banachiewicz of a CSR matrix (column order rows); returns
 // false if a pivot is not positive.  Output rows in column order, diagonal last.
 bool cholesky(const RowMatrix& a, RowMatrix& lower);
+
+// Row-wise LU without pivoting (linear-solver/direct/lu/decomposition.hpp:30-111)
+// into dense row-major unit-lower and upper factors; false on a zero pivot.
+bool lu_dense(const RowMatrix& a, std::vector<double>& lower, std::vector<double>& upper);
 
 }  // namespace lgpu::setup

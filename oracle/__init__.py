@@ -127,6 +127,9 @@ def lib():
             "or_vcycle_f32": (None, [hp, f32p, f32p, C.c_int]),
             "or_pcg": (C.c_int, [mp, hp, C.c_int, C.c_int, f64p, f64p, C.c_double, C.c_int, f64p]),
             "or_pbicgstab": (C.c_int, [mp, hp, C.c_int, C.c_int, f64p, f64p, C.c_double, C.c_int, f64p]),
+            "or_amg_is_lu": (C.c_int, [hp]),
+            "or_lu_decompose_dense": (C.c_int, [cp, f64p, f64p]),
+            "or_lu_solve_dense": (None, [C.c_int64, f64p, f64p, f64p, f64p]),
             "or_outlet_flux": (C.c_double, [sp, f64p]),
             "or_outlet_flux_adjoint_rhs": (None, [sp, f64p]),
             "or_outlet_flux_gradient": (None, [sp, f64p, f64p, f64p]),
@@ -305,6 +308,8 @@ class Hierarchy:
         self.A = A  # keep the finest operator alive
         self.params = params or default_params(coarse_f32)
         self.h = lib().or_amg_setup(C.byref(A.mat), C.byref(self.params))
+        if not self.h:
+            raise ValueError("the coarsest factorisation failed")
         lib().or_amg_set_coarse_segment(self.h, int(coarse_segment))
 
     def __del__(self):
@@ -341,6 +346,11 @@ class Hierarchy:
         lib().or_amg_stats(self.h, level, _p(out, C.c_int64))
         return out
 
+    @property
+    def is_lu(self):
+        """True if the coarsest level is solved by LU (nonsymmetric hierarchy)."""
+        return bool(lib().or_amg_is_lu(self.h))
+
     def vcycle(self, r, ultra=False):
         z = np.zeros_like(r)
         _fn("or_vcycle", r.dtype)(self.h, _pf(r), _pf(z), int(ultra))
@@ -350,6 +360,23 @@ class Hierarchy:
 def cholesky_solve(L: Csr, b):
     x = np.zeros_like(b)
     lib().or_cholesky_solve(C.byref(L.c), _pf(b), _pf(x))
+    return x
+
+
+def lu_dense(A: Csr):
+    """Dense row-wise LU without pivoting: (unit lower L, upper U)."""
+    n = A.N
+    lo = np.zeros((n, n))
+    up = np.zeros((n, n))
+    if lib().or_lu_decompose_dense(C.byref(A.c), _pf(lo), _pf(up)) != 0:
+        raise ValueError("zero pivot")
+    return lo, up
+
+
+def lu_solve_dense(lo, up, b):
+    x = np.zeros_like(b)
+    lib().or_lu_solve_dense(lo.shape[0], _pf(np.ascontiguousarray(lo)), _pf(np.ascontiguousarray(up)),
+                            _pf(np.ascontiguousarray(b, dtype=np.float64)), _pf(x))
     return x
 
 

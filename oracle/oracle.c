@@ -44,6 +44,9 @@ struct or_hierarchy {
   or_csr mats[32];       /* mats[l] for l >= 1 */
   int64_t stats[32][5];
   or_csr chol;
+  int lu;             /* nonsymmetric coarsest matrix: dense LU factors instead of Cholesky */
+  double* lu_l;       /* unit lower factor, dense row major */
+  double* lu_u;       /* upper factor, dense row major */
   int64_t coarse_seg; /* rows per smoother segment on levels >= 1 (0: one segment) */
 };
 
@@ -734,6 +737,81 @@ void or_cholesky_solve(const or_csr* l, const double* b, double* x) {
   free(aux);
 }
 
+/* Row-wise LU without pivoting, linear-solver/direct/lu/decomposition.hpp:30-111,
+ * restated with dense storage and structure flags: row i starts from its own
+ * entries; every finished row k < i holding a (possibly filled-in) entry w_k
+ * eliminates it in ascending k, l_ik = w_k / u_kk, and updates the entries
+ * of its upper structure (w_j -= l_ik u_kj, or w_j = -(l_ik u_kj) as fill-in);
+ * then u_ii = w_i and u_ij = w_j for the remaining j > i, l_ii = 1.
+ * Returns 0, or -1 on a zero or missing pivot. */
+int or_lu_decompose_dense(const or_csr* a, double* lo, double* up) {
+  const int64_t n = a->n;
+  double* w = (double*)malloc(sizeof(double) * (size_t)n);
+  char* present = (char*)malloc((size_t)n);
+  char* ustruct = (char*)calloc((size_t)(n * n), 1);
+  int status = 0;
+  for (int64_t t = 0; t < n * n; ++t) lo[t] = up[t] = 0.0;
+  for (int64_t i = 0; i < n && status == 0; ++i) {
+    for (int64_t j = 0; j < n; ++j) {
+      w[j] = 0.0;
+      present[j] = 0;
+    }
+    for (int64_t k = a->rowptr[i]; k < a->rowptr[i + 1]; ++k) {
+      w[a->cols[k]] = a->vals[k];
+      present[a->cols[k]] = 1;
+    }
+    for (int64_t k = 0; k < i; ++k) {
+      if (!present[k]) continue;
+      const double l = w[k] / up[k * n + k];
+      lo[i * n + k] = l;
+      for (int64_t j = k + 1; j < n; ++j) {
+        if (!ustruct[k * n + j]) continue;
+        const double sub = l * up[k * n + j];
+        if (present[j]) {
+          w[j] -= sub;
+        } else {
+          w[j] = -sub;
+          present[j] = 1;
+        }
+      }
+    }
+    if (!present[i] || w[i] == 0.0) {
+      status = -1;
+      break;
+    }
+    lo[i * n + i] = 1.0;
+    up[i * n + i] = w[i];
+    ustruct[i * n + i] = 1;
+    for (int64_t j = i + 1; j < n; ++j) {
+      if (!present[j]) continue;
+      up[i * n + j] = w[j];
+      ustruct[i * n + j] = 1;
+    }
+  }
+  free(w);
+  free(present);
+  free(ustruct);
+  return status;
+}
+
+/* forward substitution with the unit lower factor, then backward
+ * substitution with the upper factor (linear-solver/direct/substitution.hpp:27-80),
+ * both column by column: after x_j is final, it is subtracted from the rows
+ * below (forward) or above (backward) it */
+void or_lu_solve_dense(int64_t n, const double* lo, const double* up, const double* b,
+                       double* x) {
+  for (int64_t i = 0; i < n; ++i) x[i] = b[i];
+  for (int64_t j = 0; j < n; ++j) {
+    const double xj = x[j];
+    for (int64_t i = j + 1; i < n; ++i) x[i] = x[i] - lo[i * n + j] * xj;
+  }
+  for (int64_t j = n - 1; j >= 0; --j) {
+    const double xj = x[j] / up[j * n + j];
+    x[j] = xj;
+    for (int64_t i = 0; i < j; ++i) x[i] = x[i] - up[i * n + j] * xj;
+  }
+}
+
 /* hierarchy/wright.hpp:120-143, 145-309 */
 or_hierarchy* or_amg_setup(const or_mat* finest, const or_amg_params* p) {
   or_hierarchy* h = (or_hierarchy*)calloc(1, sizeof(or_hierarchy));
@@ -774,6 +852,18 @@ or_hierarchy* or_amg_setup(const or_mat* finest, const or_amg_params* p) {
   }
   const or_csr* c = &h->mats[h->nlev - 1];
   const int64_t n = c->n;
+  if (!c->symmetric) {
+    /* BiCgStab's nonsymmetric hierarchies take an LU coarsest solver
+     * (test/aux/iterative.hpp:122-181) */
+    h->lu = 1;
+    h->lu_l = (double*)malloc(sizeof(double) * (size_t)(n * n));
+    h->lu_u = (double*)malloc(sizeof(double) * (size_t)(n * n));
+    if (or_lu_decompose_dense(c, h->lu_l, h->lu_u) != 0) {
+      or_amg_free(h);
+      return NULL;
+    }
+    return h;
+  }
   const int64_t cap = n * (n + 1) / 2 + 1;
   int64_t* rp = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
   int64_t* cc = (int64_t*)malloc(sizeof(int64_t) * (size_t)cap);
@@ -799,6 +889,8 @@ void or_amg_free(or_hierarchy* h) {
   free((void*)h->chol.rowptr);
   free((void*)h->chol.cols);
   free((void*)h->chol.vals);
+  free(h->lu_l);
+  free(h->lu_u);
   free(h);
 }
 
@@ -809,6 +901,7 @@ const int64_t* or_amg_c2f_ptr(const or_hierarchy* h, int level) { return h->c2f_
 const int64_t* or_amg_c2f_idx(const or_hierarchy* h, int level) { return h->c2f_idx[level]; }
 const or_csr* or_amg_matrix(const or_hierarchy* h, int level) { return &h->mats[level]; }
 const or_csr* or_amg_cholesky(const or_hierarchy* h) { return &h->chol; }
+int or_amg_is_lu(const or_hierarchy* h) { return h->lu; }
 void or_amg_stats(const or_hierarchy* h, int level, int64_t out[5]) {
   for (int k = 0; k < 5; ++k) out[k] = h->stats[level][k];
 }

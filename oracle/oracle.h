@@ -121,6 +121,13 @@ typedef struct {
 typedef struct or_hierarchy or_hierarchy;
 
 or_hierarchy* or_amg_setup(const or_mat* finest, const or_amg_params* p);
+/* 1 if the coarsest level is solved by LU (nonsymmetric hierarchy) */
+int or_amg_is_lu(const or_hierarchy* h);
+/* dense row-wise LU without pivoting of a CSR matrix (unit lower factor) and
+ * the matching forward/backward substitution */
+int or_lu_decompose_dense(const or_csr* a, double* lo, double* up);
+void or_lu_solve_dense(int64_t n, const double* lo, const double* up, const double* b,
+                       double* x);
 void or_amg_free(or_hierarchy* h);
 int or_amg_level_num(const or_hierarchy* h);               /* including the finest */
 int64_t or_amg_level_size(const or_hierarchy* h, int level);

@@ -223,3 +223,52 @@ def test_hierarchy_and_vcycle_full_size_256():
     z = la.DenseVector(S.N)
     M.apply(la.DenseVector(r), z)
     np.testing.assert_array_equal(z.numpy(), H.vcycle(r))
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_advection_hierarchy_and_vcycle_equal_oracle(f32):
+    """A hierarchy built from the nonsymmetric upwind-advection operator
+    (two-phase field, Pe = 10): aggregates, coarse matrices and one V-cycle
+    with the LU coarsest solve equal the oracle's bit for bit."""
+    dims = (24, 20, 16)
+    m = problems.spheres(dims, 3, rmin=2, rmax=4)
+    mc = (1.0, 1e-6, 0.0, 0.0)
+    v = 10.0 / dims[0]
+    A = la.StencilMatrix.lookup(la.BoxInfo(dims, velocity_x=v), m, mc)
+    S = O.Stencil(dims, materials=m, mat_coeffs=mc, velocity=v)
+    H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=64)
+    assert H.is_lu
+    M = la.MultiGridSolver(precision=np.float32 if f32 else np.float64).instantiate(A)
+    assert M.level_num == H.level_num
+    for l in range(H.level_num - 1):
+        np.testing.assert_array_equal(M.aggregates(l), H.agg_map(l))
+    for l in range(1, H.level_num):
+        rp, cols, vals = M.matrix(l)
+        ref = H.matrix(l)
+        np.testing.assert_array_equal(cols, ref.cols)
+        np.testing.assert_array_equal(vals, ref.vals)
+    r = problems.uniform(S.N, 6)
+    if f32:
+        r = r.astype(np.float32)
+        z = M.apply(la.DenseVector(r), la.DenseVector(S.N, np.float32)).numpy()
+    else:
+        z = M.apply(la.DenseVector(r), la.DenseVector(S.N)).numpy()
+    np.testing.assert_array_equal(z, H.vcycle(r))
+
+
+def test_bicgstab_nonsymmetric_amg_matches_oracle():
+    dims = (32, 24, 20)
+    m = problems.spheres(dims, 3, rmin=2, rmax=4)
+    mc = (1.0, 1e-6, 0.0, 0.0)
+    v = 10.0 / dims[0]
+    A = la.StencilMatrix.lookup(la.BoxInfo(dims, velocity_x=v), m, mc)
+    S = O.Stencil(dims, materials=m, mat_coeffs=mc, velocity=v)
+    H = O.Hierarchy(S, coarse_segment=64)
+    M = la.MultiGridSolver().instantiate(A)
+    b = S.rhs()
+    x_ref, it_ref, _ = O.pbicgstab(S, b, H, precon=1, rel_tol=1e-10, max_it=300)
+    x = la.DenseVector(S.N, fill=0.0)
+    res = la.BiCgStabSolver(8, M).solve(A, x, la.DenseVector(b), rel_tol=1e-10, max_it=300)
+    assert res.converged and abs(res.iterations - it_ref) <= 1
+    assert np.abs(x.numpy() - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+    assert np.linalg.norm(O.residual(S, b, x.numpy())) <= 1e-9 * np.linalg.norm(b)

@@ -648,3 +648,57 @@ def test_adjoint_outlet_flux_driver_with_amg():
     assert np.abs(g - g_ref).max() <= 1e-9 * np.abs(g_ref).max()
     assert J == pytest.approx(O.outlet_flux(S, p_ref), rel=1e-11)
     assert 0 < itf < 100 and 0 < ita < 100
+
+
+# ---------------------------------------------------------------------------
+# LU coarsest solver of nonsymmetric hierarchies
+
+
+def random_nonsymmetric_csr(n, density, seed):
+    rng = np.random.default_rng(seed)
+    A = np.where(rng.random((n, n)) < density, rng.uniform(-1, 1, (n, n)), 0.0)
+    np.fill_diagonal(A, np.abs(A).sum(axis=1) + 1.0)
+    rows, cols = np.nonzero(A)
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    return O.Csr(np.cumsum(rp), cols, A[rows, cols], symmetric=False), A
+
+
+@pytest.mark.parametrize("n,density", [(1, 1.0), (7, 0.4), (40, 0.15)])
+def test_lu_dense_factors_and_solve(n, density):
+    """Unit lower L and upper U with L U = A (no pivoting), and the solve
+    against numpy's LAPACK solve."""
+    C_, A = random_nonsymmetric_csr(n, density, n)
+    L, U = O.lu_dense(C_)
+    assert np.allclose(np.diag(L), 1.0) and np.all(np.triu(L, 1) == 0) and np.all(np.tril(U, -1) == 0)
+    np.testing.assert_allclose(L @ U, A, rtol=0, atol=1e-13 * np.abs(A).max())
+    b = np.random.default_rng(1).standard_normal(n)
+    np.testing.assert_allclose(O.lu_solve_dense(L, U, b), np.linalg.solve(A, b), rtol=1e-11, atol=1e-12)
+
+
+def test_lu_fill_in_follows_the_structure():
+    """Entries outside the structural fill of the row-wise elimination stay
+    exactly zero: an arrow matrix pointing down-right has no fill-in."""
+    n = 9
+    A = np.eye(n) * 4.0
+    A[:, -1] = -1.0
+    A[-1, :] = -0.5
+    A[-1, -1] = 4.0
+    rows, cols = np.nonzero(A)
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    L, U = O.lu_dense(O.Csr(np.cumsum(rp), cols, A[rows, cols], symmetric=False))
+    assert np.count_nonzero(np.tril(L, -1)) == n - 1 and np.count_nonzero(np.triu(U, 1)) == n - 1
+    np.testing.assert_allclose(L @ U, A, atol=1e-15)
+
+
+def test_advection_hierarchy_uses_lu_and_bicgstab_converges():
+    dims = (12, 10, 8)
+    m = problems.spheres(dims, 3, rmin=2, rmax=3)
+    S = O.Stencil(dims, materials=m, mat_coeffs=(1.0, 1e-6, 0.0, 0.0), velocity=10.0 / dims[0])
+    H = O.Hierarchy(S, coarse_segment=64)
+    assert H.is_lu and not O.Hierarchy(O.Stencil(dims, materials=m, mat_coeffs=(1.0, 1e-6, 0.0, 0.0))).is_lu
+    b = S.rhs()
+    x, it, _ = O.pbicgstab(S, b, H, precon=1, rel_tol=1e-10, max_it=200)
+    assert it < 60
+    assert np.linalg.norm(O.residual(S, b, x)) <= 1e-9 * np.linalg.norm(b)
