@@ -376,7 +376,8 @@ struct HierarchyWrightParams {
 };
 
 // MultiGridSolver{relax = 1, DefaultCycle{REGULAR, 1}, wright} with SOR(1)
-// pre/post smoothers and a Cholesky coarsest solver.  TLevelReal selects the
+// pre/post smoothers and a Cholesky coarsest solver (LU for a nonsymmetric
+// operator).  TLevelReal selects the
 // hierarchy's precision: float gives the mixed-precision preconditioner
 // (FP32 hierarchy, smoothers and level operators; FP64 coarsest solve).
 template <typename TLevelReal = double>

@@ -171,7 +171,11 @@ void lgpu_amg_default_params(lgpu_amg_params* p);
 /* MultiGridSolver::instantiate -> HierarchyWrightInstance::create
  * (multigrid/solver.hpp:188-196, hierarchy/wright.hpp:145-309): the hierarchy
  * (strength criterion, aggregation with one thread segment, Galerkin products,
- * coarsest Cholesky factor) is built on the host and uploaded once. */
+ * coarsest factor) is built on the host and uploaded once.  The coarsest level
+ * is solved by Cholesky (direct/cholesky/decomposition.hpp:44-99) for a
+ * symmetric operator and by row-wise LU without pivoting
+ * (direct/lu/decomposition.hpp:30-111) for a nonsymmetric one (upwind
+ * advection, the BiCgStab preconditioner). */
 int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out);
 int lgpu_amg_destroy(lgpu_amg amg);
 int lgpu_amg_level_num(lgpu_amg amg, int* levels);
