@@ -250,6 +250,18 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
       const int32_t lz = z0 + k;
       const bool valid = line && lz >= 0 && lz < nz && t0 + k < nsteps;
       if (valid && ((take_x && is_sentinel(nwx[k])) || (take_y && is_sentinel(nwy[k])))) {
+        // one round trip refreshes this step's word and every later word of
+        // the chunk still unset, so the following steps find theirs present
+#pragma unroll
+        for (int j = k + 1; j < K; ++j) {
+          const int32_t lzj = z0 + j;
+          const bool okj = lzj >= 0 && lzj < nz;
+          const int32_t czj = lzj < 0 ? 0 : (lzj >= nz ? int32_t(nz - 1) : lzj);
+          const T vx = __hip_atomic_load(in_x + czj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const T vy = __hip_atomic_load(in_y + czj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (take_x && okj && is_sentinel(nwx[j])) nwx[j] = vx;
+          if (take_y && okj && is_sentinel(nwy[j])) nwy[j] = vy;
+        }
         if (take_x) nwx[k] = mailbox_poll(in_x + lz, state);
         if (take_y) nwy[k] = mailbox_poll(in_y + lz, state);
       }
