@@ -127,8 +127,14 @@ class GpuDenseVector {
   }
   [[nodiscard]] lgpu_vec handle() const { return h_.get(); }
   [[nodiscard]] const GpuContext& context() const { return *ctx_; }
+  // takes ownership of a vector of this dtype that the library created
+  static GpuDenseVector adopt(lgpu_vec h, const GpuContext& ctx) {
+    return GpuDenseVector(h, ctx, AdoptTag{});
+  }
 
  private:
+  struct AdoptTag {};
+  GpuDenseVector(lgpu_vec h, const GpuContext& ctx, AdoptTag) : ctx_(&ctx), h_(h) {}
   struct Del {
     void operator()(lgpu_vec v) const { lgpu_vec_destroy(v); }
   };
@@ -278,6 +284,8 @@ class GpuCsrMatrix : public GpuMatrixBase {
   // csr_from_adjacent_stencil (tensor/dynamic/csr/create-from.hpp:37-239)
   explicit GpuCsrMatrix(const GpuStencilMatrixBase& stencil)
       : GpuMatrixBase(stencil.context(), from_stencil(stencil), stencil.is_symmetric()) {}
+  // takes ownership of a CSR operator with TReal values that the library created
+  GpuCsrMatrix(lgpu_op h, bool symmetric, const GpuContext& ctx) : GpuMatrixBase(ctx, h, symmetric) {}
 
  private:
   static constexpr int dtype = std::is_same_v<TReal, double> ? LGPU_F64 : LGPU_F32;
@@ -294,6 +302,71 @@ class GpuCsrMatrix : public GpuMatrixBase {
     check(lgpu_csr_from_stencil(s.handle(), dtype, &h));
     return h;
   }
+};
+
+// The system { lhs, rhs, compress_map } of csr_from_adjacent_stencil<Mat, Vec>
+// (tensor/dynamic/csr/create-from.hpp:37-239).  With compression the rows
+// whose diagonal is zero (ZeroToZero) are dropped and the rest renumbered in
+// order (create-from.hpp:54-55, 92-99, 197-229); compress_map holds the row of
+// every cell or -1, and is empty without compression.
+template <typename TReal = double>
+struct GpuCsrSystem {
+  GpuCsrMatrix<TReal> lhs;
+  GpuDenseVector<TReal> rhs;
+  std::vector<int64_t> compress_map;
+};
+
+template <typename TReal = double>
+inline GpuCsrSystem<TReal> csr_from_adjacent_stencil(const GpuStencilMatrixBase& stencil,
+                                                     bool compress) {
+  constexpr int dtype = std::is_same_v<TReal, double> ? LGPU_F64 : LGPU_F32;
+  std::vector<int64_t> map(stencil.row_num());
+  lgpu_op lhs = nullptr;
+  lgpu_vec rhs = nullptr;
+  check(lgpu_csr_system_from_stencil(stencil.handle(), dtype, compress ? 1 : 0, &lhs, &rhs,
+                                     map.data()));
+  const GpuContext& ctx = stencil.context();
+  GpuCsrSystem<TReal> sys{GpuCsrMatrix<TReal>(lhs, stencil.is_symmetric(), ctx),
+                          GpuDenseVector<TReal>::adopt(rhs, ctx), {}};
+  if (compress) sys.compress_map = std::move(map);
+  return sys;
+}
+
+// Effective diffusivity through the mid plane x = nx / 2, EffDiffCalc
+// (component-wise/sink/effective-diffusion.hpp:229-327).  The area part is
+// computed at construction (:292-295); none_materials flags the "none"
+// materials of a lookup operator (LookupSymmetricDiffusionInfo::none_materials).
+class GpuEffDiffCalc {
+ public:
+  explicit GpuEffDiffCalc(const GpuStencilMatrixBase& valuator) {
+    check(lgpu_eff_diff_area_part(valuator.handle(), nullptr, &area_part_));
+  }
+  GpuEffDiffCalc(const GpuStencilMatrixBase& valuator, const std::array<bool, 4>& none_materials) {
+    std::array<std::uint8_t, 4> flags{};
+    for (std::size_t m = 0; m < 4; ++m) flags[m] = none_materials[m] ? 1 : 0;
+    check(lgpu_eff_diff_area_part(valuator.handle(), flags.data(), &area_part_));
+  }
+  [[nodiscard]] double area_part() const { return area_part_; }
+
+  // sol indexed by cells
+  template <typename T>
+  double operator()(const GpuStencilMatrixBase& valuator, const GpuDenseVector<T>& sol) const {
+    double out = 0.0;
+    check(lgpu_eff_diff(valuator.handle(), nullptr, sol.handle(), area_part_, &out));
+    return out;
+  }
+  // sol indexed by the rows of a compressed system (effdiff(valuator, compress_map, sol))
+  template <typename TR, typename T>
+  double operator()(const GpuStencilMatrixBase& valuator, const GpuCsrSystem<TR>& sys,
+                    const GpuDenseVector<T>& sol) const {
+    double out = 0.0;
+    const lgpu_op map = sys.compress_map.empty() ? nullptr : sys.lhs.handle();
+    check(lgpu_eff_diff(valuator.handle(), map, sol.handle(), area_part_, &out));
+    return out;
+  }
+
+ private:
+  double area_part_ = 0.0;
 };
 
 // ---------------------------------------------------------------------------

@@ -119,6 +119,16 @@ int lgpu_csr_create(lgpu_ctx ctx, int64_t n, const int64_t* rowptr, const int32_
 /* csr_from_adjacent_stencil (tensor/dynamic/csr/create-from.hpp:37-239) without
  * compression: CSR copy of a stencil operator, values stored in `dtype`. */
 int lgpu_csr_from_stencil(lgpu_op stencil, int dtype, lgpu_op* out);
+/* csr_from_adjacent_stencil<Mat, Vec> (tensor/dynamic/csr/create-from.hpp:37-239):
+ * the whole system { lhs, rhs, compress_map }.  With compress != 0 the
+ * ZeroToZero compression applies (create-from.hpp:54-55, 92-99, 197-229): rows
+ * whose diagonal is zero are dropped and the others renumbered in order.
+ * *lhs receives the CSR operator (values in `dtype`), *rhs a new vector of its
+ * row count holding the Dirichlet right-hand side in `dtype`, and cmap (host,
+ * n[0]*n[1]*n[2] entries, may be NULL) the row of every cell or -1.  A
+ * compressed operator keeps its map for lgpu_eff_diff. */
+int lgpu_csr_system_from_stencil(lgpu_op stencil, int dtype, int compress, lgpu_op* lhs,
+                                 lgpu_vec* rhs, int64_t* cmap);
 int lgpu_op_destroy(lgpu_op op);
 int lgpu_op_rows(lgpu_op op, int64_t* n);
 
@@ -211,6 +221,28 @@ int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
 /* BiCgStabSolver (bicgstab/solver.hpp:51-151), right-preconditioned. */
 int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
                         const lgpu_krylov_params* params, int* iterations, double* res_hist);
+
+/* ---- effective diffusivity (component-wise/sink/effective-diffusion.hpp) ----
+ * Both need a stencil operator with n[0] >= 3: the mid plane x = n[0] / 2 and
+ * the plane after it must exist. */
+
+/* EffDiffCalc::area_part (effective-diffusion.hpp:230-250; AreaPartSink
+ * :49-121): (number of filled cells among the mid-plane cells and their +x
+ * neighbours) / (n[1] n[2]) * 0.5.  none_mats: 4 flags, nonzero marks a
+ * "none" material (LookupSymmetricDiffusionInfo::none_materials,
+ * tensor/dynamic/stencil/matrix/diffusion.hpp:66-81; the filled lookup
+ * of tensor/dynamic/stencil/valuator/lookup-wright.hpp:98-106); NULL for none.  A
+ * per-cell coefficient operator has no materials and counts every cell. */
+int lgpu_eff_diff_area_part(lgpu_op stencil, const uint8_t* none_mats, double* area_part);
+/* EffDiffCalc::operator() (effective-diffusion.hpp:298-327; EffDiffSink
+ * :127-222): S = sum over mid-plane cells i of D(i, i') (sol[i'] - sol[i]),
+ * i' = i + n[1] n[2], D the harmonic mean of the two coefficients (lookup:
+ * H(C[m_i], C[m_i'])); *eff_diff = S q_0 / (area_part L_1 L_2) L_0 / (p_end - p_start).
+ * map: NULL when sol is indexed by cells, else a compressed system operator
+ * from lgpu_csr_system_from_stencil whose compress map indexes sol (pairs
+ * with a dropped cell are skipped).  sol is f64 or f32 (differences taken in
+ * its type, products and the sum in f64). */
+int lgpu_eff_diff(lgpu_op stencil, lgpu_op map, lgpu_vec sol, double area_part, double* eff_diff);
 
 /* ---- adjoint of the solve (no counterpart in the reference) ---------------
  * Objective: the total outlet flux of a per-cell coefficient stencil without

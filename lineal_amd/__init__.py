@@ -110,6 +110,9 @@ SIGNATURES = {
     "lgpu_stencil_lookup_create": (C.c_int, [_vp, C.POINTER(_Box), _u8p, _f64p, C.POINTER(_vp)]),
     "lgpu_csr_create": (C.c_int, [_vp, _i64, _i64p, _i32p, _f64p, C.c_int, C.c_int, C.POINTER(_vp)]),
     "lgpu_csr_from_stencil": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "lgpu_csr_system_from_stencil": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_vp), C.POINTER(_vp), _i64p]),
+    "lgpu_eff_diff_area_part": (C.c_int, [_vp, _u8p, _f64p]),
+    "lgpu_eff_diff": (C.c_int, [_vp, _vp, _vp, C.c_double, _f64p]),
     "lgpu_op_destroy": (C.c_int, [_vp]),
     "lgpu_op_rows": (C.c_int, [_vp, _i64p]),
     "lgpu_op_rhs": (C.c_int, [_vp, _vp]),
@@ -255,6 +258,13 @@ class DenseVector:
             self.upload(host)
         elif fill is not None:
             self.fill(fill)
+
+    @classmethod
+    def _adopt(cls, h, n: int, dtype, ctx: "Context") -> "DenseVector":
+        """Wrap a vector handle the library created."""
+        v = cls.__new__(cls)
+        v.ctx, v.n, v.dtype, v.h = ctx, int(n), np.dtype(dtype), h
+        return v
 
     @property
     def size(self):
@@ -402,6 +412,30 @@ class StencilMatrix(_Matrix):
         h = _vp()
         _check(lib().lgpu_csr_from_stencil(self.h, _dtype_code(dtype), C.byref(h)))
         return CsrMatrix(h, self.ctx)
+
+
+@dataclass
+class CsrSystem:
+    """The { lhs, rhs, compress_map } of csr_from_adjacent_stencil
+    (tensor/dynamic/csr/create-from.hpp:37-239).  compress_map holds the row of
+    every cell or -1 (None without compression)."""
+
+    lhs: "CsrMatrix"
+    rhs: DenseVector
+    compress_map: np.ndarray | None
+
+
+def csr_from_adjacent_stencil(A: StencilMatrix, dtype=np.float64, compress: bool = False) -> CsrSystem:
+    """CSR copy of a stencil system; with compress, rows whose diagonal is zero
+    (ZeroToZero) are dropped and the rest renumbered in order
+    (create-from.hpp:54-55, 92-99, 197-229)."""
+    lhs, rhs = _vp(), _vp()
+    cmap = np.empty(A.row_num, np.int64)
+    _check(lib().lgpu_csr_system_from_stencil(A.h, _dtype_code(dtype), int(compress), C.byref(lhs),
+                                              C.byref(rhs), cmap.ctypes.data_as(_i64p)))
+    M = CsrMatrix(lhs, A.ctx)
+    b = DenseVector._adopt(rhs, M.row_num, dtype, A.ctx)
+    return CsrSystem(M, b, cmap if compress else None)
 
 
 class CsrMatrix(_Matrix):
@@ -604,6 +638,30 @@ def run_iterative_solver(solver: _Krylov, lhs, sol, rhs, rel_tol=1e-8, max_it=10
 
 # ---------------------------------------------------------------------------
 # adjoint of the solve (not in the reference)
+
+
+class EffDiffCalc:
+    """Effective diffusivity through the mid plane x = nx // 2
+    (component-wise/sink/effective-diffusion.hpp:229-327).  The area part is
+    computed once, at construction (:292-295); none_mats flags the "none"
+    materials of a lookup operator."""
+
+    def __init__(self, valuator: StencilMatrix, none_mats=None):
+        flags = None
+        if none_mats is not None:
+            self._none = np.ascontiguousarray([1 if f else 0 for f in none_mats], np.uint8)
+            assert self._none.size == 4
+            flags = self._none.ctypes.data_as(_u8p)
+        out = C.c_double()
+        _check(lib().lgpu_eff_diff_area_part(valuator.h, flags, C.byref(out)))
+        self.area_part = out.value
+
+    def __call__(self, valuator: StencilMatrix, sol: DenseVector, system: CsrSystem | None = None) -> float:
+        """sol is indexed by cells, or by the rows of a compressed system."""
+        m = None if system is None or system.compress_map is None else system.lhs.h
+        out = C.c_double()
+        _check(lib().lgpu_eff_diff(valuator.h, m, sol.h, self.area_part, C.byref(out)))
+        return out.value
 
 
 def outlet_flux(A: StencilMatrix, p: DenseVector) -> float:

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <thread>
@@ -164,6 +165,9 @@ struct lgpu_op_s {
   // csr
   CsrDev csr;
   setup::RowMatrix host_csr;
+  // compressed system from a stencil: row of every cell or -1 (i64 per cell)
+  DevBuf cmap;
+  int64_t cmap_cells = 0;
 
   int64_t rows() const { return is_csr ? csr.n : desc.nx * desc.ny * desc.nz; }
   void ensure_f32() {
@@ -778,27 +782,46 @@ extern "C" int lgpu_csr_create(lgpu_ctx ctx, int64_t n, const int64_t* rowptr, c
   API_END
 }
 
-extern "C" int lgpu_csr_from_stencil(lgpu_op stencil, int dtype, lgpu_op* out) {
-  API_BEGIN
-  LGPU_REQUIRE(stencil && out && !stencil->is_csr, "needs a stencil operator");
+namespace {
+
+// csr_from_adjacent_stencil (tensor/dynamic/csr/create-from.hpp:37-239): rows in
+// order, an off-diagonal kept iff nonzero, values cast to the matrix type,
+// columns sorted (csr/shared.hpp:751-763).  With compression a row whose
+// diagonal is zero is dropped and the kept rows renumbered in order
+// (create-from.hpp:92-99, 197-229); `map` receives the row of every cell or -1
+// and `keep` the cell of every row.
+std::unique_ptr<lgpu_op_s> csr_copy(lgpu_op stencil, int dtype, bool compress,
+                                    std::vector<int64_t>& map, std::vector<int64_t>& keep) {
+  LGPU_REQUIRE(stencil && !stencil->is_csr, "needs a stencil operator");
   LGPU_REQUIRE(dtype == LGPU_F64 || dtype == LGPU_F32, "unknown dtype");
   setup::RowMatrix rows;
   stencil_host_rows(stencil, rows);
-  // create-from.hpp:152-220: zero off-diagonals dropped, columns sorted, values cast
+  const int64_t n = rows.n;
+  map.assign(size_t(n), 0);
+  keep.clear();
+  for (int64_t i = 0; i < n; ++i) {
+    const bool kept = !compress || rows.vals[size_t(rows.rowptr[i + 1] - 1)] != 0.0;
+    map[size_t(i)] = kept ? int64_t(keep.size()) : -1;
+    if (kept) keep.push_back(i);
+  }
   auto op = std::make_unique<lgpu_op_s>();
   op->ctx = stencil->ctx;
   op->is_csr = true;
   setup::RowMatrix& m = op->host_csr;
-  m.n = rows.n;
+  m.n = int64_t(keep.size());
   m.f32 = dtype == LGPU_F32;
-  m.symmetric = true;
+  m.symmetric = rows.symmetric;
   m.rowptr.assign(1, 0);
   std::vector<std::pair<int32_t, double>> row;
-  for (int64_t i = 0; i < rows.n; ++i) {
+  for (const int64_t i : keep) {
     row.clear();
     for (int64_t k = rows.rowptr[i]; k < rows.rowptr[i + 1]; ++k) {
       if (rows.cols[k] != i && rows.vals[k] == 0.0) continue;
-      row.emplace_back(rows.cols[k], m.f32 ? double(float(rows.vals[k])) : rows.vals[k]);
+      // a kept coupling never reaches a dropped cell: the coupling to a cell
+      // with a zero diagonal is itself zero
+      const int64_t c = map[size_t(rows.cols[k])];
+      LGPU_REQUIRE(c >= 0, "a nonzero coupling reaches a dropped row");
+      row.emplace_back(int32_t(c), m.f32 ? double(float(rows.vals[k])) : rows.vals[k]);
     }
     std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
     for (const auto& [c, v] : row) {
@@ -808,8 +831,55 @@ extern "C" int lgpu_csr_from_stencil(lgpu_op stencil, int dtype, lgpu_op* out) {
     m.rowptr.push_back(int64_t(m.cols.size()));
   }
   op->csr.upload(m, m.f32);
+  return op;
+}
+
+}  // namespace
+
+extern "C" int lgpu_csr_from_stencil(lgpu_op stencil, int dtype, lgpu_op* out) {
+  API_BEGIN
+  LGPU_REQUIRE(out, "NULL output");
+  std::vector<int64_t> map, keep;
+  auto op = csr_copy(stencil, dtype, false, map, keep);
   LGPU_HIP_CHECK(hipDeviceSynchronize());
   *out = op.release();
+  API_END
+}
+
+extern "C" int lgpu_csr_system_from_stencil(lgpu_op stencil, int dtype, int compress,
+                                            lgpu_op* lhs, lgpu_vec* rhs, int64_t* cmap) {
+  API_BEGIN
+  LGPU_REQUIRE(lhs && rhs, "NULL output");
+  std::vector<int64_t> map, keep;
+  auto op = csr_copy(stencil, dtype, compress != 0, map, keep);
+  const bool f32 = dtype == LGPU_F32;
+  if (f32) stencil->ensure_f32();
+  hipStream_t st = stencil->ctx->stream;
+  auto b = std::make_unique<lgpu_vec_s>();
+  b->ctx = stencil->ctx;
+  b->n = op->csr.n;
+  b->f32 = f32;
+  b->own.alloc(size_t(b->n) * esize(f32));
+  b->ptr = b->own.ptr;
+  if (compress) {
+    const int64_t n = stencil->rows();
+    op->cmap.alloc(size_t(n) * sizeof(int64_t));
+    op->cmap_cells = n;
+    LGPU_HIP_CHECK(hipMemcpy(op->cmap.ptr, map.data(), size_t(n) * sizeof(int64_t), hipMemcpyHostToDevice));
+    DevBuf full, rows;
+    full.alloc(size_t(n) * esize(f32));
+    rows.alloc(keep.size() * sizeof(int64_t));
+    LGPU_HIP_CHECK(hipMemcpy(rows.ptr, keep.data(), keep.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    stencil_rhs(stencil->desc, f32, full.ptr, st);
+    vec_gather(full.ptr, f32, rows.as<int64_t>(), b->n, b->ptr, st);
+    LGPU_HIP_CHECK(hipStreamSynchronize(st));
+  } else {
+    stencil_rhs(stencil->desc, f32, b->ptr, st);
+    LGPU_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  if (cmap) std::copy(map.begin(), map.end(), cmap);
+  *lhs = op.release();
+  *rhs = b.release();
   API_END
 }
 
@@ -1288,6 +1358,74 @@ extern "C" int lgpu_outlet_flux_gradient(lgpu_op op, lgpu_vec p, lgpu_vec lambda
   outlet_flux_gradient(op->desc, static_cast<const double*>(p->ptr),
                        static_cast<const double*>(lambda->ptr), static_cast<double*>(grad->ptr),
                        op->ctx->stream);
+  API_END
+}
+
+namespace {
+
+void check_eff_diff_op(lgpu_op op) {
+  LGPU_REQUIRE(op && !op->is_csr, "the effective diffusivity needs a stencil operator");
+  // the plane x = nx / 2 needs a +x neighbour plane
+  LGPU_REQUIRE(op->desc.nx >= 3, "the effective diffusivity needs at least three cells along x");
+}
+
+// the valuators' filled and diffusion-factor lookups
+// (tensor/dynamic/stencil/valuator/lookup-wright.hpp:91-106)
+EffDiffTables eff_diff_tables(lgpu_op op, const uint8_t* none_mats) {
+  EffDiffTables t{};
+  for (int m = 0; m < 4; ++m) t.filled[m] = none_mats && none_mats[m] ? 0.0 : 1.0;
+  for (int m = 0; m < 16; ++m) t.factor[m] = host_hmean(op->desc.mc[m / 4], op->desc.mc[m % 4]);
+  return t;
+}
+
+double eff_diff_reduce(lgpu_op op, const std::function<void(double*)>& launch) {
+  lgpu_ctx ctx = op->ctx;
+  const int64_t np = eff_diff_partials(op->desc);
+  double* part = ctx->partials_for(np);
+  launch(part);
+  reduce_partials(part, np, ctx->scal.as<double>() + SC_EFFDIFF, ctx->stream);
+  return read_scalar(ctx, SC_EFFDIFF);
+}
+
+}  // namespace
+
+extern "C" int lgpu_eff_diff_area_part(lgpu_op op, const uint8_t* none_mats, double* area_part) {
+  API_BEGIN
+  check_eff_diff_op(op);
+  LGPU_REQUIRE(area_part, "NULL area_part");
+  const EffDiffTables t = eff_diff_tables(op, none_mats);
+  const double filled = eff_diff_reduce(
+      op, [&](double* part) { eff_diff_filled(op->desc, t, part, op->ctx->stream); });
+  // effective-diffusion.hpp:113-116
+  *area_part = filled / double(op->desc.ny * op->desc.nz) * 0.5;
+  API_END
+}
+
+extern "C" int lgpu_eff_diff(lgpu_op op, lgpu_op map, lgpu_vec sol, double area_part,
+                             double* eff_diff) {
+  API_BEGIN
+  check_eff_diff_op(op);
+  LGPU_REQUIRE(eff_diff, "NULL eff_diff");
+  const int64_t* cmap = nullptr;
+  int64_t n = op->rows();
+  if (map) {
+    LGPU_REQUIRE(map->is_csr && map->cmap.ptr && map->cmap_cells == n,
+                 "map must be a compressed system of this stencil's size");
+    cmap = map->cmap.as<int64_t>();
+    n = map->rows();
+  }
+  check_vec(sol, n, "sol");
+  const EffDiffTables t = eff_diff_tables(op, nullptr);
+  const double sum = eff_diff_reduce(op, [&](double* part) {
+    eff_diff_flux(op->desc, t, sol->ptr, sol->f32, cmap, part, op->ctx->stream);
+  });
+  // effective-diffusion.hpp:320-326
+  const lgpu_box& b = op->box;
+  double out = sum;
+  out *= op->desc.q[0];
+  out /= area_part * ((1.0 * b.len[1]) * b.len[2]);
+  out *= b.len[0];
+  *eff_diff = out / (b.p_end - b.p_start);
   API_END
 }
 

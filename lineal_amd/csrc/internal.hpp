@@ -153,6 +153,23 @@ void outlet_flux_adjoint_rhs(const StencilDesc& s, double* g, hipStream_t st);
 void outlet_flux_gradient(const StencilDesc& s, const double* p, const double* lam, double* grad,
                           hipStream_t st);
 
+// effective diffusivity through the mid plane x = nx / 2 (needs nx >= 3)
+struct EffDiffTables {
+  double filled[4];   // lookup: 1 for a filled material, 0 for a "none" material
+  double factor[16];  // lookup: H(C[m], C[m']) at m * 4 + m'
+};
+int64_t eff_diff_partials(const StencilDesc& s);
+// partials += filled(i) + filled(i + ny nz) over the mid plane
+void eff_diff_filled(const StencilDesc& s, const EffDiffTables& t, double* partials,
+                     hipStream_t st);
+// partials += D(i, i + ny nz) (sol[i + ny nz] - sol[i]); with cmap (cell -> row
+// or -1) sol is indexed by rows and pairs with a dropped cell are skipped
+void eff_diff_flux(const StencilDesc& s, const EffDiffTables& t, const void* sol, bool f32,
+                   const int64_t* cmap, double* partials, hipStream_t st);
+// dst[r] = src[idx[r]], r < n
+void vec_gather(const void* src, bool f32, const int64_t* idx, int64_t n, void* dst,
+                hipStream_t st);
+
 // vector kernels
 void vec_fill(void* x, int64_t n, bool f32, double v, hipStream_t st);
 void vec_fill_int(int* x, int64_t n, int v, hipStream_t st);
@@ -188,6 +205,7 @@ enum : int {
   SC_ALPHA = 7,
   SC_OMEGA = 8,
   SC_FLUX = 9,    // outlet flux
+  SC_EFFDIFF = 10, // mid-plane filled count / flux sum
   SC_COUNT = 16,
 };
 // x += (num/den) p, r -= (num/den) q, partial += r^2

@@ -521,7 +521,7 @@ int64_t csr_partials(const CsrDesc& a) { return (a.n + NT - 1) / NT; }
 // values, f32 vectors with f32 values.
 void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y, double omega,
                double* partials, hipStream_t st) {
-  (void)0;
+  if (a.n == 0) return;  // an empty system (every row compressed away): nothing to launch
   if (a.f32) dispatch_apply<float, float>(a, mode, x, b, y, omega, partials, st);
   else dispatch_apply<double, double>(a, mode, x, b, y, omega, partials, st);
 }
@@ -530,6 +530,7 @@ void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const vo
                        double omega, bool forward, bool ultra, int seg, hipStream_t st) {
   LGPU_REQUIRE(x_in != x_out, "SOR sweep needs distinct input and output vectors");
   LGPU_REQUIRE(seg >= 1 && seg <= kMaxSegmentRows, "segment length out of range");
+  if (a.n == 0) return;
   if (a.f32) dispatch_segment<float, float>(a, x_in, x_out, b, omega, forward, ultra, seg, st);
   else dispatch_segment<double, double>(a, x_in, x_out, b, omega, forward, ultra, seg, st);
 }
@@ -537,6 +538,7 @@ void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const vo
 void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
              bool forward, bool ultra, int* sched, int* flags, int epoch, hipStream_t st) {
   LGPU_REQUIRE(x_in != x_out, "SOR sweep needs distinct input and output vectors");
+  if (a.n == 0) return;
   vec_fill_int(sched, 1, 0, st);
   if (a.f32) dispatch_sor<float, float>(a, x_in, x_out, b, omega, forward, ultra, sched, flags, epoch, st);
   else dispatch_sor<double, double>(a, x_in, x_out, b, omega, forward, ultra, sched, flags, epoch, st);

@@ -133,6 +133,8 @@ def lib():
             "or_outlet_flux": (C.c_double, [sp, f64p]),
             "or_outlet_flux_adjoint_rhs": (None, [sp, f64p]),
             "or_outlet_flux_gradient": (None, [sp, f64p, f64p, f64p]),
+            "or_eff_diff_area_part": (C.c_double, [sp, C.POINTER(C.c_uint8)]),
+            "or_eff_diff": (C.c_double, [sp, i64p, f64p, C.c_double]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -450,3 +452,30 @@ def adjoint_outlet_flux(S, hierarchy=None, precon=1, rel_tol=1e-10, max_it=1000)
     lam, it_a, _ = pcg(S, g, hierarchy, precon=precon if hierarchy else 0, rel_tol=rel_tol,
                        max_it=max_it)
     return outlet_flux(S, p), outlet_flux_gradient(S, p, lam), p, lam, it_f, it_a
+
+
+def eff_diff_area_part(S, none_mats=None):
+    """EffDiffCalc's area part: filled cells of the mid plane x = nx // 2 and
+    the plane after it, / (ny nz) * 0.5."""
+    if S.dims[0] < 3:
+        raise ValueError("the effective diffusivity needs at least three cells along x")
+    flags = None
+    if none_mats is not None:
+        arr = np.ascontiguousarray([1 if f else 0 for f in none_mats], dtype=np.uint8)
+        assert arr.size == 4
+        flags = _p(arr, C.c_uint8)
+    return float(lib().or_eff_diff_area_part(C.byref(S.s), flags))
+
+
+def eff_diff(S, sol, area_part, cmap=None):
+    """EffDiffCalc::operator(): the mid-plane flux normalised to an effective
+    diffusivity; with cmap, sol is indexed by the compressed system's rows."""
+    if S.dims[0] < 3:
+        raise ValueError("the effective diffusivity needs at least three cells along x")
+    sol = np.ascontiguousarray(sol, dtype=np.float64)
+    cm = None
+    if cmap is not None:
+        cmap = np.ascontiguousarray(cmap, dtype=np.int64)
+        assert cmap.size == S.N
+        cm = _p(cmap, C.c_int64)
+    return float(lib().or_eff_diff(C.byref(S.s), cm, _pf(sol), float(area_part)))

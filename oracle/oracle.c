@@ -24,6 +24,11 @@
  *    "parity unpinned": the reference cannot be built here.
  *  - CG / BiCGStab / PCG-AMG: exact termination on tiny SPD systems, the
  *    reference's thresholds (test/shared-stencil.hpp:157-224), scipy solves.
+ *  - Effective diffusivity: the reference's known answer for diagonal lines
+ *    (test/aux/materials.hpp:235-248 via test/shared-stencil.hpp:226-239) on
+ *    exact and solved fields, with anisotropic lengths and reversed borders;
+ *    a brute-force count of the area part with "none" materials; and the
+ *    compressed 4x7x6 system of test/shared-csr.cpp:193-252.
  */
 #include "oracle.h"
 
@@ -1127,4 +1132,62 @@ void or_outlet_flux_gradient(const or_stencil* s, const double* p, const double*
         }
         grad[k] = acc;
       }
+}
+
+/* ---------------------------------------------------------------------------
+ * Effective diffusivity, EffDiffCalc (component-wise/sink/effective-diffusion.hpp:
+ * 229-327), one thread: the mid plane x = nx / 2 holds the cells
+ * [mid ny nz, (mid + 1) ny nz) and each pairs with its +x neighbour.
+ *
+ * area part (AreaPartSink, :49-121, and :230-250): the integer count of
+ * filled cells among both cells of every pair, divided by ny nz, times 0.5.
+ * A lookup cell is filled unless its material is "none"
+ * (lookup-wright.hpp:98-106); per-cell coefficients have no materials and
+ * every cell counts. */
+double or_eff_diff_area_part(const or_stencil* s, const uint8_t* none_mats) {
+  if (s->n[0] < 3) return NAN;
+  const int64_t other = s->n[1] * s->n[2], mid = s->n[0] / 2;
+  int64_t sum = 0;
+  for (int64_t i1 = mid * other; i1 < (mid + 1) * other; ++i1) {
+    const int64_t i2 = i1 + other;
+    int64_t f1 = 1, f2 = 1;
+    if (s->kind == 1 && none_mats) {
+      f1 = none_mats[s->mat[i1]] ? 0 : 1;
+      f2 = none_mats[s->mat[i2]] ? 0 : 1;
+    }
+    sum += f1 + f2;
+  }
+  return (double)sum / (double)other * 0.5;
+}
+
+/* the sink's sum (EffDiffSink, :127-222): D(i1, i2) (sol[i2] - sol[i1]) in index
+ * order, D the valuator's diffusion coefficient without the axis quotient
+ * (stationary-diffusion-value.hpp:91-96, stationary-diffusion-lookup.hpp:83-89);
+ * with cmap sol is indexed by compressed rows and pairs with a dropped cell
+ * are skipped (:175-190).  Then the normalisation of :320-326. */
+double or_eff_diff(const or_stencil* s, const int64_t* cmap, const double* sol,
+                   double area_part) {
+  if (s->n[0] < 3) return NAN;
+  double q[3];
+  or_axis_quotients(s, q);
+  const int64_t other = s->n[1] * s->n[2], mid = s->n[0] / 2;
+  double sum = 0.0;
+  for (int64_t i1 = mid * other; i1 < (mid + 1) * other; ++i1) {
+    const int64_t i2 = i1 + other;
+    int64_t k1 = i1, k2 = i2;
+    if (cmap) {
+      k1 = cmap[i1];
+      k2 = cmap[i2];
+      if (k1 < 0 || k2 < 0) continue;
+    }
+    const double d = s->kind == 1
+                         ? or_harmonic_mean_f64(s->mat_coeffs[s->mat[i1]], s->mat_coeffs[s->mat[i2]])
+                         : or_harmonic_mean_f64(s->coeff[i1], s->coeff[i2]);
+    sum += d * (sol[k2] - sol[k1]);
+  }
+  double out = sum;
+  out *= q[0];
+  out /= area_part * ((1.0 * s->len[1]) * s->len[2]);
+  out *= s->len[0];
+  return out / (s->p_end - s->p_start);
 }

@@ -176,6 +176,15 @@ void or_outlet_flux_adjoint_rhs(const or_stencil* s, double* g);
 void or_outlet_flux_gradient(const or_stencil* s, const double* p, const double* lam,
                              double* grad);
 
+/* Effective diffusivity through the mid plane x = nx / 2 (EffDiffCalc,
+ * component-wise/sink/effective-diffusion.hpp:229-327; needs nx >= 3 so
+ * that the plane after the mid plane exists, NaN otherwise).
+ * none_mats: 4 flags or NULL; cmap: the compress map of a ZeroToZero CSR
+ * system (sol indexed by its rows) or NULL (sol indexed by cells). */
+double or_eff_diff_area_part(const or_stencil* s, const uint8_t* none_mats);
+double or_eff_diff(const or_stencil* s, const int64_t* cmap, const double* sol,
+                   double area_part);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,8 @@
 //   CSR copy:         CSR SpMV equals the matrix-free SpMV
 //   adjoint:          outlet flux of a constant-coefficient values stencil
 //                     equals n_y n_z c dp / n_x and dJ/dlog c_k = J_col^2 / c_k
+//   effective diffusivity: the compressed 4 x 7 x 6 system of
+//                     test/shared-csr.cpp:193-252 gives the mean coefficient
 // Prints one line per check; the exit status is the number of failed checks.
 #include <cmath>
 #include <cstdio>
@@ -120,6 +122,34 @@ int main() {
       const double jc = 1.5 / double(d2[0]);
       const double e = inf_norm_diff(g.to_host(), jc * jc / 1.5);
       expect(e <= 1e-11, "outlet flux gradient closed form", e);
+    }
+    // effective diffusivity of the compressed system (test/shared-csr.cpp:193-252):
+    // 4 x 7 x 6 diagonal lines, ZeroToZero, p_start = 1, p_end = 0; the mean
+    // coefficient over a cross-section (test/aux/materials.hpp:235-248)
+    {
+      const std::array<int64_t, 3> d3{4, 7, 6};
+      lg::BoxInfo b3(d3);
+      b3.zero_to_one = false;
+      std::vector<std::uint8_t> m3(size_t(b3.cell_num()));
+      double mean = 0.0;
+      for (int64_t x = 0; x < d3[0]; ++x)
+        for (int64_t y = 0; y < d3[1]; ++y)
+          for (int64_t z = 0; z < d3[2]; ++z) {
+            const int id = int((y + z) % 4);
+            m3[size_t((x * d3[1] + y) * d3[2] + z)] = std::uint8_t(id);
+            if (x == 0) mean += double(id);
+          }
+      mean /= double(d3[1] * d3[2]);
+      lg::GpuStencilLookupMatrix L(b3, m3, {0.0, 1.0, 2.0, 3.0}, ctx);
+      auto sys = lg::csr_from_adjacent_stencil<double>(L, true);
+      const size_t nc = sys.lhs.row_num();
+      lg::GpuDenseVector<double> x(nc, 0.0, ctx);
+      lg::GpuConjugateGradients<> cg(8, nullptr);
+      const auto r = lg::run_iterative_solver(cg, sys.lhs, x, sys.rhs, {1e-14, 400});
+      lg::GpuEffDiffCalc calc(L);
+      const double eff = calc(L, sys, x);
+      expect(r.converged() && nc < L.row_num() && std::fabs(eff - mean) <= 1e-10,
+             "effective diffusivity of the compressed system", std::fabs(eff - mean));
     }
   } catch (const lg::GpuError& e) {
     std::printf("FAIL exception: %s\n", e.what());

@@ -1,6 +1,7 @@
 """The header-only C++ classes on the GPU: tests/cpp/stencil_api.cpp runs the
 reference's stencil test scenarios (known solution, Dirichlet residual, CSR
-copy, smoothers, adjoint closed form) through them."""
+copy, smoothers, adjoint closed form, effective diffusivity of the compressed
+system) through them."""
 
 import os
 import subprocess
@@ -19,4 +20,4 @@ def test_cpp_api_program():
     res = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
-    assert res.stdout.count("ok  ") == 8
+    assert res.stdout.count("ok  ") == 9

@@ -437,6 +437,86 @@ def test_reference_thresholds_dirichlet_rhs_and_effective_diffusion():
     assert abs(eff - problems.diagonal_lines_eff_diff(dims, mc)) <= 1e-13
 
 
+# ---------------------------------------------------------------------------
+# effective diffusivity (component-wise/sink/effective-diffusion.hpp:229-327)
+
+
+def linear_field(dims, p_start, p_end):
+    """p = p_start + (p_end - p_start)(x + 1/2)/nx, the exact discrete solution
+    for coefficients constant along x (test/shared-stencil.hpp:226-239)."""
+    x = np.arange(dims[0], dtype=np.float64)[:, None, None]
+    p = p_start + (p_end - p_start) * (x + 0.5) / dims[0]
+    return np.ascontiguousarray(np.broadcast_to(p, dims)).reshape(-1)
+
+
+@pytest.mark.parametrize("dims,lengths,ps,pe", [((4, 9, 7), None, 1.0, 0.0),
+                                                ((7, 5, 6), (3.0, 2.0, 7.5), 0.2, 1.7),
+                                                ((3, 3, 1), (1.0, 4.0, 0.5), 5.0, -1.0)])
+@pytest.mark.parametrize("kind", ["lookup", "values"])
+def test_eff_diff_known_answer_on_the_exact_field(dims, lengths, ps, pe, kind):
+    """The reference's known answer for diagonal lines: the mean coefficient over
+    a cross-section (test/aux/materials.hpp:235-248).  Anisotropic lengths pin
+    q_0 and L_0 L_1 L_2; reversed borders pin the sign."""
+    mc = (0.5, 1.0, 2.0, 3.0)
+    m = problems.diagonal_lines(dims)
+    kw = dict(lengths=lengths, p_start=ps, p_end=pe)
+    S = (O.Stencil(dims, materials=m, mat_coeffs=mc, **kw) if kind == "lookup"
+         else O.Stencil(dims, coeff=np.array(mc)[m], **kw))
+    area = O.eff_diff_area_part(S)
+    assert area == 1.0
+    got = O.eff_diff(S, linear_field(dims, ps, pe), area)
+    ref = problems.diagonal_lines_eff_diff(dims, mc)
+    assert abs(got - ref) <= 1e-13 * ref
+
+
+def test_eff_diff_on_a_solved_field_meets_the_reference_bound():
+    """test/shared-stencil.hpp:226-239: after PCG-AMG the effective diffusivity
+    equals the known answer within the residual bound 2e-14."""
+    S, dims, m = reference_system()
+    H = O.Hierarchy(S)
+    x, _, _ = O.pcg(S, S.rhs(), H, precon=1, rel_tol=1e-16, max_it=32)
+    got = O.eff_diff(S, x, O.eff_diff_area_part(S))
+    assert abs(got - problems.diagonal_lines_eff_diff(dims, (0.0, 1.0, 2.0, 3.0))) <= 2e-14
+
+
+@pytest.mark.parametrize("dims", [(5, 6, 7), (8, 3, 4), (3, 1, 9)])
+def test_eff_diff_area_part_brute_force(dims):
+    """Filled cells of the planes x = nx // 2 and x = nx // 2 + 1 counted by
+    hand, with random materials (varying along x) and two "none" materials."""
+    rng = np.random.default_rng(sum(dims))
+    m = rng.integers(0, 4, size=int(np.prod(dims)), dtype=np.uint8)
+    none = (True, False, False, True)
+    S = O.Stencil(dims, materials=m, mat_coeffs=(0.0, 1.0, 2.0, 3.0))
+    g = m.reshape(dims)
+    mid = dims[0] // 2
+    filled = sum(int(not none[g[x, y, z]]) for x in (mid, mid + 1)
+                 for y in range(dims[1]) for z in range(dims[2]))
+    assert O.eff_diff_area_part(S, none) == filled / (dims[1] * dims[2]) * 0.5
+    assert O.eff_diff_area_part(S) == 1.0
+
+
+def test_eff_diff_compressed_system_of_the_reference():
+    """test/shared-csr.cpp:193-252: 4x7x6 diagonal lines, coefficients
+    {0, 1, 2, 3}, ZeroToZero, compressed CSR; the effective diffusivity of the
+    solved compressed system through the compress map is the known answer
+    (the reference asserts |eff - 1.5| <= bound^(1/3))."""
+    dims = (4, 7, 6)
+    mc = (0.0, 1.0, 2.0, 3.0)
+    m = problems.diagonal_lines(dims)
+    S = O.Stencil(dims, materials=m, mat_coeffs=mc, zero_to_one=False)
+    Cc, rhs, cmap = S.to_csr(compress=True)
+    x, _, hist = O.pcg(Cc, rhs, None, precon=0, rel_tol=1e-15, max_it=400)
+    area = O.eff_diff_area_part(S)
+    got = O.eff_diff(S, x, area, cmap)
+    ref = problems.diagonal_lines_eff_diff(dims, mc)
+    assert abs(got - ref) <= 1e-12
+    assert abs(got - 1.5) <= 1e-13 ** (1.0 / 3.0)
+    # the same field expanded to cells gives the same value without the map
+    full = np.zeros(S.N)
+    full[cmap >= 0] = x[cmap[cmap >= 0]]
+    assert abs(O.eff_diff(S, full, area) - got) <= 1e-15
+
+
 def test_cg_exact_termination_tiny_spd():
     # constant coefficients: few distinct eigenvalues, so CG ends after at most that many steps
     dims = (3, 4, 2)
