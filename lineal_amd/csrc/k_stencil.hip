@@ -34,7 +34,7 @@ namespace lgpu {
 namespace {
 
 constexpr int WZ = 64;        // lanes of a wavefront, along z
-constexpr int kMaxWaves = 16;  // wavefronts per block (at most)
+constexpr int kMaxWaves = 8;   // wavefronts per block (at most; caps VGPRs at 256)
 // y rows per wavefront, and plane slots held in registers (the current plane,
 // the next one, and kSlots - 2 planes in flight ahead of them)
 template <typename T>
@@ -312,6 +312,17 @@ __global__ __launch_bounds__(WZ * kMaxWaves) void stencil_kernel(StencilDesc s,
   T px[NS][YR];
   // carried state: the face weight and value behind the march
   T wbh[YR], xbh[YR];
+  // values from outside the block's own rows (right-hand side, y halo rows,
+  // z halo cells, the batched z face's cell), loaded one plane ahead into the
+  // slot of their plane
+  struct Halo {
+    R cym, cyp, fa;
+    T xym, xyp;
+    R hc[YR];
+    T hx[YR];
+    T pb[YR];
+  };
+  Halo hl[NS];
 
   auto march = [&](auto DIR) {
     constexpr int dir = decltype(DIR)::value;
@@ -324,8 +335,40 @@ __global__ __launch_bounds__(WZ * kMaxWaves) void stencil_kernel(StencilDesc s,
         else px[slot][k] = T(0);
       }
     };
+    // planes outside [x0, x1) get empty ranges: no memory traffic
+    auto load_halo = [&](Halo& h, int p) {
+      if constexpr (NEED_B) {
 #pragma unroll
-    for (int j = 0; j + 1 < NS; ++j) load_rows(j, first + j * dir);
+        for (int k = 0; k < YR; ++k) h.pb[k] = bload<T>(trs(bv, p, rin[k], x0, x1 - 1), vzt, rot[k]);
+      }
+      h.cym = h.cyp = h.fa = R(0);
+      h.xym = h.xyp = T(0);
+#pragma unroll
+      for (int k = 0; k < YR; ++k) {
+        h.hc[k] = R(0);
+        h.hx[k] = T(0);
+      }
+      if (wy == 0) {
+        h.cym = C::widen(bload<E>(crs(p, has_ym, x0, x1 - 1), vze, om * SE));
+        if constexpr (HAS_X) h.xym = bload<T>(trs(xv, p, has_ym, x0, x1 - 1), vzt, om * ST);
+      }
+      if (wy == g.nwy - 1) {
+        h.cyp = C::widen(bload<E>(crs(p, has_yp, x0, x1 - 1), vze, op * SE));
+        if constexpr (HAS_X) h.xyp = bload<T>(trs(xv, p, has_yp, x0, x1 - 1), vzt, op * ST);
+      }
+      if (zhalo_mem) {
+#pragma unroll
+        for (int k = 0; k < YR; ++k) {
+          h.hc[k] = C::widen(bload<E>(crs(p, rin[k], x0, x1 - 1), hze, roe[k]));
+          if constexpr (HAS_X) h.hx[k] = bload<T>(trs(xv, p, rin[k], x0, x1 - 1), hzt, rot[k]);
+        }
+      }
+      if (wz == 0 && fh_lane) h.fa = C::widen(bload<E>(crs(p, true, x0, x1 - 1), fh_cell * SE, 0));
+    };
+    load_rows(0, first);
+    load_halo(hl[0], first);
+#pragma unroll
+    for (int j = 1; j + 1 < NS; ++j) load_rows(j, first + j * dir);
     load_rows(NS - 1, first - dir);
 #pragma unroll
     for (int k = 0; k < YR; ++k) {
@@ -340,42 +383,24 @@ __global__ __launch_bounds__(WZ * kMaxWaves) void stencil_kernel(StencilDesc s,
     auto plane = [&](auto J, int p) {
       constexpr int s0 = decltype(J)::value % NS, s1 = (decltype(J)::value + 1) % NS;
       constexpr int s2 = (decltype(J)::value + NS - 1) % NS;
-      // Loads for this plane that lie outside the block (right-hand side,
-      // halo rows and cells) are issued before the prefetch of plane
+      // This plane's values from outside the block were loaded one plane
+      // ago.  The next plane's are issued before the prefetch of plane
       // p+(NS-1)dir: memory operations complete in issue order, so waiting for
-      // them does not wait for the prefetch.
+      // them one plane later does not wait for that prefetch.
+      load_halo(hl[s1], p + dir);
+      load_rows(s2, p + (NS - 1) * dir);
+      const Halo& h = hl[s0];
       T pb[YR];
-      if constexpr (NEED_B) {
-#pragma unroll
-        for (int k = 0; k < YR; ++k) pb[k] = bload<T>(trs(bv, p, rin[k], x0, x1 - 1), vzt, rot[k]);
-      }
-      R cym = R(0), cyp = R(0);
-      T xym = T(0), xyp = T(0);
       R hc[YR];
       T hx[YR];
 #pragma unroll
       for (int k = 0; k < YR; ++k) {
-        hc[k] = R(0);
-        hx[k] = T(0);
+        pb[k] = h.pb[k];
+        hc[k] = h.hc[k];
+        hx[k] = h.hx[k];
       }
-      R fa = R(0);
-      if (wy == 0) {
-        cym = C::widen(bload<E>(crs(p, has_ym, p, p), vze, om * SE));
-        if constexpr (HAS_X) xym = bload<T>(trs(xv, p, has_ym, p, p), vzt, om * ST);
-      }
-      if (wy == g.nwy - 1) {
-        cyp = C::widen(bload<E>(crs(p, has_yp, p, p), vze, op * SE));
-        if constexpr (HAS_X) xyp = bload<T>(trs(xv, p, has_yp, p, p), vzt, op * ST);
-      }
-      if (zhalo_mem) {
-#pragma unroll
-        for (int k = 0; k < YR; ++k) {
-          hc[k] = C::widen(bload<E>(crs(p, rin[k], p, p), hze, roe[k]));
-          if constexpr (HAS_X) hx[k] = bload<T>(trs(xv, p, rin[k], p, p), hzt, rot[k]);
-        }
-      }
-      if (wz == 0 && fh_lane) fa = C::widen(bload<E>(crs(p, true, p, p), fh_cell * SE, 0));
-      load_rows(s2, p + (NS - 1) * dir);
+      R cym = h.cym, cyp = h.cyp, fa = h.fa;
+      T xym = h.xym, xyp = h.xyp;
       // publish this plane's boundary rows and cells
       const int par = p & 1;
       ex.rc[ex.row(par, 0, wave, lane)] = pc[s0][0];
