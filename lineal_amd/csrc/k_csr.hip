@@ -252,6 +252,170 @@ size_t wave_lds_bytes(int64_t entries) {
 // through its trailing old-value entries without further checks.  Every row
 // performs the oracle's operations in its order (sweep.hpp:134-200 with the
 // segment's own new values).
+// Register path of the wave segment sweep.  In the prologue every lane takes
+// its row apart in parallel: the leading old-value entries go into the
+// accumulator at once, the new-value entries (at most kNewMax, one contiguous
+// column range) become (value, segment index) pairs plus a bit mask of the
+// rows they wait for, and the trailing old-value entries (at most kTrailMax)
+// become (value, old value) pairs.  A round is then a mask test, independent
+// LDS reads of the finished rows' values and the FMA chain, in the row's
+// column order, so the results are those of the LDS-walking rounds below.
+// Returns false (nothing written) when some row of the wave has longer lists.
+constexpr int kNewMax = 16;    // multiples of 4
+constexpr int kTrailMax = 24;
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
+__device__ __forceinline__ bool wave_segment_registers(const Staged<T, V>* st, T* s_new,
+                                                       int64_t s0, int64_t s1, int64_t e0,
+                                                       const int64_t* __restrict__ rp,
+                                                       const T* __restrict__ xin,
+                                                       T* __restrict__ xout,
+                                                       const T* __restrict__ b, T omega) {
+  using S = Staged<T, V>;
+  constexpr int NM = kNewMax, TM = kTrailMax;
+  const int lane = threadIdx.x;
+  const int64_t i = s0 + lane;
+  const bool live = i < s1;
+  T acc = T(0), d = T(1), div = T(0), xi = T(0);
+  bool has_diag = false;
+  V nval[NM], tval[TM];
+  int nidx[NM];
+  T told[TM];
+  uint64_t deps = 0;
+  int nn = 0, nt = 0;
+  auto outside_div = [&](const S& e) {
+    if constexpr (ULTRA) {
+      if (e.col < s0 || e.col >= s1) {
+        const T a = T(e.val);
+        div += a < T(0) ? -a : a;
+      }
+    }
+  };
+  if (live) {
+    int64_t k = rp[i] - e0;
+    const int64_t kend = rp[i + 1] - e0;
+    acc = b[i];
+    xi = HAS_X ? xin[i] : T(0);
+    // leading old-value entries: forward the columns below s0, backward below i
+    const int64_t lead = FWD ? s0 : i;
+    while (k < kend) {
+      const S e = st[k];
+      if (e.col >= lead) break;
+      outside_div(e);
+      acc = fma(-T(e.val), e.old, acc);
+      ++k;
+    }
+    int64_t kn0 = k, kn1 = k;
+    if constexpr (FWD) {
+      while (k < kend && st[k].col < i) ++k;
+      kn1 = k;
+      if (k < kend && st[k].col == i) {
+        d = T(st[k].val);
+        has_diag = true;
+        ++k;
+      }
+    } else {
+      if (k < kend && st[k].col == i) {
+        d = T(st[k].val);
+        if constexpr (ULTRA) acc = fma(-d, xi, acc);
+        ++k;
+      }
+      kn0 = k;
+      while (k < kend && st[k].col < s1) ++k;
+      kn1 = k;
+    }
+    nn = int(kn1 - kn0);
+    nt = int(kend - k);
+    if (nn <= NM && nt <= TM) {
+#pragma unroll
+      for (int u = 0; u < NM; ++u) {
+        nval[u] = V(0);
+        nidx[u] = lane;
+        if (u < nn) {
+          const S e = st[kn0 + u];
+          nval[u] = e.val;
+          nidx[u] = int(e.col - s0);
+          deps |= uint64_t(1) << (e.col - s0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < TM; ++u) {
+        tval[u] = V(0);
+        told[u] = T(0);
+        if (u < nt) {
+          const S e = st[k + u];
+          tval[u] = e.val;
+          told[u] = e.old;
+          outside_div(e);
+        }
+      }
+    }
+  }
+  if (__ballot(nn > NM || nt > TM)) return false;
+  // Unused list slots hold value 0 with old value 0 (trailing) or point at the
+  // lane's own row, which reads 0 until the lane finishes (new values), and
+  // fma(-0, 0, a) == a for every a: padded slots leave the accumulator's bits
+  // unchanged, so every lane runs the same chain.  Its length is the wave's
+  // longest list, rounded up to groups of four.
+  s_new[lane] = T(0);
+  const int nn_w = wave_max_int(nn), nt_w = wave_max_int(nt);
+  const T m1w = T(1) - omega;
+  bool finished = !live;
+  uint64_t done = __ballot(finished);
+  while (done != ~uint64_t(0)) {
+    if (!finished && (deps & ~done) == 0) {
+      T nv[NM];
+#pragma unroll
+      for (int c = 0; c < NM; c += 4) {
+        if (c < nn_w) {
+#pragma unroll
+          for (int u = c; u < c + 4; ++u) nv[u] = s_new[nidx[u]];
+        }
+      }
+      T a = acc;
+#pragma unroll
+      for (int c = 0; c < NM; c += 4) {
+        if (c < nn_w) {
+#pragma unroll
+          for (int u = c; u < c + 4; ++u) a = fma(-T(nval[u]), nv[u], a);
+        }
+      }
+      if constexpr (FWD && ULTRA) {
+        if (has_diag) a = fma(-d, xi, a);
+      }
+#pragma unroll
+      for (int c = 0; c < TM; c += 4) {
+        if (c < nt_w) {
+#pragma unroll
+          for (int u = c; u < c + 4; ++u) a = fma(-T(tval[u]), told[u], a);
+        }
+      }
+      T xn;
+      if (ULTRA) {
+        const T ad = d < T(0) ? -d : d;
+        xn = xi + (omega * a) / (ad + T(0.5) * div);
+      } else {
+        const T quot = a / d;
+        xn = fma(omega, quot, m1w * xi);
+      }
+      s_new[lane] = xn;
+      xout[i] = xn;
+      finished = true;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    done = __ballot(finished);
+  }
+  return true;
+}
+
 template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, bool STAGED>
 __global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
     int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
@@ -275,6 +439,11 @@ __global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
     }
   }
   __syncthreads();
+  if constexpr (STAGED) {
+    if (wave_segment_registers<T, V, FWD, ULTRA, HAS_X>(st, s_new, s0, s1, e0, rp, xin, xout, b,
+                                                        omega))
+      return;
+  }
   auto entry = [&](int64_t k) {
     if constexpr (STAGED) {
       return st[k];

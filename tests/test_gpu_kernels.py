@@ -144,6 +144,55 @@ def test_csr_kernels(dims, dtype):
                     same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.1), forward, ultra, bounds))
 
 
+def long_row_csr(n, seed, long_rows):
+    """Random diagonally dominant CSR with sorted columns: short rows (at most 8
+    neighbours, mostly near the diagonal) and, at the listed rows, rows of up to
+    60 entries with more than 16 new-value entries inside their 64-row segment
+    and more than 24 trailing ones."""
+    rng = np.random.default_rng(seed)
+    rp, cols, vals = [0], [], []
+    for i in range(n):
+        if i in long_rows:
+            s0 = (i // 64) * 64
+            before = np.arange(s0, i)[-20:]
+            after = rng.choice(np.arange(i + 1, n), size=min(30, n - i - 1), replace=False) if i + 1 < n else []
+            c = np.union1d(np.union1d(before, after), [i])
+        else:
+            near = i + rng.integers(-5, 6, size=6)
+            far = rng.integers(0, n, size=2)
+            c = np.union1d(np.clip(np.concatenate([near, far]), 0, n - 1), [i])
+        v = rng.uniform(-1.0, 1.0, size=c.size)
+        v[c == i] = np.abs(v).sum() + 1.0
+        cols.extend(c.tolist())
+        vals.extend(v.tolist())
+        rp.append(len(cols))
+    return np.array(rp, np.int64), np.array(cols, np.int64), np.array(vals)
+
+
+@pytest.mark.parametrize("long_rows", [(), (70, 130), (3, 64, 100, 190, 250)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_csr_segment_sor_register_and_fallback_paths(long_rows, dtype):
+    """64-row segment SOR: waves whose rows fit the register lists take the
+    register path, waves with a long row the LDS path; both equal the oracle
+    bit for bit, forward and backward, regular and ultra, with and without
+    x_in."""
+    n = 260
+    rp, cols, vals = long_row_csr(n, 17 + len(long_rows), set(long_rows))
+    if dtype == np.float32:
+        vals = vals.astype(np.float32).astype(np.float64)
+    C = O.Csr(rp, cols, vals, symmetric=False, is_f32=dtype == np.float32)
+    G = la.CsrMatrix.from_arrays(rp, cols, vals, dtype, symmetric=False)
+    x = problems.uniform(n, 4, dtype)
+    b = problems.uniform(n, 5, dtype)
+    bounds = list(range(0, n, 64)) + [n]
+    for forward in (True, False):
+        for ultra in (False, True):
+            for with_x in (True, False):
+                out = G.sor(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype),
+                            la.DenseVector(b), 1.1, forward, ultra, segment_rows=64)
+                same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.1), forward, ultra, bounds))
+
+
 @pytest.mark.parametrize("dims", [(13, 17, 5), (33, 17, 65), (6, 21, 130)])
 @pytest.mark.parametrize("kind", ["values", "lookup"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
