@@ -118,16 +118,21 @@ struct CsrDev {
   bool symmetric = true;
   DevBuf rowptr, cols, vals, sched, flags;
   int epoch = 0;
-  int64_t seg_entries = 0;
+  int64_t seg_entries = 0, blk_entries = 0;
   CsrDesc desc() const {
-    return CsrDesc{n, rowptr.as<int64_t>(), cols.as<int32_t>(), vals.ptr, f32 ? 1 : 0, seg_entries};
+    return CsrDesc{n,    rowptr.as<int64_t>(), cols.as<int32_t>(), vals.ptr, f32 ? 1 : 0,
+                   seg_entries, blk_entries};
   }
   void upload(const setup::RowMatrix& m, bool as_f32) {
     n = m.n;
-    seg_entries = 0;
-    for (int64_t s0 = 0; s0 < n; s0 += kMaxSegmentRows)
-      seg_entries = std::max(seg_entries, m.rowptr[size_t(std::min(s0 + kMaxSegmentRows, n))] -
-                                              m.rowptr[size_t(s0)]);
+    auto most = [&](int64_t rows) {
+      int64_t e = 0;
+      for (int64_t s0 = 0; s0 < n; s0 += rows)
+        e = std::max(e, m.rowptr[size_t(std::min(s0 + rows, n))] - m.rowptr[size_t(s0)]);
+      return e;
+    };
+    seg_entries = most(kMaxSegmentRows);
+    blk_entries = most(kCsrBlockRows);
     f32 = as_f32;
     symmetric = m.symmetric;
     rowptr.alloc(size_t(n + 1) * sizeof(int64_t));

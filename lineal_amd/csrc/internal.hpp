@@ -110,7 +110,10 @@ struct CsrDesc {
   const void* vals;    // f64 or f32
   int f32;
   int64_t seg_entries;  // most entries in any 64-row segment (rows 64s .. 64s+63)
+  int64_t blk_entries;  // most entries in any block of kCsrBlockRows rows
 };
+// rows per block of the CSR passes
+constexpr int kCsrBlockRows = 256;
 
 // ---------------------------------------------------------------------------
 // kernel launchers (stream ordered)

@@ -78,6 +78,77 @@ __global__ __launch_bounds__(NT) void csr_kernel(int64_t n, const int64_t* __res
   }
 }
 
+// The same passes with the block's entries staged: the entry range of the
+// block's rows is read with coalesced loads into LDS, together with x at each
+// entry's column, and every thread then runs its row from LDS.  One thread
+// per row strides through memory by the row length, so its loads touch many
+// more cache lines per instruction than they use.  The row arithmetic is
+// csr_kernel's, in the same order.
+template <typename T, typename V, int MODE>
+__host__ __device__ constexpr size_t staged_entry_bytes() {
+  return sizeof(T) + sizeof(V) + (MODE == SM_JACOBI ? sizeof(int32_t) : 0);
+}
+
+template <typename T, typename V, int MODE>
+__global__ __launch_bounds__(NT) void csr_staged_kernel(int64_t n, const int64_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ ci,
+                                                        const V* __restrict__ va,
+                                                        const T* __restrict__ x,
+                                                        const T* __restrict__ b,
+                                                        T* __restrict__ y, T omega, int64_t cap,
+                                                        double* __restrict__ partials) {
+  extern __shared__ double staged_lds[];
+  T* const sx = reinterpret_cast<T*>(staged_lds);
+  V* const sv = reinterpret_cast<V*>(sx + cap);
+  int32_t* const sc = reinterpret_cast<int32_t*>(sv + cap);
+  const int64_t r0 = blockIdx.x * int64_t(NT);
+  const int64_t r1 = min(r0 + NT, n);
+  const int64_t e0 = rp[r0], e1 = rp[r1];
+  for (int64_t k = e0 + threadIdx.x; k < e1; k += NT) {
+    const int32_t j = ci[k];
+    sv[k - e0] = va[k];
+    sx[k - e0] = x ? x[j] : T(0);
+    if constexpr (MODE == SM_JACOBI) sc[k - e0] = j;
+  }
+  __syncthreads();
+  const int64_t i = r0 + threadIdx.x;
+  double part = 0.0;
+  if (i < n) {
+    const int64_t k0 = rp[i] - e0, k1 = rp[i + 1] - e0;
+    if constexpr (MODE == SM_JACOBI) {
+      // jacobi/sweep.hpp:66-87
+      T acc = b[i], d = T(1);
+      for (int64_t k = k0; k < k1; ++k) {
+        const T aij = T(sv[k]);
+        if (sc[k] == i) d = aij;
+        else acc = fma(-aij, sx[k], acc);
+      }
+      const T quot = acc / d;
+      y[i] = fma(omega, quot, (T(1) - omega) * (x ? x[i] : T(0)));
+    } else {
+      T sum = T(0);
+#pragma unroll 4
+      for (int64_t k = k0; k < k1; ++k) sum = fma(T(sv[k]), sx[k], sum);
+      if constexpr (MODE == SM_SPMV) {
+        y[i] = sum;
+      } else if constexpr (MODE == SM_SPMV_DOT) {
+        y[i] = sum;
+        part = double(x[i]) * double(sum);
+      } else if constexpr (MODE == SM_RESID) {
+        y[i] = b[i] - sum;
+      } else {
+        const T r = b[i] - sum;
+        y[i] = r;
+        part = double(r) * double(r);
+      }
+    }
+  }
+  if constexpr (MODE == SM_SPMV_DOT || MODE == SM_RESID_NORM) {
+    const double t = block_sum<NT>(part);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  }
+}
+
 // gauss-seidel-variant/sweep.hpp:134-200 with one segment covering all rows
 template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
 __global__ __launch_bounds__(NT) void csr_sor_kernel(int64_t n, const int64_t* __restrict__ rp,
@@ -632,10 +703,20 @@ void dispatch_segment(const CsrDesc& a, const void* x_in, void* x_out, const voi
 template <typename T, typename V, int MODE>
 void launch_apply(const CsrDesc& a, const void* x, const void* b, void* y, double omega,
                   double* partials, hipStream_t st) {
+  static_assert(NT == kCsrBlockRows, "blk_entries counts blocks of the pass");
   const int64_t blocks = (a.n + NT - 1) / NT;
-  hipLaunchKernelGGL((csr_kernel<T, V, MODE>), dim3(blocks), dim3(NT), 0, st, a.n, a.rowptr,
-                     a.cols, static_cast<const V*>(a.vals), static_cast<const T*>(x),
-                     static_cast<const T*>(b), static_cast<T*>(y), T(omega), partials);
+  // an even capacity keeps the LDS arrays (T, then V, then i32) aligned
+  const int64_t cap = (a.blk_entries + 1) & ~int64_t(1);
+  const size_t lds = size_t(cap) * staged_entry_bytes<T, V, MODE>();
+  if (a.blk_entries > 0 && lds <= wave_lds_budget()) {
+    hipLaunchKernelGGL((csr_staged_kernel<T, V, MODE>), dim3(blocks), dim3(NT), lds, st, a.n,
+                       a.rowptr, a.cols, static_cast<const V*>(a.vals), static_cast<const T*>(x),
+                       static_cast<const T*>(b), static_cast<T*>(y), T(omega), cap, partials);
+  } else {
+    hipLaunchKernelGGL((csr_kernel<T, V, MODE>), dim3(blocks), dim3(NT), 0, st, a.n, a.rowptr,
+                       a.cols, static_cast<const V*>(a.vals), static_cast<const T*>(x),
+                       static_cast<const T*>(b), static_cast<T*>(y), T(omega), partials);
+  }
   LGPU_HIP_CHECK(hipGetLastError());
 }
 
