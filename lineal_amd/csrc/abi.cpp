@@ -1197,16 +1197,32 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
     double* part = ctx->partials_for(np);
     op_apply(op, SM_SPMV_DOT, false, p, nullptr, q, 0.0, part);
     reduce_partials(part, np, scal + SC_PQ, st);
+    // FP32 V-cycle: the update also rounds r into the cycle's input, and the
+    // dot and the direction update read the cycle's f32 output directly (it
+    // widens exactly), so no conversion pass runs in between
+    const bool mixed = precon && precon->f32;
+    float* r32 = mixed ? static_cast<float*>(precon->r32.ptr) : nullptr;
+    bool r32_ready = false;
     if (rere(it, params->rere_every)) {
       axpy_scal(xv, p, scal, cur, SC_PQ, 1.0, n, st);
       residual_norm(op, bv, xv, r, SC_RR);
     } else {
       const int64_t nd = dot_partials(n);
       double* pd = ctx->partials_for(nd);
-      cg_update(xv, r, p, q, scal, cur, SC_PQ, n, pd, st);
+      cg_update(xv, r, r32, p, q, scal, cur, SC_PQ, n, pd, st);
       reduce_partials(pd, nd, scal + SC_RR, st);
+      r32_ready = mixed;
     }
-    if (precon) {
+    if (mixed) {
+      if (!r32_ready) vec_convert(r, false, r32, true, n, st);
+      float* z32 = static_cast<float*>(precon->z32.ptr);
+      vcycle(precon, r32, z32, op);
+      const int64_t nd = dot_partials(n);
+      double* pd = ctx->partials_for(nd);
+      vec_dot_mixed(r, z32, n, pd, st);
+      reduce_partials(pd, nd, scal + nxt, st);
+      xpby_scal(p, z32, scal, nxt, cur, n, st);
+    } else if (precon) {
       precondition(precon, r, q, op);
       dot_into(ctx, r, q, false, n, nxt);
       xpby_scal(p, q, scal, nxt, cur, n, st);

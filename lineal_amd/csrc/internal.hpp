@@ -211,15 +211,20 @@ enum : int {
   SC_EFFDIFF = 10, // mid-plane filled count / flux sum
   SC_COUNT = 16,
 };
-// x += (num/den) p, r -= (num/den) q, partial += r^2
-void cg_update(double* x, double* r, const double* p, const double* q, const double* scal,
-               int num, int den, int64_t n, double* partials, hipStream_t st);
+// x += (num/den) p, r -= (num/den) q, partial += r^2; r32 (may be NULL)
+// receives r rounded to f32
+void cg_update(double* x, double* r, float* r32, const double* p, const double* q,
+               const double* scal, int num, int den, int64_t n, double* partials, hipStream_t st);
 // x += (num/den) p only
 void axpy_scal(double* x, const double* p, const double* scal, int num, int den, double sign,
                int64_t n, hipStream_t st);
-// p = z + (num/den) p
+// p = z + (num/den) p, z in f64 or f32 (widened exactly)
 void xpby_scal(double* p, const double* z, const double* scal, int num, int den, int64_t n,
                hipStream_t st);
+void xpby_scal(double* p, const float* z, const double* scal, int num, int den, int64_t n,
+               hipStream_t st);
+// partials of <a, b> for an f32 b, equal to the f64 dot with b widened
+void vec_dot_mixed(const double* a, const float* b, int64_t n, double* partials, hipStream_t st);
 // BiCGStab: p = r + ((rho1/rho0)*(alpha/omega)) (p - omega v)
 void bicg_p_update(double* p, const double* r, const double* v, const double* scal, int rho_new,
                    int rho_old, int64_t n, hipStream_t st);

@@ -35,6 +35,17 @@ __global__ __launch_bounds__(NT) void dot_kernel(const T* a, const T* b, int64_t
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// <a, b> with an f32 b: b widens exactly, so the result is the f64 dot of a
+// with b's f64 copy, bit for bit (same grid, same order)
+__global__ __launch_bounds__(NT) void dot_mixed_kernel(const double* a, const float* b, int64_t n,
+                                                       double* part) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT)
+    s += double(a[i]) * double(b[i]);
+  const double t = block_sum<NT>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 __global__ __launch_bounds__(NT) void dot2_kernel(const double* a, const double* b, int64_t n,
                                                   double* part) {
   double s0 = 0, s1 = 0;
@@ -70,16 +81,17 @@ __global__ __launch_bounds__(NT) void reduce_kernel(const double* part, int64_t 
 
 // CG: alpha = scal[num] / scal[den]; x += alpha p; r -= alpha q; partial += r^2
 // (conjugate-gradients/solver.hpp:84-95: sol + dir*alpha, res - prod*alpha)
-__global__ __launch_bounds__(NT) void cg_update_kernel(double* x, double* r, const double* p,
-                                                       const double* q, const double* scal,
-                                                       int num, int den, int64_t n,
-                                                       double* part) {
+__global__ __launch_bounds__(NT) void cg_update_kernel(double* x, double* r, float* r32,
+                                                       const double* p, const double* q,
+                                                       const double* scal, int num, int den,
+                                                       int64_t n, double* part) {
   const double alpha = scal[num] / scal[den];
   double s = 0;
   for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT) {
     x[i] = x[i] + p[i] * alpha;
     const double ri = r[i] - q[i] * alpha;
     r[i] = ri;
+    if (r32) r32[i] = float(ri);  // the FP32 V-cycle's input, as vec_convert rounds it
     s += ri * ri;
   }
   const double t = block_sum<NT>(s);
@@ -96,11 +108,12 @@ __global__ void axpy_scal_kernel(double* x, const double* p, const double* scal,
 }
 
 // p = z + p * beta, beta = scal[num] / scal[den] (conjugate-gradients/solver.hpp:106-107)
-__global__ void xpby_scal_kernel(double* p, const double* z, const double* scal, int num, int den,
+template <typename Z>
+__global__ void xpby_scal_kernel(double* p, const Z* z, const double* scal, int num, int den,
                                  int64_t n) {
   const double beta = scal[num] / scal[den];
   for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT)
-    p[i] = z[i] + p[i] * beta;
+    p[i] = double(z[i]) + p[i] * beta;
 }
 
 // bicgstab/solver.hpp:145-146: beta = (rho/old_rho) * (alpha/omega); p = r + (p - v*omega)*beta
@@ -266,9 +279,9 @@ void reduce_partials2(const double* partials, int64_t np, double* out0, double* 
   LGPU_HIP_CHECK(hipGetLastError());
 }
 
-void cg_update(double* x, double* r, const double* p, const double* q, const double* scal, int num,
-               int den, int64_t n, double* partials, hipStream_t st) {
-  hipLaunchKernelGGL(cg_update_kernel, dim3(grid_for(n)), dim3(NT), 0, st, x, r, p, q, scal, num, den, n, partials);
+void cg_update(double* x, double* r, float* r32, const double* p, const double* q,
+               const double* scal, int num, int den, int64_t n, double* partials, hipStream_t st) {
+  hipLaunchKernelGGL(cg_update_kernel, dim3(grid_for(n)), dim3(NT), 0, st, x, r, r32, p, q, scal, num, den, n, partials);
   LGPU_HIP_CHECK(hipGetLastError());
 }
 
@@ -280,7 +293,18 @@ void axpy_scal(double* x, const double* p, const double* scal, int num, int den,
 
 void xpby_scal(double* p, const double* z, const double* scal, int num, int den, int64_t n,
                hipStream_t st) {
-  hipLaunchKernelGGL(xpby_scal_kernel, dim3(grid_for(n)), dim3(NT), 0, st, p, z, scal, num, den, n);
+  hipLaunchKernelGGL(xpby_scal_kernel<double>, dim3(grid_for(n)), dim3(NT), 0, st, p, z, scal, num, den, n);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void xpby_scal(double* p, const float* z, const double* scal, int num, int den, int64_t n,
+               hipStream_t st) {
+  hipLaunchKernelGGL(xpby_scal_kernel<float>, dim3(grid_for(n)), dim3(NT), 0, st, p, z, scal, num, den, n);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+void vec_dot_mixed(const double* a, const float* b, int64_t n, double* partials, hipStream_t st) {
+  hipLaunchKernelGGL(dot_mixed_kernel, dim3(grid_for(n)), dim3(NT), 0, st, a, b, n, partials);
   LGPU_HIP_CHECK(hipGetLastError());
 }
 
