@@ -129,9 +129,34 @@ def main():
         r = solve(M32)
         log(f"warmup mixed: {r.iterations} it, status {r.status}")
     ctx.synchronize()
-    # every launch of the hot kernel classes is bracketed by HIP events on the
-    # library's stream during the timed solves
+    all_classes = (("stencil_sor_fwd", la.KT_STENCIL_SOR_FWD), ("stencil_sor_bwd", la.KT_STENCIL_SOR_BWD),
+                   ("stencil_pass", la.KT_STENCIL_PASS), ("csr_sor", la.KT_CSR_SOR),
+                   ("csr_pass", la.KT_CSR_PASS))
+
+    def class_times(names, solves):
+        out = {}
+        for name, cls in all_classes:
+            if name in names:
+                ms, cnt = ctx.kernel_time(cls)
+                out[name] = {"ms_total_per_solve": ms / solves, "launches_per_solve": cnt / solves,
+                             "ms_per_launch": ms / max(cnt, 1)}
+        return out
+
+    # time split of one untimed solve with every kernel class bracketed by HIP
+    # events (each event pair adds a few microseconds between kernels, so the
+    # timed solves below bracket only the finest-level sweeps)
     ctx.kernel_timing(True)
+    solve(M32)
+    ctx.synchronize()
+    split = class_times([k for k, _ in all_classes], 1)
+    ctx.kernel_timing(False)
+    for k, v in split.items():
+        log(f"time split {k}: {v['ms_total_per_solve']:.1f} ms/solve over {v['launches_per_solve']:.0f} "
+            f"launches, {v['ms_per_launch'] * 1e3:.1f} us each")
+    # timed region: every launch of the finest-level SOR sweeps (the dominant
+    # kernel class) is bracketed by HIP events on the library's stream
+    sor_names = ("stencil_sor_fwd", "stencil_sor_bwd")
+    ctx.kernel_timing((1 << la.KT_STENCIL_SOR_FWD) | (1 << la.KT_STENCIL_SOR_BWD))
     t0 = time.perf_counter()
     its32 = []
     for _ in range(args.steps):
@@ -140,13 +165,7 @@ def main():
         assert r.converged, r
     ctx.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
-    classes = {}
-    for name, cls in (("stencil_sor_fwd", la.KT_STENCIL_SOR_FWD), ("stencil_sor_bwd", la.KT_STENCIL_SOR_BWD),
-                      ("stencil_pass", la.KT_STENCIL_PASS), ("csr_sor", la.KT_CSR_SOR),
-                      ("csr_pass", la.KT_CSR_PASS)):
-        ms, cnt = ctx.kernel_time(cls)
-        classes[name] = {"ms_total_per_solve": ms / args.steps, "launches_per_solve": cnt / args.steps,
-                         "ms_per_launch": ms / max(cnt, 1)}
+    classes = class_times(sor_names, args.steps)
     ctx.kernel_timing(False)
     for k, v in classes.items():
         log(f"timed region {k}: {v['ms_total_per_solve']:.1f} ms/solve over {v['launches_per_solve']:.0f} "
@@ -235,6 +254,7 @@ def main():
             "levels": [M32.level_size(l) for l in range(M32.level_num)],
             "kernels": kern,
             "timed_region_kernel_classes": classes,
+            "time_split_untimed_solve": split,
             "spmv_roofline": {"bound": "hbm", "achieved": spmv_bytes / t_spmv / 1e6, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": spmv_bytes / t_spmv / 1e6 / HBM_PEAK_GBS},
         },

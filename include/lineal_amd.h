@@ -54,18 +54,21 @@ int lgpu_ctx_synchronize(lgpu_ctx ctx);
 int lgpu_timer_start(lgpu_ctx ctx);
 int lgpu_timer_stop(lgpu_ctx ctx, double* ms);
 /* Kernel-class timing (no counterpart in the reference; used by the
- * benchmark to time the dominant kernel inside a timed solve).  While
- * enabled, every launch of a class is bracketed by HIP events on the
- * context's stream; enabling or disabling clears the records. */
+ * benchmark to time the dominant kernel inside a timed solve).  Every launch
+ * of a class whose bit (1 << class) is set in the mask is bracketed by HIP
+ * events on the context's stream; each event pair can add a few microseconds
+ * between kernels, so time only the classes needed.  Setting a mask (0
+ * disables) clears the records. */
 enum {
   LGPU_KT_STENCIL_SOR_FWD = 0, /* forward SOR sweeps on a stencil operator */
   LGPU_KT_STENCIL_SOR_BWD = 1, /* backward SOR sweeps on a stencil operator */
   LGPU_KT_STENCIL_PASS = 2,    /* stencil SpMV / residual / Jacobi passes */
   LGPU_KT_CSR_SOR = 3,         /* CSR SOR sweeps (coarse levels included) */
   LGPU_KT_CSR_PASS = 4,        /* CSR SpMV / residual passes */
-  LGPU_KT_COUNT = 5
+  LGPU_KT_COUNT = 5,
+  LGPU_KT_ALL = (1 << 5) - 1
 };
-int lgpu_ctx_kernel_timing(lgpu_ctx ctx, int enable);
+int lgpu_ctx_kernel_timing(lgpu_ctx ctx, int class_mask);
 /* total milliseconds and launch count of a class since timing was enabled */
 int lgpu_ctx_kernel_time(lgpu_ctx ctx, int kernel_class, double* total_ms, int64_t* launches);
 

@@ -41,6 +41,7 @@ F64, F32 = 0, 1
 OK, INVALID_ARG, OUT_OF_MEMORY, HIP_ERROR, BREAKDOWN, NONFINITE, NOT_CONVERGED, NOT_SUPPORTED, SYNC_TIMEOUT = range(9)
 # kernel classes of Context.kernel_time
 KT_STENCIL_SOR_FWD, KT_STENCIL_SOR_BWD, KT_STENCIL_PASS, KT_CSR_SOR, KT_CSR_PASS = range(5)
+KT_ALL = (1 << 5) - 1
 
 _vp = C.c_void_p
 _i64 = C.c_int64
@@ -203,9 +204,11 @@ class Context:
         _check(lib().lgpu_timer_stop(self.h, C.byref(ms)))
         return ms.value
 
-    def kernel_timing(self, enable: bool):
-        """Bracket every launch of each kernel class with HIP events (clears records)."""
-        _check(lib().lgpu_ctx_kernel_timing(self.h, int(bool(enable))))
+    def kernel_timing(self, enable):
+        """Bracket every launch of the chosen kernel classes with HIP events
+        (True: all, False: none, or a mask of 1 << KT_* bits); clears records."""
+        mask = (KT_ALL if enable else 0) if isinstance(enable, bool) else int(enable)
+        _check(lib().lgpu_ctx_kernel_timing(self.h, mask))
 
     def kernel_time(self, kernel_class: int):
         """(total milliseconds, launches) of a kernel class since timing was enabled."""

@@ -59,8 +59,8 @@ struct lgpu_ctx_s {
   DevBuf scal;            // SC_COUNT device scalars
   double* host_scal = nullptr;  // pinned mirror
   // opt-in timing of kernel classes (lgpu_ctx_kernel_timing): start/stop
-  // event pairs recorded around every launch of the class
-  bool ktime_on = false;
+  // event pairs recorded around every launch of a class whose bit is set
+  int ktime_mask = 0;
   std::vector<hipEvent_t> kev[LGPU_KT_COUNT];
   size_t kused[LGPU_KT_COUNT] = {};
   double* partials_for(int64_t need) {
@@ -81,7 +81,7 @@ struct KTimed {
   lgpu_ctx ctx;
   hipEvent_t stop = nullptr;
   KTimed(lgpu_ctx c, int cls) : ctx(c) {
-    if (!ctx->ktime_on) return;
+    if (!((ctx->ktime_mask >> cls) & 1)) return;
     auto& v = ctx->kev[cls];
     size_t& u = ctx->kused[cls];
     if (u + 2 > v.size()) {
@@ -558,12 +558,13 @@ extern "C" int lgpu_timer_stop(lgpu_ctx ctx, double* ms) {
   API_END
 }
 
-extern "C" int lgpu_ctx_kernel_timing(lgpu_ctx ctx, int enable) {
+extern "C" int lgpu_ctx_kernel_timing(lgpu_ctx ctx, int class_mask) {
   API_BEGIN
   LGPU_REQUIRE(ctx, "ctx is NULL");
   LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   for (size_t& u : ctx->kused) u = 0;
-  ctx->ktime_on = enable != 0;
+  LGPU_REQUIRE((class_mask & ~LGPU_KT_ALL) == 0, "unknown kernel class in the mask");
+  ctx->ktime_mask = class_mask;
   API_END
 }
 
