@@ -119,9 +119,10 @@ struct CsrDev {
   DevBuf rowptr, cols, vals, sched, flags;
   int epoch = 0;
   int64_t seg_entries = 0, blk_entries = 0;
+  int seg_new_max = 0, seg_trail_max = 0;
   CsrDesc desc() const {
-    return CsrDesc{n,    rowptr.as<int64_t>(), cols.as<int32_t>(), vals.ptr, f32 ? 1 : 0,
-                   seg_entries, blk_entries};
+    return CsrDesc{n,          rowptr.as<int64_t>(), cols.as<int32_t>(), vals.ptr, f32 ? 1 : 0,
+                   seg_entries, blk_entries,          seg_new_max,       seg_trail_max};
   }
   void upload(const setup::RowMatrix& m, bool as_f32) {
     n = m.n;
@@ -133,6 +134,23 @@ struct CsrDev {
     };
     seg_entries = most(kMaxSegmentRows);
     blk_entries = most(kCsrBlockRows);
+    // per row of a 64-row segment [s0, s1): forward new [s0, i), trailing
+    // (i, n); backward new (i, s1), trailing [s1, n)
+    seg_new_max = seg_trail_max = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t s0 = i / kMaxSegmentRows * kMaxSegmentRows;
+      const int64_t s1 = std::min(s0 + kMaxSegmentRows, n);
+      int fn = 0, ft = 0, bn = 0, bt = 0;
+      for (int64_t k = m.rowptr[size_t(i)]; k < m.rowptr[size_t(i + 1)]; ++k) {
+        const int64_t c = m.cols[size_t(k)];
+        fn += c >= s0 && c < i;
+        ft += c > i;
+        bn += c > i && c < s1;
+        bt += c >= s1;
+      }
+      seg_new_max = std::max({seg_new_max, fn, bn});
+      seg_trail_max = std::max({seg_trail_max, ft, bt});
+    }
     f32 = as_f32;
     symmetric = m.symmetric;
     rowptr.alloc(size_t(n + 1) * sizeof(int64_t));

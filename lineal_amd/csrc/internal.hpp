@@ -111,6 +111,10 @@ struct CsrDesc {
   int f32;
   int64_t seg_entries;  // most entries in any 64-row segment (rows 64s .. 64s+63)
   int64_t blk_entries;  // most entries in any block of kCsrBlockRows rows
+  // 64-row segment sweeps: the longest run of a row's new-value entries (in
+  // its own segment, on the swept side) and of its trailing old-value
+  // entries, over both sweep directions
+  int seg_new_max, seg_trail_max;
 };
 // rows per block of the CSR passes
 constexpr int kCsrBlockRows = 256;

@@ -298,18 +298,39 @@ size_t wave_lds_budget() {
   return budget;
 }
 
-// One entry of a staged segment: column, value and the column's old value,
-// read with one LDS access.
+// One entry of a staged segment: column, value and the column's old value.
 template <typename T, typename V>
-struct alignas(sizeof(V) >= sizeof(T) ? 2 * sizeof(V) : 2 * sizeof(T)) Staged {
+struct Staged {
   int32_t col;
   T old;
   V val;
 };
 
+// The staged entries in LDS as three arrays, so no entry carries padding (a
+// structure would take 16 bytes per f32 entry and 32 per f64 one): LDS is
+// what limits how many segment wavefronts share a CU on the large levels.
+// Layout: values[cap], old values[cap], the segment's new values[64],
+// columns[cap], with cap even so that every array stays aligned.
+template <typename T, typename V>
+struct StageView {
+  V* vals;
+  T* olds;
+  int32_t* cols;
+  __device__ __forceinline__ Staged<T, V> operator[](int64_t k) const {
+    return Staged<T, V>{cols[k], olds[k], vals[k]};
+  }
+  __device__ __forceinline__ void put(int64_t k, const Staged<T, V>& e) const {
+    cols[k] = e.col;
+    olds[k] = e.old;
+    vals[k] = e.val;
+  }
+};
+
+__host__ __device__ constexpr int64_t stage_cap(int64_t entries) { return (entries + 1) & ~int64_t(1); }
+
 template <typename T, typename V>
 size_t wave_lds_bytes(int64_t entries) {
-  return size_t(entries) * sizeof(Staged<T, V>) + WSEG * sizeof(T);
+  return size_t(stage_cap(entries)) * (sizeof(V) + sizeof(T) + sizeof(int32_t)) + WSEG * sizeof(T);
 }
 
 // One wavefront per segment of 64 rows.  The segment's entries (and the old
@@ -332,8 +353,11 @@ size_t wave_lds_bytes(int64_t entries) {
 // LDS reads of the finished rows' values and the FMA chain, in the row's
 // column order, so the results are those of the LDS-walking rounds below.
 // Returns false (nothing written) when some row of the wave has longer lists.
-constexpr int kNewMax = 16;    // multiples of 4
-constexpr int kTrailMax = 24;
+// list sizes (multiples of 4) of the two variants: a level whose rows all fit
+// the short lists runs a kernel with fewer registers, so more wavefronts
+// share a SIMD
+constexpr int kNewShort = 8, kTrailShort = 16;
+constexpr int kNewLong = 16, kTrailLong = 24;
 
 __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll
@@ -341,15 +365,15 @@ __device__ __forceinline__ int wave_max_int(int v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
-template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
-__device__ __forceinline__ bool wave_segment_registers(const Staged<T, V>* st, T* s_new,
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, int NM, int TM>
+__device__ __forceinline__ bool wave_segment_registers(const StageView<T, V>& st, T* s_new,
                                                        int64_t s0, int64_t s1, int64_t e0,
                                                        const int64_t* __restrict__ rp,
                                                        const T* __restrict__ xin,
                                                        T* __restrict__ xout,
                                                        const T* __restrict__ b, T omega) {
   using S = Staged<T, V>;
-  constexpr int NM = kNewMax, TM = kTrailMax;
+  static_assert(NM % 4 == 0 && TM % 4 == 0, "lists run in groups of four");
   const int lane = threadIdx.x;
   const int64_t i = s0 + lane;
   const bool live = i < s1;
@@ -487,15 +511,18 @@ __device__ __forceinline__ bool wave_segment_registers(const Staged<T, V>* st, T
   return true;
 }
 
-template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, bool STAGED>
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, bool STAGED, bool SHORT>
 __global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
     int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
     const V* __restrict__ va, const T* __restrict__ xin, T* __restrict__ xout,
     const T* __restrict__ b, T omega, int64_t cap) {
   using S = Staged<T, V>;
   extern __shared__ double wave_lds[];
-  S* const st = reinterpret_cast<S*>(wave_lds);
-  T* const s_new = reinterpret_cast<T*>(st + (STAGED ? cap : 0));
+  const int64_t scap = STAGED ? stage_cap(cap) : 0;
+  V* const svals = reinterpret_cast<V*>(wave_lds);
+  T* const solds = reinterpret_cast<T*>(svals + scap);
+  T* const s_new = solds + scap;
+  const StageView<T, V> st{svals, solds, reinterpret_cast<int32_t*>(s_new + WSEG)};
   const int lane = threadIdx.x;
   const int64_t s0 = blockIdx.x * int64_t(WSEG);
   const int64_t s1 = min(s0 + WSEG, n);
@@ -506,13 +533,14 @@ __global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
       e.col = ci[k];
       e.val = va[k];
       e.old = HAS_X ? xin[e.col] : T(0);
-      st[k - e0] = e;
+      st.put(k - e0, e);
     }
   }
   __syncthreads();
   if constexpr (STAGED) {
-    if (wave_segment_registers<T, V, FWD, ULTRA, HAS_X>(st, s_new, s0, s1, e0, rp, xin, xout, b,
-                                                        omega))
+    constexpr int NM = SHORT ? kNewShort : kNewLong, TM = SHORT ? kTrailShort : kTrailLong;
+    if (wave_segment_registers<T, V, FWD, ULTRA, HAS_X, NM, TM>(st, s_new, s0, s1, e0, rp, xin,
+                                                                xout, b, omega))
       return;
   }
   auto entry = [&](int64_t k) {
@@ -639,12 +667,17 @@ void launch_wave_segment2(const CsrDesc& a, const void* x_in, void* x_out, const
   const auto bp = static_cast<const T*>(b);
   const auto vp = static_cast<const V*>(a.vals);
   if (staged <= wave_lds_budget()) {
-    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, HAS_X, true>), dim3(nseg),
-                       dim3(WSEG), staged, st, a.n, a.rowptr, a.cols, vp, xi, xo, bp, T(omega),
-                       a.seg_entries);
+    if (a.seg_new_max <= kNewShort && a.seg_trail_max <= kTrailShort)
+      hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, HAS_X, true, true>), dim3(nseg),
+                         dim3(WSEG), staged, st, a.n, a.rowptr, a.cols, vp, xi, xo, bp, T(omega),
+                         a.seg_entries);
+    else
+      hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, HAS_X, true, false>), dim3(nseg),
+                         dim3(WSEG), staged, st, a.n, a.rowptr, a.cols, vp, xi, xo, bp, T(omega),
+                         a.seg_entries);
   } else {
     const size_t direct = wave_lds_bytes<T, V>(0);
-    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, HAS_X, false>), dim3(nseg),
+    hipLaunchKernelGGL((csr_sor_wave_kernel<T, V, FWD, ULTRA, HAS_X, false, false>), dim3(nseg),
                        dim3(WSEG), direct, st, a.n, a.rowptr, a.cols, vp, xi, xo, bp, T(omega),
                        int64_t(0));
   }
