@@ -7,6 +7,34 @@
 
 namespace lgpu {
 
+// DPP whole-wave lane shifts (gfx9 wave_shr:1 / wave_shl:1): lane l receives
+// the value of lane l-1 / l+1; the lane without a source (0 / 63) keeps `old`.
+constexpr int kWaveShr1 = 0x138;
+constexpr int kWaveShl1 = 0x130;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ int shift(int old, int v) { return dpp_i32<CTRL>(old, v); }
+template <int CTRL>
+__device__ __forceinline__ uint8_t shift(uint8_t old, uint8_t v) {
+  return uint8_t(dpp_i32<CTRL>(int(old), int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ float shift(float old, float v) {
+  return __int_as_float(dpp_i32<CTRL>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double shift(double old, double v) {
+  const long long o = __double_as_longlong(old), b = __double_as_longlong(v);
+  const int lo = dpp_i32<CTRL>(static_cast<int>(o), static_cast<int>(b));
+  const int hi = dpp_i32<CTRL>(static_cast<int>(o >> 32), static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
 // Harmonic mean H(a,b) = (ab != 0) ? (ab + ab) / (a + b) : 0,
 // math/harmonic-mean.hpp:14-25.
 template <typename T>

@@ -266,15 +266,16 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
         if (take_y) nwy[k] = mailbox_poll(in_y + lz, state);
       }
       // successor data of the neighbouring lanes (same logical z)
+      // (lane +-1 by DPP whole-wave shifts, lane +-8 by permutes)
       Raw csx = shfl_raw(cw[k + 2], lane + CB < 64 ? lane + CB : lane);
-      Raw csy = shfl_raw(cw[k + 2], lane + 1 < 64 ? lane + 1 : lane);
+      Raw csy = shift<kWaveShl1>(cw[k + 2], cw[k + 2]);
       T xsx = __shfl(xw[k + 1], lane + CB < 64 ? lane + CB : lane, 64);
-      T xsy = __shfl(xw[k + 1], lane + 1 < 64 ? lane + 1 : lane, 64);
+      T xsy = shift<kWaveShl1>(xw[k + 1], xw[k + 1]);
       // predecessor results of the neighbouring lanes (one step ago)
       const T npx = __shfl(prev, lane >= CB ? lane - CB : lane, 64);
-      const T npy = __shfl(prev, lane >= 1 ? lane - 1 : lane, 64);
+      const T npy = shift<kWaveShr1>(prev, prev);
       const T wpx_in = __shfl(wpx_prev, lane >= CB ? lane - CB : lane, 64);
-      const T wpy_in = __shfl(wpy_prev, lane >= 1 ? lane - 1 : lane, 64);
+      const T wpy_in = shift<kWaveShr1>(wpy_prev, wpy_prev);
       if (give_x) {
         csx = hx[k + 1];
         xsx = exx[k];
