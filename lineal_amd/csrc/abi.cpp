@@ -206,6 +206,7 @@ struct AmgLevel {
   CsrDev a;            // level operator (levels >= 1)
   DevBuf agg;          // this level -> next level map
   DevBuf c2f_ptr, c2f_idx;
+  int64_t c2f_blk = 0;  // most list entries of any 256 consecutive coarse rows
   DevBuf rhs, sol, tmp, aux;
   int64_t n = 0;
 };
@@ -460,7 +461,7 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine) {
     }
     const int64_t nc = amg->lv[size_t(l + 1)].n;
     restrict_sum(cur.tmp.ptr, f32, cur.c2f_ptr.as<int64_t>(), cur.c2f_idx.as<int32_t>(), nc,
-                 amg->lv[size_t(l + 1)].rhs.ptr, st);
+                 cur.c2f_blk, amg->lv[size_t(l + 1)].rhs.ptr, st);
   }
   {
     AmgLevel& c = amg->lv[size_t(L - 1)];
@@ -1068,6 +1069,11 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
       LGPU_HIP_CHECK(hipMemcpy(a.c2f_ptr.ptr, t.c2f_ptr.data(), t.c2f_ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
       a.c2f_idx.alloc(t.c2f_idx.size() * sizeof(int32_t));
       LGPU_HIP_CHECK(hipMemcpy(a.c2f_idx.ptr, t.c2f_idx.data(), t.c2f_idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      const int64_t ncl = int64_t(t.c2f_ptr.size()) - 1;
+      a.c2f_blk = 0;
+      for (int64_t c0 = 0; c0 < ncl; c0 += kCsrBlockRows)
+        a.c2f_blk = std::max(a.c2f_blk, t.c2f_ptr[size_t(std::min(c0 + kCsrBlockRows, ncl))] -
+                                            t.c2f_ptr[size_t(c0)]);
       a.tmp.alloc(size_t(a.n) * es);
       a.aux.alloc(size_t(a.n) * es);
     }

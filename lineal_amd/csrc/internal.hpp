@@ -187,8 +187,10 @@ void vec_dot(const void* a, const void* b, bool f32, int64_t n, double* partials
 void reduce_partials(const double* partials, int64_t np, double* out, hipStream_t st);
 
 // AMG transfer: coarse[c] = sum_{k in c2f[c]} fine[idx[k]]
+// blk_entries: most list entries of any 256 consecutive coarse rows (staging
+// size; 0 runs the unstaged kernel)
 void restrict_sum(const void* fine, bool f32, const int64_t* ptr, const int32_t* idx, int64_t nc,
-                  void* coarse, hipStream_t st);
+                  int64_t blk_entries, void* coarse, hipStream_t st);
 // out[v] = x[v] + (agg[v] aggregate ? xc[agg[v]] : 0) * inv_relax
 void prolong_add(const void* x, const void* xc, bool f32, const uint32_t* agg, int64_t n,
                  double inv_relax, void* out, hipStream_t st);

@@ -155,6 +155,29 @@ __global__ void restrict_kernel(const T* fine, const int64_t* ptr, const int32_t
   }
 }
 
+// The same sums with the block's list entries staged: the fine values of the
+// 256 coarse rows' lists are gathered into LDS with coalesced index loads, then
+// each thread adds its list in order.
+template <typename T>
+__global__ __launch_bounds__(NT) void restrict_staged_kernel(const T* fine, const int64_t* ptr,
+                                                             const int32_t* idx, int64_t nc,
+                                                             T* coarse) {
+  extern __shared__ double restrict_lds[];
+  T* const sv = reinterpret_cast<T*>(restrict_lds);
+  const int64_t c0 = blockIdx.x * int64_t(NT);
+  const int64_t c1 = c0 + NT < nc ? c0 + NT : nc;
+  const int64_t e0 = ptr[c0], e1 = ptr[c1];
+  for (int64_t k = e0 + threadIdx.x; k < e1; k += NT) sv[k - e0] = fine[idx[k]];
+  __syncthreads();
+  const int64_t c = c0 + threadIdx.x;
+  if (c < nc) {
+    T s = T(0);
+    const int64_t k1 = ptr[c + 1] - e0;
+    for (int64_t k = ptr[c] - e0; k < k1; ++k) s += sv[k];
+    coarse[c] = s;
+  }
+}
+
 // multigrid/solver.hpp:125-137 with transport/vector-shared.hpp:121-125
 template <typename T>
 __global__ void prolong_kernel(const T* x, const T* xc, const uint32_t* agg, int64_t n,
@@ -322,7 +345,20 @@ void bicg_r_update(double* r, const double* v, double* scal, int num, int den, i
 }
 
 void restrict_sum(const void* fine, bool f32, const int64_t* ptr, const int32_t* idx, int64_t nc,
-                  void* coarse, hipStream_t st) {
+                  int64_t blk_entries, void* coarse, hipStream_t st) {
+  static_assert(NT == kCsrBlockRows, "blk_entries counts blocks of NT coarse rows");
+  const size_t lds = size_t(blk_entries) * (f32 ? sizeof(float) : sizeof(double));
+  if (nc > 0 && blk_entries > 0 && lds <= 48 * 1024) {
+    const dim3 g(unsigned((nc + NT - 1) / NT)), b(NT);
+    if (f32)
+      hipLaunchKernelGGL(restrict_staged_kernel<float>, g, b, lds, st, static_cast<const float*>(fine), ptr, idx, nc,
+                         static_cast<float*>(coarse));
+    else
+      hipLaunchKernelGGL(restrict_staged_kernel<double>, g, b, lds, st, static_cast<const double*>(fine), ptr, idx,
+                         nc, static_cast<double*>(coarse));
+    LGPU_HIP_CHECK(hipGetLastError());
+    return;
+  }
   if (f32) hipLaunchKernelGGL(restrict_kernel<float>, dim3(grid_for(nc)), dim3(NT), 0, st, static_cast<const float*>(fine), ptr, idx, nc, static_cast<float*>(coarse));
   else hipLaunchKernelGGL(restrict_kernel<double>, dim3(grid_for(nc)), dim3(NT), 0, st, static_cast<const double*>(fine), ptr, idx, nc, static_cast<double*>(coarse));
   LGPU_HIP_CHECK(hipGetLastError());
