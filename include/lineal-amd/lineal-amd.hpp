@@ -332,6 +332,62 @@ inline GpuCsrSystem<TReal> csr_from_adjacent_stencil(const GpuStencilMatrixBase&
   return sys;
 }
 
+// Matrix and vector files (io/tensor.hpp:25-263).  SmfReader builds the CSR
+// matrix as the reference does (nonsymmetric unless declared symmetric).
+template <typename TReal = double>
+class SmfReader {
+ public:
+  explicit SmfReader(std::string path) : path_(std::move(path)) {
+    check(lgpu_smf_info(path_.c_str(), &rows_, &cols_, &nnz_));
+  }
+  [[nodiscard]] std::uint64_t row_num() const { return std::uint64_t(rows_); }
+  [[nodiscard]] std::uint64_t col_num() const { return std::uint64_t(cols_); }
+  [[nodiscard]] std::uint64_t nnz_num() const { return std::uint64_t(nnz_); }
+  [[nodiscard]] GpuCsrMatrix<TReal> read(const GpuContext& ctx, bool symmetric = false) const {
+    constexpr int dtype = std::is_same_v<TReal, double> ? LGPU_F64 : LGPU_F32;
+    lgpu_op h = nullptr;
+    check(lgpu_smf_read(ctx.handle(), path_.c_str(), dtype, symmetric ? 1 : 0, &h));
+    return GpuCsrMatrix<TReal>(h, symmetric, ctx);
+  }
+
+ private:
+  std::string path_;
+  int64_t rows_ = 0, cols_ = 0, nnz_ = 0;
+};
+
+struct SmfWriter {
+  explicit SmfWriter(std::string path) : path(std::move(path)) {}
+  void write(const GpuMatrixBase& mat) const { check(lgpu_smf_write(mat.handle(), path.c_str())); }
+  std::string path;
+};
+
+template <typename TReal = double>
+class DvfReader {
+ public:
+  explicit DvfReader(std::string path) : path_(std::move(path)) {
+    check(lgpu_dvf_size(path_.c_str(), &size_));
+  }
+  [[nodiscard]] std::uint64_t size() const { return std::uint64_t(size_); }
+  [[nodiscard]] GpuDenseVector<TReal> read(const GpuContext& ctx) const {
+    lgpu_vec h = nullptr;
+    check(lgpu_dvf_read(ctx.handle(), path_.c_str(), GpuDenseVector<TReal>::dtype, &h));
+    return GpuDenseVector<TReal>::adopt(h, ctx);
+  }
+
+ private:
+  std::string path_;
+  int64_t size_ = 0;
+};
+
+struct DvfWriter {
+  explicit DvfWriter(std::string path) : path(std::move(path)) {}
+  template <typename T>
+  void write(const GpuDenseVector<T>& vec) const {
+    check(lgpu_dvf_write(vec.handle(), path.c_str()));
+  }
+  std::string path;
+};
+
 // Effective diffusivity through the mid plane x = nx / 2, EffDiffCalc
 // (component-wise/sink/effective-diffusion.hpp:229-327).  The area part is
 // computed at construction (:292-295); none_materials flags the "none"

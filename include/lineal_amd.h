@@ -225,6 +225,32 @@ int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
 int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
                         const lgpu_krylov_params* params, int* iterations, double* res_hist);
 
+/* ---- matrix and vector files (io/tensor.hpp:25-263) -----------------------
+ * SMF: u64 rows, u64 columns, u64 nonzeros, u64 row offsets[rows + 1], then
+ * (u64 column, f64 value) pairs row by row.  DVF: u64 size, f64 values.
+ * Little-endian.  Paths are host file names. */
+
+/* SmfReader::row_num / col_num / nnz_num (io/tensor.hpp:35-46) */
+int lgpu_smf_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz);
+/* SmfReader::read (io/tensor.hpp:47-75): a CSR operator, duplicate columns of a
+ * row summed in f64 in file order and the row sorted by column (the row-wise
+ * builder, tensor/dynamic/csr/shared.hpp:730-764), values stored in `dtype`.
+ * The reference builds the matrix as nonsymmetric; symmetric != 0 declares it
+ * symmetric instead (an AMG hierarchy then uses the symmetric strength
+ * criterion and a Cholesky coarsest solve).  Needs a square matrix that
+ * stores every diagonal entry. */
+int lgpu_smf_read(lgpu_ctx ctx, const char* path, int dtype, int symmetric, lgpu_op* out);
+/* SmfWriter::write (io/tensor.hpp:158-213) of a CSR or stencil operator: the
+ * nonzero entries in column order; the header's nonzero count is the stored
+ * entry count for CSR operators (CsrMatrix::non_zero_num) and the written
+ * count for stencils. */
+int lgpu_smf_write(lgpu_op op, const char* path);
+/* DvfReader::size / read (io/tensor.hpp:215-233): a new vector in `dtype` */
+int lgpu_dvf_size(const char* path, int64_t* n);
+int lgpu_dvf_read(lgpu_ctx ctx, const char* path, int dtype, lgpu_vec* out);
+/* DvfWriter::write (io/tensor.hpp:253-263): values written as f64 */
+int lgpu_dvf_write(lgpu_vec v, const char* path);
+
 /* ---- effective diffusivity (component-wise/sink/effective-diffusion.hpp) ----
  * Both need a stencil operator with n[0] >= 3: the mid plane x = n[0] / 2 and
  * the plane after it must exist. */

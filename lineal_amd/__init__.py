@@ -112,6 +112,12 @@ SIGNATURES = {
     "lgpu_csr_create": (C.c_int, [_vp, _i64, _i64p, _i32p, _f64p, C.c_int, C.c_int, C.POINTER(_vp)]),
     "lgpu_csr_from_stencil": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "lgpu_csr_system_from_stencil": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_vp), C.POINTER(_vp), _i64p]),
+    "lgpu_smf_info": (C.c_int, [C.c_char_p, _i64p, _i64p, _i64p]),
+    "lgpu_smf_read": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "lgpu_smf_write": (C.c_int, [_vp, C.c_char_p]),
+    "lgpu_dvf_size": (C.c_int, [C.c_char_p, _i64p]),
+    "lgpu_dvf_read": (C.c_int, [_vp, C.c_char_p, C.c_int, C.POINTER(_vp)]),
+    "lgpu_dvf_write": (C.c_int, [_vp, C.c_char_p]),
     "lgpu_eff_diff_area_part": (C.c_int, [_vp, _u8p, _f64p]),
     "lgpu_eff_diff": (C.c_int, [_vp, _vp, _vp, C.c_double, _f64p]),
     "lgpu_op_destroy": (C.c_int, [_vp]),
@@ -641,6 +647,72 @@ def run_iterative_solver(solver: _Krylov, lhs, sol, rhs, rel_tol=1e-8, max_it=10
 
 # ---------------------------------------------------------------------------
 # adjoint of the solve (not in the reference)
+
+
+class SmfReader:
+    """SmfReader (io/tensor.hpp:26-156): the header on construction, the CSR
+    matrix on read().  The reference builds it as nonsymmetric; symmetric=True
+    declares symmetry."""
+
+    def __init__(self, path):
+        self.path = os.fspath(path)
+        r, c, z = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().lgpu_smf_info(self.path.encode(), C.byref(r), C.byref(c), C.byref(z)))
+        self._dims = (r.value, c.value, z.value)
+
+    def row_num(self) -> int:
+        return self._dims[0]
+
+    def col_num(self) -> int:
+        return self._dims[1]
+
+    def nnz_num(self) -> int:
+        return self._dims[2]
+
+    def read(self, dtype=np.float64, symmetric=False, ctx: Context | None = None) -> "CsrMatrix":
+        ctx = ctx or default_context()
+        h = _vp()
+        _check(lib().lgpu_smf_read(ctx.h, self.path.encode(), _dtype_code(dtype), int(symmetric), C.byref(h)))
+        return CsrMatrix(h, ctx)
+
+
+class SmfWriter:
+    """SmfWriter (io/tensor.hpp:158-213) of a CSR or stencil matrix."""
+
+    def __init__(self, path):
+        self.path = os.fspath(path)
+
+    def write(self, matrix: _Matrix):
+        _check(lib().lgpu_smf_write(matrix.h, self.path.encode()))
+
+
+class DvfReader:
+    """DvfReader (io/tensor.hpp:215-251)."""
+
+    def __init__(self, path):
+        self.path = os.fspath(path)
+        n = C.c_int64()
+        _check(lib().lgpu_dvf_size(self.path.encode(), C.byref(n)))
+        self._size = n.value
+
+    def size(self) -> int:
+        return self._size
+
+    def read(self, dtype=np.float64, ctx: Context | None = None) -> DenseVector:
+        ctx = ctx or default_context()
+        h = _vp()
+        _check(lib().lgpu_dvf_read(ctx.h, self.path.encode(), _dtype_code(dtype), C.byref(h)))
+        return DenseVector._adopt(h, self._size, dtype, ctx)
+
+
+class DvfWriter:
+    """DvfWriter (io/tensor.hpp:253-263): values written as f64."""
+
+    def __init__(self, path):
+        self.path = os.fspath(path)
+
+    def write(self, vec: DenseVector):
+        _check(lib().lgpu_dvf_write(vec.h, self.path.encode()))
 
 
 class EffDiffCalc:

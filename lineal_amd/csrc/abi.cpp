@@ -1407,6 +1407,112 @@ extern "C" int lgpu_outlet_flux_gradient(lgpu_op op, lgpu_vec p, lgpu_vec lambda
   API_END
 }
 
+// ---------------------------------------------------------------------------
+// matrix and vector files
+
+extern "C" int lgpu_smf_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz) {
+  API_BEGIN
+  LGPU_REQUIRE(rows && cols && nnz, "NULL argument");
+  const io::SmfHeader h = io::smf_header(path);
+  *rows = int64_t(h.rows);
+  *cols = int64_t(h.cols);
+  *nnz = int64_t(h.nnz);
+  API_END
+}
+
+extern "C" int lgpu_smf_read(lgpu_ctx ctx, const char* path, int dtype, int symmetric,
+                             lgpu_op* out) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && out, "NULL argument");
+  io::SmfHeader h;
+  std::vector<int64_t> rowptr, cols;
+  std::vector<double> vals;
+  io::smf_read(path, h, rowptr, cols, vals);
+  LGPU_REQUIRE(h.rows == h.cols, "only square SMF matrices are supported");
+  LGPU_REQUIRE(h.rows > 0 && h.rows < (uint64_t(1) << 31), "SMF matrix size out of range");
+  std::vector<int32_t> c32(cols.begin(), cols.end());
+  const int st = lgpu_csr_create(ctx, int64_t(h.rows), rowptr.data(), c32.data(), vals.data(),
+                                 dtype, symmetric, out);
+  if (st != LGPU_OK) return st;
+  API_END
+}
+
+extern "C" int lgpu_smf_write(lgpu_op op, const char* path) {
+  API_BEGIN
+  LGPU_REQUIRE(op, "NULL operator");
+  if (op->is_csr) {
+    const setup::RowMatrix& m = op->host_csr;
+    std::vector<int64_t> cols(m.cols.begin(), m.cols.end());
+    io::smf_write(path, uint64_t(m.n), uint64_t(m.n), m.rowptr.data(), cols.data(), m.vals.data(),
+                  true);
+  } else {
+    // the row iterate's entries in column order (ordered_tag)
+    setup::RowMatrix rows;
+    stencil_host_rows(op, rows);
+    std::vector<int64_t> cols(rows.cols.size());
+    std::vector<double> vals(rows.vals.size());
+    std::vector<std::pair<int64_t, double>> row;
+    for (int64_t i = 0; i < rows.n; ++i) {
+      row.clear();
+      for (int64_t k = rows.rowptr[i]; k < rows.rowptr[i + 1]; ++k)
+        row.emplace_back(rows.cols[size_t(k)], rows.vals[size_t(k)]);
+      std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+      for (size_t e = 0; e < row.size(); ++e) {
+        cols[size_t(rows.rowptr[i]) + e] = row[e].first;
+        vals[size_t(rows.rowptr[i]) + e] = row[e].second;
+      }
+    }
+    io::smf_write(path, uint64_t(rows.n), uint64_t(rows.n), rows.rowptr.data(), cols.data(),
+                  vals.data(), false);
+  }
+  API_END
+}
+
+extern "C" int lgpu_dvf_size(const char* path, int64_t* n) {
+  API_BEGIN
+  LGPU_REQUIRE(n, "NULL argument");
+  *n = int64_t(io::dvf_size(path));
+  API_END
+}
+
+extern "C" int lgpu_dvf_read(lgpu_ctx ctx, const char* path, int dtype, lgpu_vec* out) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && out, "NULL argument");
+  LGPU_REQUIRE(dtype == LGPU_F64 || dtype == LGPU_F32, "unknown dtype");
+  std::vector<double> v;
+  io::dvf_read(path, v);
+  lgpu_vec h = nullptr;
+  int st = lgpu_vec_create(ctx, int64_t(v.size()), dtype, &h);
+  if (st != LGPU_OK) return st;
+  std::unique_ptr<lgpu_vec_s> owned(h);
+  if (dtype == LGPU_F32) {
+    std::vector<float> f(v.begin(), v.end());
+    st = lgpu_vec_upload(h, f.data(), int64_t(f.size()));
+  } else {
+    st = lgpu_vec_upload(h, v.data(), int64_t(v.size()));
+  }
+  if (st != LGPU_OK) return st;
+  *out = owned.release();
+  API_END
+}
+
+extern "C" int lgpu_dvf_write(lgpu_vec v, const char* path) {
+  API_BEGIN
+  LGPU_REQUIRE(v, "NULL vector");
+  std::vector<double> host(size_t(v->n));
+  if (v->f32) {
+    std::vector<float> f(size_t(v->n));
+    LGPU_HIP_CHECK(hipMemcpyAsync(f.data(), v->ptr, f.size() * sizeof(float), hipMemcpyDeviceToHost, v->ctx->stream));
+    LGPU_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+    std::copy(f.begin(), f.end(), host.begin());
+  } else {
+    LGPU_HIP_CHECK(hipMemcpyAsync(host.data(), v->ptr, host.size() * sizeof(double), hipMemcpyDeviceToHost, v->ctx->stream));
+    LGPU_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+  }
+  io::dvf_write(path, host.data(), uint64_t(host.size()));
+  API_END
+}
+
 namespace {
 
 void check_eff_diff_op(lgpu_op op) {

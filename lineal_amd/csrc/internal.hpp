@@ -177,6 +177,22 @@ void eff_diff_flux(const StencilDesc& s, const EffDiffTables& t, const void* sol
 void vec_gather(const void* src, bool f32, const int64_t* idx, int64_t n, void* dst,
                 hipStream_t st);
 
+// matrix and vector files (host side, io.cpp)
+namespace io {
+struct SmfHeader {
+  uint64_t rows = 0, cols = 0, nnz = 0;
+};
+SmfHeader smf_header(const char* path);
+void smf_read(const char* path, SmfHeader& h, std::vector<int64_t>& rowptr,
+              std::vector<int64_t>& cols, std::vector<double>& vals);
+// stored_count: the header counts every stored entry (CSR), else the entries written
+void smf_write(const char* path, uint64_t rows, uint64_t cols_num, const int64_t* rowptr,
+               const int64_t* cols, const double* vals, bool stored_count);
+uint64_t dvf_size(const char* path);
+void dvf_read(const char* path, std::vector<double>& v);
+void dvf_write(const char* path, const double* v, uint64_t n);
+}  // namespace io
+
 // vector kernels
 void vec_fill(void* x, int64_t n, bool f32, double v, hipStream_t st);
 void vec_fill_int(int* x, int64_t n, int v, hipStream_t st);

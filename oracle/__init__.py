@@ -479,3 +479,62 @@ def eff_diff(S, sol, area_part, cmap=None):
         assert cmap.size == S.N
         cm = _p(cmap, C.c_int64)
     return float(lib().or_eff_diff(C.byref(S.s), cm, _pf(sol), float(area_part)))
+
+
+# ---------------------------------------------------------------------------
+# matrix and vector files, io/tensor.hpp:25-263 (little-endian u64 / f64)
+
+
+def smf_read(path):
+    """SmfReader::read (io/tensor.hpp:47-75) with the row-wise builder's
+    semantics (tensor/dynamic/csr/shared.hpp:730-764): per row, an entry whose
+    column was seen before is added to the first one (f64, file order), then
+    the row is sorted by column.  Returns (rows, columns, header nonzeros,
+    rowptr, cols, vals)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    rows, ncols, nnz = (int(v) for v in np.frombuffer(raw[:24], "<u8"))
+    off = np.frombuffer(raw[24:24 + (rows + 1) * 8], "<u8").astype(np.int64)
+    ent = np.frombuffer(raw[24 + (rows + 1) * 8:24 + (rows + 1) * 8 + int(off[-1]) * 16],
+                        dtype=[("c", "<u8"), ("v", "<f8")])
+    rp, cs, vs = [0], [], []
+    for r in range(rows):
+        acc = {}
+        for c, v in zip(ent["c"][off[r]:off[r + 1]].tolist(), ent["v"][off[r]:off[r + 1]].tolist()):
+            acc[c] = acc[c] + v if c in acc else v
+        for c in sorted(acc):
+            cs.append(c)
+            vs.append(acc[c])
+        rp.append(len(cs))
+    return rows, ncols, nnz, np.array(rp, np.int64), np.array(cs, np.int64), np.array(vs, np.float64)
+
+
+def smf_write(path, rows, ncols, rowptr, cols, vals, stored_count=True):
+    """SmfWriter::write (io/tensor.hpp:158-213): nonzero entries only, in the
+    given (column) order; the header counts the stored entries (stored_count,
+    a CsrMatrix's non_zero_num) or else the written ones."""
+    rowptr = np.asarray(rowptr, np.int64)
+    vals = np.asarray(vals, np.float64)
+    keep = vals != 0
+    written = np.concatenate([[0], np.cumsum(keep)])[rowptr]
+    head = np.array([rows, ncols, int(rowptr[-1]) if stored_count else int(written[-1])], "<u8")
+    ent = np.empty(int(keep.sum()), dtype=[("c", "<u8"), ("v", "<f8")])
+    ent["c"] = np.asarray(cols, np.int64)[keep]
+    ent["v"] = vals[keep]
+    with open(path, "wb") as f:
+        f.write(head.tobytes() + written.astype("<u8").tobytes() + ent.tobytes())
+
+
+def dvf_read(path):
+    """DvfReader::read (io/tensor.hpp:226-232)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    n = int(np.frombuffer(raw[:8], "<u8")[0])
+    return np.frombuffer(raw[8:8 + 8 * n], "<f8").copy()
+
+
+def dvf_write(path, v):
+    """DvfWriter::write (io/tensor.hpp:253-263)."""
+    v = np.asarray(v, np.float64)
+    with open(path, "wb") as f:
+        f.write(np.array([v.size], "<u8").tobytes() + v.astype("<f8").tobytes())

@@ -10,9 +10,18 @@
 //                     equals n_y n_z c dp / n_x and dJ/dlog c_k = J_col^2 / c_k
 //   effective diffusivity: the compressed 4 x 7 x 6 system of
 //                     test/shared-csr.cpp:193-252 gives the mean coefficient
+//   files (argv[1]: folder of UFZ_CT_02_LHS.smf / UFZ_CT_02_RHS.dvf): read,
+//                     written back byte for byte (test/tensor-io.cpp:58-100),
+//                     solved with AMG-CG
 // Prints one line per check; the exit status is the number of failed checks.
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iterator>
+#include <string>
 #include <vector>
 
 #include "lineal-amd/lineal-amd.hpp"
@@ -26,13 +35,18 @@ static void expect(bool ok, const char* what, double value) {
   if (!ok) ++failures;
 }
 
+static std::vector<char> file_bytes(const std::string& path) {
+  std::ifstream in(path, std::ios::binary);
+  return {std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>()};
+}
+
 static double inf_norm_diff(const std::vector<double>& a, double v) {
   double m = 0;
   for (double x : a) m = std::fmax(m, std::fabs(x - v));
   return m;
 }
 
-int main() {
+int main(int argc, char** argv) {
   try {
     lg::GpuContext ctx(0);
     const std::array<int64_t, 3> dims{4, 9, 7};
@@ -150,6 +164,29 @@ int main() {
       const double eff = calc(L, sys, x);
       expect(r.converged() && nc < L.row_num() && std::fabs(eff - mean) <= 1e-10,
              "effective diffusivity of the compressed system", std::fabs(eff - mean));
+    }
+    if (argc > 1) {
+      const std::string lhs = std::string(argv[1]) + "/UFZ_CT_02_LHS.smf";
+      const std::string rhs = std::string(argv[1]) + "/UFZ_CT_02_RHS.dvf";
+      const std::string tmp = (std::filesystem::temp_directory_path() /
+                               ("lineal_amd_io_" + std::to_string(getpid())))
+                                  .string();
+      lg::SmfReader<double> reader(lhs);
+      const auto U = reader.read(ctx, /*symmetric=*/true);
+      lg::SmfWriter(tmp + ".smf").write(U);
+      const auto b = lg::DvfReader<double>(rhs).read(ctx);
+      lg::DvfWriter(tmp + ".dvf").write(b);
+      const bool same = file_bytes(lhs) == file_bytes(tmp + ".smf") &&
+                        file_bytes(rhs) == file_bytes(tmp + ".dvf");
+      std::remove((tmp + ".smf").c_str());
+      std::remove((tmp + ".dvf").c_str());
+      lg::GpuDenseVector<double> x(U.row_num(), 0.0, ctx);
+      lg::GpuMultiGridSolver<double> m3;
+      lg::GpuConjugateGradients<> cg(8, &m3);
+      const auto r = lg::run_iterative_solver(cg, U, x, b, {1e-10, 200});
+      const double rel = r.residual_norms.back() / r.residual_norms.front();
+      expect(same && reader.nnz_num() == 3367 && r.converged() && rel <= 1e-10,
+             "SMF/DVF round trip and AMG-CG solve of UFZ_CT_02", rel);
     }
   } catch (const lg::GpuError& e) {
     std::printf("FAIL exception: %s\n", e.what());

@@ -1,7 +1,8 @@
 """The header-only C++ classes on the GPU: tests/cpp/stencil_api.cpp runs the
 reference's stencil test scenarios (known solution, Dirichlet residual, CSR
 copy, smoothers, adjoint closed form, effective diffusivity of the compressed
-system) through them."""
+system) through them, and reads, writes and solves the reference's UFZ_CT_02
+files from tests/golden."""
 
 import os
 import subprocess
@@ -17,7 +18,8 @@ EXE = os.path.join(ROOT, "tests", "cpp", "_build", "stencil_api")
 def test_cpp_api_program():
     assert os.path.exists(EXE), "build first: python __graft_entry__.py build (tests/cpp/Makefile)"
     print(f"running {EXE}", flush=True)
-    res = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    golden = os.path.join(ROOT, "tests", "golden")
+    res = subprocess.run([EXE, golden], capture_output=True, text=True, timeout=120)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
-    assert res.stdout.count("ok  ") == 9
+    assert res.stdout.count("ok  ") == 10

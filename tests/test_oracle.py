@@ -782,3 +782,70 @@ def test_advection_hierarchy_uses_lu_and_bicgstab_converges():
     x, it, _ = O.pbicgstab(S, b, H, precon=1, rel_tol=1e-10, max_it=200)
     assert it < 60
     assert np.linalg.norm(O.residual(S, b, x)) <= 1e-9 * np.linalg.norm(b)
+
+
+# ---------------------------------------------------------------------------
+# matrix and vector files (io/tensor.hpp:25-263)
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def test_smf_dvf_reference_fixtures_round_trip(tmp_path):
+    """test/tensor-io.cpp:58-100: reading the reference's UFZ_CT_02 files and
+    writing them again gives the same bytes."""
+    import filecmp
+    import os
+
+    lhs = os.path.join(GOLDEN, "UFZ_CT_02_LHS.smf")
+    rows, ncols, nnz, rp, cols, vals = O.smf_read(lhs)
+    assert (rows, ncols, nnz) == (539, 539, 3367) and rp[-1] == nnz
+    assert np.all(np.diff(rp) > 0) and np.all(vals != 0)
+    O.smf_write(tmp_path / "a.smf", rows, ncols, rp, cols, vals)
+    assert filecmp.cmp(tmp_path / "a.smf", lhs, shallow=False)
+    rhs = os.path.join(GOLDEN, "UFZ_CT_02_RHS.dvf")
+    b = O.dvf_read(rhs)
+    assert b.size == 539 and np.count_nonzero(b) == 49
+    O.dvf_write(tmp_path / "b.dvf", b)
+    assert filecmp.cmp(tmp_path / "b.dvf", rhs, shallow=False)
+    # the system is the SPD matrix the reference's readers build
+    A = np.zeros((rows, rows))
+    for i in range(rows):
+        A[i, cols[rp[i]:rp[i + 1]]] = vals[rp[i]:rp[i + 1]]
+    assert np.array_equal(A, A.T) and np.linalg.eigvalsh(A).min() > 0
+
+
+def write_raw_smf(path, rows, ncols, entries_per_row):
+    """Bytes of an SMF file written by hand (io/tensor.hpp:158-213 layout)."""
+    off = np.concatenate([[0], np.cumsum([len(r) for r in entries_per_row])])
+    ent = np.array([e for r in entries_per_row for e in r], dtype=[("c", "<u8"), ("v", "<f8")])
+    head = np.array([rows, ncols, off[-1]], "<u8")
+    with open(path, "wb") as f:
+        f.write(head.tobytes() + off.astype("<u8").tobytes() + ent.tobytes())
+
+
+def test_smf_read_merges_duplicates_and_sorts_rows(tmp_path):
+    """The row-wise builder (tensor/dynamic/csr/shared.hpp:730-764): duplicate
+    columns of a row add up, rows come out sorted; the writer then skips the
+    zero entries but its header keeps the stored count (non_zero_num)."""
+    rows = [[(2, 1.5), (0, 4.0), (2, 0.25)],
+            [(1, 3.0), (0, -1.0), (1, 1e-16), (1, 2.0)],
+            [(2, 0.0), (1, -2.5)]]
+    write_raw_smf(tmp_path / "d.smf", 3, 3, rows)
+    n, ncols, nnz, rp, cols, vals = O.smf_read(tmp_path / "d.smf")
+    dense = np.zeros((3, 3))
+    for i, r in enumerate(rows):
+        for c, v in r:
+            dense[i, c] += v
+    got = np.zeros((3, 3))
+    for i in range(3):
+        assert np.all(np.diff(cols[rp[i]:rp[i + 1]]) > 0)
+        got[i, cols[rp[i]:rp[i + 1]]] = vals[rp[i]:rp[i + 1]]
+    assert np.array_equal(got, dense) and nnz == 9 and rp[-1] == 6
+    assert vals[list(cols[rp[1]:rp[2]]).index(1) + rp[1]] == (3.0 + 1e-16) + 2.0
+    O.smf_write(tmp_path / "w.smf", n, ncols, rp, cols, vals)
+    raw = (tmp_path / "w.smf").read_bytes()
+    head = np.frombuffer(raw[:24], "<u8")
+    assert list(head) == [3, 3, 6]                      # stored entries, zero included
+    assert list(np.frombuffer(raw[24:56], "<u8")) == [0, 2, 4, 5]  # offsets skip the zero
+    O.smf_write(tmp_path / "s.smf", n, ncols, rp, cols, vals, stored_count=False)
+    assert np.frombuffer((tmp_path / "s.smf").read_bytes()[16:24], "<u8")[0] == 5
