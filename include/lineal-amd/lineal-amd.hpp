@@ -491,6 +491,53 @@ class GpuSorSolver {
   TReal relax_;
 };
 
+// SsorSolver<Real, FixedIterationManager, variant>{relax, IterMan{iterations}}
+// (gauss-seidel-variant/solver.hpp:45-96): apply runs `iterations` SSOR steps,
+// each a forward and a backward lexicographic sweep.  The instance is also a
+// Krylov preconditioner (GpuConjugateGradients<GpuSsorSolver<...>>), applied
+// from a zero initial guess.
+template <typename TReal, SorVariant tVariant = SorVariant::REGULAR>
+class GpuSsorSolver {
+ public:
+  explicit GpuSsorSolver(TReal relax, int iterations = 1) : relax_(relax), iterations_(iterations) {}
+  class Instance {
+   public:
+    Instance(const GpuMatrixBase& lhs, TReal relax, int iterations)
+        : lhs_(&lhs), relax_(relax), iterations_(iterations) {
+      lgpu_amg h = nullptr;
+      check(lgpu_ssor_precon_create(lhs.handle(), double(relax), ultra(), iterations, &h));
+      h_.reset(h);
+    }
+    // sol_in may be null (zero) or &sol_out
+    void apply(const GpuDenseVector<TReal>* sol_in, GpuDenseVector<TReal>& sol_out,
+               const GpuDenseVector<TReal>& rhs) const {
+      check(lgpu_op_ssor(lhs_->handle(), sol_in ? sol_in->handle() : nullptr, sol_out.handle(),
+                         rhs.handle(), double(relax_), ultra()));
+      for (int k = 1; k < iterations_; ++k)
+        check(lgpu_op_ssor(lhs_->handle(), sol_out.handle(), sol_out.handle(), rhs.handle(),
+                           double(relax_), ultra()));
+    }
+    [[nodiscard]] lgpu_amg handle() const { return h_.get(); }
+
+   private:
+    static constexpr int ultra() { return tVariant == SorVariant::ULTRA ? 1 : 0; }
+    struct Del {
+      void operator()(lgpu_amg a) const { lgpu_amg_destroy(a); }
+    };
+    const GpuMatrixBase* lhs_;
+    TReal relax_;
+    int iterations_;
+    std::unique_ptr<lgpu_amg_s, Del> h_;
+  };
+  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const {
+    return Instance(lhs, relax_, iterations_);
+  }
+
+ private:
+  TReal relax_;
+  int iterations_;
+};
+
 // ---------------------------------------------------------------------------
 // aggregation AMG
 

@@ -152,6 +152,11 @@ int lgpu_op_jacobi(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double
  * must not alias x_out. */
 int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
                 int forward, int ultra);
+/* One SSOR step, SsorBroom::sweep
+ * (linear-solver/iterative-stationary/gauss-seidel-variant/solver.hpp:45-88):
+ * a forward sweep x_in -> aux, then a backward sweep aux -> x_out, both
+ * lexicographic; x_in may be NULL (= 0) and may be x_out. */
+int lgpu_op_ssor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega, int ultra);
 /* One SOR sweep of a CSR operator with the reference's thread-segment
  * semantics (gauss-seidel-variant/sweep.hpp:153-169; the segments of
  * parallel/execution-policy/adaptive.hpp:266-279): rows are cut into segments
@@ -190,6 +195,13 @@ void lgpu_amg_default_params(lgpu_amg_params* p);
  * (direct/lu/decomposition.hpp:30-111) for a nonsymmetric one (upwind
  * advection, the BiCgStab preconditioner). */
 int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out);
+/* SsorSolver<Real, FixedIterationManager, variant>{omega, IterMan{iterations}}
+ * as a Krylov preconditioner (gauss-seidel-variant/solver.hpp:94-96,
+ * broom.hpp:159-171; test/aux/iterative.hpp:143-165): a preconditioner handle
+ * of the lgpu_amg type without a hierarchy.  Applying it runs `iterations`
+ * SSOR steps from a zero initial guess with the Krylov operator; it has no
+ * levels, and lgpu_amg_vcycle applies it to f64 vectors. */
+int lgpu_ssor_precon_create(lgpu_op op, double omega, int ultra, int iterations, lgpu_amg* out);
 int lgpu_amg_destroy(lgpu_amg amg);
 int lgpu_amg_level_num(lgpu_amg amg, int* levels);
 int lgpu_amg_level_size(lgpu_amg amg, int level, int64_t* n);

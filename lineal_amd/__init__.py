@@ -128,6 +128,8 @@ SIGNATURES = {
     "lgpu_op_jacobi": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
     "lgpu_op_sor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int]),
     "lgpu_op_sor_segmented": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _i64]),
+    "lgpu_op_ssor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int]),
+    "lgpu_ssor_precon_create": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int, C.POINTER(_vp)]),
     "lgpu_amg_default_params": (None, [C.POINTER(_AmgParams)]),
     "lgpu_amg_create": (C.c_int, [_vp, C.POINTER(_AmgParams), C.POINTER(_vp)]),
     "lgpu_amg_destroy": (C.c_int, [_vp]),
@@ -496,6 +498,44 @@ class SorSolver:
                 lhs.sor(sol_in, sol_out, rhs, solver.relax, solver.forward, solver.ultra)
 
         return Instance()
+
+
+class SsorSolver:
+    """SsorSolver<Real, IterMan, variant>{relax, IterMan{iterations}}
+    (gauss-seidel-variant/solver.hpp:45-96): each apply runs `iterations`
+    SSOR steps (a forward and a backward lexicographic sweep).  An instance is
+    also a Krylov preconditioner (applied from a zero initial guess)."""
+
+    def __init__(self, relax: float = 1.0, ultra: bool = False, iterations: int = 1):
+        self.relax, self.ultra, self.iterations = relax, ultra, iterations
+
+    def instantiate(self, lhs: _Matrix) -> "SsorInstance":
+        return SsorInstance(lhs, self)
+
+
+class SsorInstance:
+    def __init__(self, lhs: _Matrix, solver: SsorSolver):
+        self.lhs, self.solver = lhs, solver
+        h = _vp()
+        _check(lib().lgpu_ssor_precon_create(lhs.h, float(solver.relax), int(solver.ultra),
+                                             int(solver.iterations), C.byref(h)))
+        self.h = h
+
+    def apply(self, sol_in, sol_out, rhs):
+        """sol_out = SSOR^iterations(sol_in); sol_in may be None (zero) or sol_out."""
+        s = self.solver
+        _check(lib().lgpu_op_ssor(self.lhs.h, _vh(sol_in), sol_out.h, rhs.h, float(s.relax), int(s.ultra)))
+        for _ in range(s.iterations - 1):
+            _check(lib().lgpu_op_ssor(self.lhs.h, sol_out.h, sol_out.h, rhs.h, float(s.relax), int(s.ultra)))
+        return sol_out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and not _finalizing:
+                lib().lgpu_amg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -191,6 +191,8 @@ struct lgpu_op_s {
   // compressed system from a stencil: row of every cell or -1 (i64 per cell)
   DevBuf cmap;
   int64_t cmap_cells = 0;
+  // intermediate vector of SSOR (the forward sweep's output), f64-sized
+  DevBuf ssor_aux;
 
   int64_t rows() const { return is_csr ? csr.n : desc.nx * desc.ny * desc.nz; }
   void ensure_f32() {
@@ -222,6 +224,11 @@ struct lgpu_amg_s {
   double setup_seconds = 0.0;
   bool f32 = false;
   DevBuf r32, z32;  // finest f32 copies for the mixed-precision cycle
+  // an SSOR preconditioner instead of a hierarchy (lgpu_ssor_precon_create):
+  // ssor_iters applications of a forward and a backward sweep from zero
+  int ssor_iters = 0;
+  double ssor_omega = 1.0;
+  bool ssor_ultra = false;
 };
 
 namespace {
@@ -480,7 +487,23 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine) {
 }
 
 // z (f64) = M r (f64)
+// SsorBroom::sweep (gauss-seidel-variant/solver.hpp:45-88): a forward sweep
+// x_in -> aux, then a backward sweep aux -> x_out; x_out may be x_in
+void ssor_sweep(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, double omega,
+                bool ultra) {
+  if (!op->ssor_aux.ptr) op->ssor_aux.alloc(size_t(op->rows()) * sizeof(double));
+  op_sor(op, f32, x_in, op->ssor_aux.ptr, b, omega, true, ultra);
+  op_sor(op, f32, op->ssor_aux.ptr, x_out, b, omega, false, ultra);
+}
+
 void precondition(lgpu_amg amg, const double* r, double* z, lgpu_op fine) {
+  if (amg->ssor_iters > 0) {
+    // DoubleSweepImpl with a fixed iteration count (broom.hpp:159-171), from zero
+    ssor_sweep(fine, false, nullptr, z, r, amg->ssor_omega, amg->ssor_ultra);
+    for (int k = 1; k < amg->ssor_iters; ++k)
+      ssor_sweep(fine, false, z, z, r, amg->ssor_omega, amg->ssor_ultra);
+    return;
+  }
   if (!amg->f32) {
     vcycle(amg, r, z, fine);
     return;
@@ -982,6 +1005,36 @@ extern "C" int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b
   API_END
 }
 
+extern "C" int lgpu_op_ssor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
+                            int ultra) {
+  API_BEGIN
+  LGPU_REQUIRE(op, "NULL operator");
+  check_vec(x_out, op->rows(), "x_out");
+  check_vec(b, op->rows(), "b");
+  if (x_in) {
+    check_vec(x_in, op->rows(), "x_in");
+    LGPU_REQUIRE(x_in->f32 == x_out->f32, "x_in and x_out must share a dtype");
+  }
+  LGPU_REQUIRE(b->f32 == x_out->f32, "vectors must share a dtype");
+  ssor_sweep(op, x_out->f32, x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega, ultra != 0);
+  API_END
+}
+
+extern "C" int lgpu_ssor_precon_create(lgpu_op op, double omega, int ultra, int iterations,
+                                       lgpu_amg* out) {
+  API_BEGIN
+  LGPU_REQUIRE(op && out, "NULL argument");
+  LGPU_REQUIRE(iterations >= 1, "iterations must be at least 1");
+  auto amg = std::make_unique<lgpu_amg_s>();
+  amg->ctx = op->ctx;
+  amg->finest = op;
+  amg->ssor_iters = iterations;
+  amg->ssor_omega = omega;
+  amg->ssor_ultra = ultra != 0;
+  *out = amg.release();
+  API_END
+}
+
 extern "C" int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b,
                                      double omega, int forward, int ultra, int64_t segment_rows) {
   API_BEGIN
@@ -1146,6 +1199,15 @@ extern "C" int lgpu_amg_setup_seconds(lgpu_amg amg, double* seconds) {
 extern "C" int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z) {
   API_BEGIN
   LGPU_REQUIRE(amg, "NULL AMG");
+  if (amg->ssor_iters > 0) {
+    check_vec(r, amg->finest->rows(), "r");
+    check_vec(z, amg->finest->rows(), "z");
+    LGPU_REQUIRE(!r->f32 && !z->f32, "the SSOR preconditioner works on f64 vectors");
+    LGPU_REQUIRE(r->ptr != z->ptr, "r and z must not alias");
+    precondition(amg, static_cast<const double*>(r->ptr), static_cast<double*>(z->ptr),
+                 amg->finest);
+    return LGPU_OK;
+  }
   check_vec(r, amg->lv[0].n, "r");
   check_vec(z, amg->lv[0].n, "z");
   LGPU_REQUIRE(r->ptr != z->ptr, "r and z must not alias");

@@ -133,6 +133,7 @@ def lib():
             "or_outlet_flux": (C.c_double, [sp, f64p]),
             "or_outlet_flux_adjoint_rhs": (None, [sp, f64p]),
             "or_outlet_flux_gradient": (None, [sp, f64p, f64p, f64p]),
+            "or_ssor_f64": (None, [mp, f64p, f64p, f64p, C.c_double, C.c_int, C.c_int]),
             "or_eff_diff_area_part": (C.c_double, [sp, C.POINTER(C.c_uint8)]),
             "or_eff_diff": (C.c_double, [sp, i64p, f64p, C.c_double]),
         }
@@ -293,6 +294,17 @@ def sor(A, x_in, b, omega=1.0, forward=True, ultra=False, segments=None):
     x_out = np.zeros_like(b)
     _fn("or_sor", b.dtype)(C.byref(A.mat), _pf(x_in), _pf(x_out), _pf(b), omega,
                            int(forward), int(ultra), seg.size - 1, _p(seg, C.c_int64))
+    return x_out
+
+
+def ssor(A, x_in, b, omega=1.0, ultra=False, iterations=1):
+    """SsorSolver steps (gauss-seidel-variant/solver.hpp:45-96): forward then
+    backward lexicographic sweeps, `iterations` times; f64."""
+    b = np.ascontiguousarray(b, np.float64)
+    x_out = np.zeros_like(b)
+    xin = None if x_in is None else np.ascontiguousarray(x_in, np.float64)
+    lib().or_ssor_f64(C.byref(A.mat), _pf(xin), _pf(x_out), _pf(b), float(omega), int(ultra),
+                      int(iterations))
     return x_out
 
 

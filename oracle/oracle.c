@@ -940,9 +940,31 @@ static or_hierarchy* precon_view(const or_hierarchy* h, const or_mat* a) {
   return v;
 }
 
-static void apply_precon(const or_hierarchy* h, int precon, int ultra, const double* r, double* z,
-                         int64_t n) {
-  if (precon == 1) {
+/* SsorBroom::sweep (gauss-seidel-variant/solver.hpp:45-88) repeated `iters`
+ * times as DoubleSweepImpl does with a fixed iteration count
+ * (broom.hpp:159-171): forward x_in -> aux, backward aux -> x_out, then
+ * forward x_out -> aux, backward aux -> x_out; one thread segment. */
+void or_ssor_f64(const or_mat* a, const double* x_in, double* x_out, const double* b, double w,
+                 int ultra, int iters) {
+  const int64_t n = a->is_csr ? a->csr->n : a->st->n[0] * a->st->n[1] * a->st->n[2];
+  const int64_t seg[2] = {0, n};
+  double* aux = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  or_sor_f64(a, x_in, aux, b, w, 1, ultra, 1, seg);
+  or_sor_f64(a, aux, x_out, b, w, 0, ultra, 1, seg);
+  for (int k = 1; k < iters; ++k) {
+    or_sor_f64(a, x_out, aux, b, w, 1, ultra, 1, seg);
+    or_sor_f64(a, aux, x_out, b, w, 0, ultra, 1, seg);
+  }
+  free(aux);
+}
+
+/* precon 1: FP64 V-cycle, 2: FP32 V-cycle, 3: one SSOR step with omega = 1
+ * from zero (test/aux/iterative.hpp:143-165), else none */
+static void apply_precon(const or_mat* a, const or_hierarchy* h, int precon, int ultra,
+                         const double* r, double* z, int64_t n) {
+  if (precon == 3) {
+    or_ssor_f64(a, NULL, z, r, 1.0, ultra, 1);
+  } else if (precon == 1) {
     or_vcycle_f64(h, r, z, ultra);
   } else if (precon == 2) {
     float* r32 = (float*)calloc((size_t)n, sizeof(float));
@@ -971,7 +993,7 @@ int or_pcg(const or_mat* a, const or_hierarchy* h, int precon, int ultra, const 
   or_residual_f64(a, b, x, r);
   double rr = or_dot_f64(r, r, n), rho;
   if (precon) {
-    apply_precon(h, precon, ultra, r, p, n);
+    apply_precon(a, h, precon, ultra, r, p, n);
     rho = or_dot_f64(p, r, n);
   } else {
     memcpy(p, r, sizeof(double) * (size_t)n);
@@ -993,7 +1015,7 @@ int or_pcg(const or_mat* a, const or_hierarchy* h, int precon, int ultra, const 
     rr = or_dot_f64(r, r, n);
     const double old = rho;
     if (precon) {
-      apply_precon(h, precon, ultra, r, q, n);
+      apply_precon(a, h, precon, ultra, r, q, n);
       rho = or_dot_f64(r, q, n);
       const double beta = rho / old;
       for (int64_t i = 0; i < n; ++i) p[i] = q[i] + p[i] * beta;
@@ -1033,13 +1055,13 @@ int or_pbicgstab(const or_mat* a, const or_hierarchy* h, int precon, int ultra,
   if (res_hist) res_hist[0] = rn0;
   int it = 0;
   while (it < max_it && sqrt(rr) > rel_tol * rn0 && rho != 0) {
-    apply_precon(h, precon, ultra, p, y, n);
+    apply_precon(a, h, precon, ultra, p, y, n);
     or_spmv_f64(a, y, v);
     const double alpha = rho / or_dot_f64(r0, v, n);
     for (int64_t i = 0; i < n; ++i) x[i] = x[i] + y[i] * alpha;
     for (int64_t i = 0; i < n; ++i) r[i] = r[i] - v[i] * alpha;
     rr = or_dot_f64(r, r, n);
-    apply_precon(h, precon, ultra, r, y, n);
+    apply_precon(a, h, precon, ultra, r, y, n);
     or_spmv_f64(a, y, t);
     const double omega = or_dot_f64(t, r, n) / or_dot_f64(t, t, n);
     for (int64_t i = 0; i < n; ++i) x[i] = x[i] + y[i] * omega;

@@ -95,6 +95,12 @@ void or_sor_f64(const or_mat* a, const double* x_in, double* x_out, const double
 void or_sor_f32(const or_mat* a, const float* x_in, float* x_out, const float* b, float w,
                 int forward, int ultra, int nseg, const int64_t* seg);
 
+/* SSOR: `iters` times a forward sweep into an intermediate vector and a
+ * backward sweep from it (gauss-seidel-variant/solver.hpp:45-96), one segment;
+ * x_in may be NULL (= 0) and may equal x_out */
+void or_ssor_f64(const or_mat* a, const double* x_in, double* x_out, const double* b, double w,
+                 int ultra, int iters);
+
 /* sequential double accumulation of a[i]*b[i] */
 double or_dot_f64(const double* a, const double* b, int64_t n);
 double or_dot_f32(const float* a, const float* b, int64_t n);
@@ -161,7 +167,8 @@ void or_vcycle_f32(const or_hierarchy* h, const float* r, float* z, int ultra);
 /* Krylov drivers (conjugate-gradients/solver.hpp, bicgstab/solver.hpp) with the
  * true residual recomputed at iterations 8, 16, ... and stopping on
  * ||r_k|| <= rel_tol ||r_0|| or max_it.  precon: 0 none, 1 AMG FP64, 2 AMG mixed
- * (FP32 V-cycle).  Returns iterations; res_hist (max_it+1 entries) may be NULL. */
+ * (FP32 V-cycle), 3 one SSOR step with omega = 1 (h may be NULL).  Returns
+ * iterations; res_hist (max_it+1 entries) may be NULL. */
 int or_pcg(const or_mat* a, const or_hierarchy* h, int precon, int ultra, const double* b,
            double* x, double rel_tol, int max_it, double* res_hist);
 int or_pbicgstab(const or_mat* a, const or_hierarchy* h, int precon, int ultra,

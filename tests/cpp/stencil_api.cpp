@@ -89,6 +89,21 @@ int main(int argc, char** argv) {
       A.residual(rhs, x, res);
       const double rinf = inf_norm_diff(res.to_host(), 0.0);
       expect(rinf <= 2e-14, "pcg_amg dirichlet residual", rinf);
+      // ssor1 (129 applications, 1.1e-14) and pcg_ssor1 (1e-13),
+      // test/aux/iterative.hpp:122-153 with test/shared-stencil.hpp:198-224
+      lg::GpuSsorSolver<double> ssor(1.0);
+      const auto si = ssor.instantiate(A);
+      lg::GpuDenseVector<double> y(n, 0.0, ctx);
+      for (int k = 0; k < 129; ++k) si.apply(&y, y, rhs);
+      A.residual(rhs, y, res);
+      const double sinf = inf_norm_diff(res.to_host(), 0.0);
+      lg::GpuDenseVector<double> z(n, 0.0, ctx);
+      lg::GpuConjugateGradients<lg::GpuSsorSolver<double>> pcg(8, &ssor);
+      const auto rp = lg::run_iterative_solver(pcg, A, z, rhs, {1e-300, 129});
+      A.residual(rhs, z, res);
+      const double pinf = inf_norm_diff(res.to_host(), 0.0);
+      expect(sinf <= 1.1e-14 && pinf <= 1e-13 && rp.residual_norm() <= 1e-13,
+             "ssor1 and pcg_ssor1 dirichlet residual", std::fmax(sinf, pinf));
       (void)r;
     }
 

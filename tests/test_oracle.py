@@ -849,3 +849,64 @@ def test_smf_read_merges_duplicates_and_sorts_rows(tmp_path):
     assert list(np.frombuffer(raw[24:56], "<u8")) == [0, 2, 4, 5]  # offsets skip the zero
     O.smf_write(tmp_path / "s.smf", n, ncols, rp, cols, vals, stored_count=False)
     assert np.frombuffer((tmp_path / "s.smf").read_bytes()[16:24], "<u8")[0] == 5
+
+
+# ---------------------------------------------------------------------------
+# SSOR (gauss-seidel-variant/solver.hpp:45-96)
+
+
+@pytest.mark.parametrize("ultra", [False, True])
+@pytest.mark.parametrize("kind", ["values", "csr"])
+def test_ssor_is_a_forward_then_backward_sweep(ultra, kind):
+    """Against the dense restatement of the python reference's SOR: one SSOR
+    step is a forward sweep into an intermediate vector, then a backward one."""
+    dims = (4, 5, 6)
+    S, c = lognormal_stencil(dims)
+    A = dense_operator(dims, c)
+    M = S if kind == "values" else S.to_csr()[0]
+    x = problems.uniform(S.N, 3)
+    b = problems.uniform(S.N, 4)
+    for iters in (1, 3):
+        ref = x.copy()
+        for _ in range(iters):
+            ref = dense_sor(A, dense_sor(A, ref, b, 1.3, True, ultra=ultra), b, 1.3, False, ultra=ultra)
+        np.testing.assert_allclose(O.ssor(M, x, b, 1.3, ultra, iters), ref, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(O.ssor(M, None, b, 1.3, ultra),
+                               dense_sor(A, dense_sor(A, np.zeros(S.N), b, 1.3, True, ultra=ultra),
+                                         b, 1.3, False, ultra=ultra), rtol=1e-12, atol=1e-12)
+
+
+def test_ssor_preconditioner_is_symmetric():
+    """From zero with omega = 1, SSOR applies (D+U)^-1 D (D+L)^-1, a symmetric
+    operator for symmetric A: <M r1, r2> = <r1, M r2>.  A sweep run in the
+    wrong direction or order breaks this."""
+    dims = (5, 4, 6)
+    S, _ = lognormal_stencil(dims)
+    r1 = problems.uniform(S.N, 7)
+    r2 = problems.uniform(S.N, 8)
+    a = np.dot(O.ssor(S, None, r1), r2)
+    b = np.dot(r1, O.ssor(S, None, r2))
+    assert abs(a - b) <= 1e-13 * abs(a)
+    # two forward sweeps are not symmetric
+    fwd2 = lambda r: O.sor(S, O.sor(S, None, r, 1.0, True), r, 1.0, True)
+    assert abs(np.dot(fwd2(r1), r2) - np.dot(r1, fwd2(r2))) > 1e-6 * abs(a)
+
+
+@pytest.mark.parametrize("ultra", [False, True])
+def test_reference_thresholds_ssor(ultra):
+    """test/shared-stencil.hpp:198-224 with test/aux/iterative.hpp:122-165 on the
+    4x9x7 diagonal-lines system with the Dirichlet rhs: ssor1 and ssor5 run 129
+    solver applications (residual <= 1.1e-14), pcg_ssor1 and pbicgstab_ssor1
+    at most 129 iterations (residual <= 1e-13)."""
+    S, _, _ = reference_system()
+    b = S.rhs()
+    for iters in (1, 5):
+        x = np.zeros(S.N)
+        for _ in range(129):
+            x = O.ssor(S, x, b, 1.0, ultra, iters)
+        r = O.residual(S, b, x)
+        assert np.linalg.norm(r) <= 1.1e-14 and np.abs(r).max() <= 1.1e-14
+    for fn in (O.pcg, O.pbicgstab):
+        x, it, hist = fn(S, b, None, precon=3, ultra=ultra, rel_tol=0.0, max_it=129)
+        r = O.residual(S, b, x)
+        assert hist[-1] <= 1e-13 and np.abs(r).max() <= 1e-13
