@@ -117,7 +117,8 @@ def main():
     M32 = la.MultiGridSolver(precision=np.float32).instantiate(A)
     log(f"  {M32.setup_seconds:.2f} s, levels {[M32.level_size(l) for l in range(M32.level_num)]}")
     M64 = la.MultiGridSolver(precision=np.float64).instantiate(A)
-    log(f"AMG setup (FP64 hierarchy) {M64.setup_seconds:.2f} s")
+    log(f"AMG setup (FP64 hierarchy) {M64.setup_seconds:.2f} s, by phase "
+        + ", ".join(f"{k} {v:.2f}" for k, v in M64.setup_split.items()))
 
     x = la.DenseVector(N, ctx=ctx)
 
@@ -251,6 +252,7 @@ def main():
             "mixed_vs_fp64_rel_error": rel_err,
             "setup_seconds_mixed": M32.setup_seconds,
             "setup_seconds_fp64": M64.setup_seconds,
+            "setup_split_fp64": M64.setup_split,
             "levels": [M32.level_size(l) for l in range(M32.level_num)],
             "kernels": kern,
             "timed_region_kernel_classes": classes,

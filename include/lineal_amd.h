@@ -213,6 +213,11 @@ int lgpu_amg_matrix(lgpu_amg amg, int level, int64_t* nnz, int64_t* rowptr, int3
                     double* vals);
 /* seconds spent in the host setup */
 int lgpu_amg_setup_seconds(lgpu_amg amg, double* seconds);
+/* The host setup's wall seconds by phase (no counterpart in the reference):
+ * [0] strength graph and criterion, [1] aggregation, [2] Galerkin products,
+ * [3] coarsest factorisation, [4] the rest (row evaluation of the finest
+ * operator, uploads). */
+int lgpu_amg_setup_split(lgpu_amg amg, double seconds[5]);
 /* z = M r, one V-cycle from zero (MultiGridSolver::Instance::apply with a
  * constant-zero in_sol, multigrid/solver.hpp:47-53); r and z are f64. */
 int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z);

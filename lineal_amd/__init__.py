@@ -138,6 +138,7 @@ SIGNATURES = {
     "lgpu_amg_aggregates": (C.c_int, [_vp, C.c_int, _u32p]),
     "lgpu_amg_matrix": (C.c_int, [_vp, C.c_int, _i64p, _i64p, _i32p, _f64p]),
     "lgpu_amg_setup_seconds": (C.c_int, [_vp, _f64p]),
+    "lgpu_amg_setup_split": (C.c_int, [_vp, _f64p]),
     "lgpu_amg_vcycle": (C.c_int, [_vp, _vp, _vp]),
     "lgpu_cg_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
     "lgpu_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
@@ -621,6 +622,13 @@ class MultiGridInstance:
         s = C.c_double()
         _check(lib().lgpu_amg_setup_seconds(self.h, C.byref(s)))
         return s.value
+
+    @property
+    def setup_split(self) -> dict:
+        """Host setup seconds by phase."""
+        out = np.zeros(5)
+        _check(lib().lgpu_amg_setup_split(self.h, out.ctypes.data_as(_f64p)))
+        return dict(zip(("criterion", "aggregation", "galerkin", "coarsest", "rest"), out.tolist()))
 
     def apply(self, r: DenseVector, z: DenseVector):
         """z = M r: one V-cycle from a zero initial guess."""

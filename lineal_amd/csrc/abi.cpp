@@ -1196,6 +1196,20 @@ extern "C" int lgpu_amg_setup_seconds(lgpu_amg amg, double* seconds) {
   API_END
 }
 
+extern "C" int lgpu_amg_setup_split(lgpu_amg amg, double seconds[5]) {
+  API_BEGIN
+  LGPU_REQUIRE(amg && seconds, "NULL argument");
+  const setup::Hierarchy& h = amg->host;
+  const double phases = h.seconds_criterion + h.seconds_aggregation + h.seconds_galerkin +
+                        h.seconds_coarsest;
+  seconds[0] = h.seconds_criterion;
+  seconds[1] = h.seconds_aggregation;
+  seconds[2] = h.seconds_galerkin;
+  seconds[3] = h.seconds_coarsest;
+  seconds[4] = std::max(0.0, amg->setup_seconds - phases);
+  API_END
+}
+
 extern "C" int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z) {
   API_BEGIN
   LGPU_REQUIRE(amg, "NULL AMG");

@@ -5,6 +5,7 @@
 #include "setup.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <thread>
 
@@ -467,18 +468,28 @@ bool lu_dense(const RowMatrix& a, std::vector<double>& lower, std::vector<double
 
 // HierarchyWrightInstance::create_impl (multigrid/hierarchy/wright.hpp:120-143, 145-309)
 Hierarchy build(const RowMatrix& finest, const Params& p) {
+  using Clock = std::chrono::steady_clock;
+  auto since = [](Clock::time_point t0) {
+    return std::chrono::duration<double>(Clock::now() - t0).count();
+  };
   Hierarchy h;
   h.sizes.push_back(finest.n);
   bool aggressive = false;
   const RowMatrix* fine = &finest;
   h.coarse.reserve(size_t(p.max_level_num) + 1);
   for (int level = 0;; ++level) {
+    auto t0 = Clock::now();
     Graph g;
     build_graph(*fine, g, p.threads);
     criterion(g, aggressive ? p.aggressive_strong_threshold : p.strong_threshold, p.threads);
+    h.seconds_criterion += since(t0);
+    t0 = Clock::now();
     Aggregator aggr(g, p.min_aggregate_size, p.max_aggregate_size, aggressive);
     h.transfer.push_back(aggr.run());
+    h.seconds_aggregation += since(t0);
+    t0 = Clock::now();
     h.coarse.push_back(galerkin(*fine, h.transfer.back(), p.coarse_f32, p.threads));
+    h.seconds_galerkin += since(t0);
     const int64_t nc = h.coarse.back().n;
     h.sizes.push_back(nc);
     bool stop = false;
@@ -493,11 +504,13 @@ Hierarchy build(const RowMatrix& finest, const Params& p) {
     if (stop) break;
     fine = &h.coarse.back();
   }
+  const auto tc = Clock::now();
   if (!h.coarse.back().symmetric) {
     // nonsymmetric hierarchies (BiCgStab) take an LU coarsest solver
     // (test/aux/iterative.hpp:122-181)
     h.lu = true;
     h.spd = lu_dense(h.coarse.back(), h.chol_dense, h.upper_dense);
+    h.seconds_coarsest = since(tc);
     return h;
   }
   RowMatrix lower;
@@ -508,6 +521,7 @@ Hierarchy build(const RowMatrix& finest, const Params& p) {
     for (int64_t i = 0; i < n; ++i)
       for (int64_t k = lower.rowptr[i]; k < lower.rowptr[i + 1]; ++k)
         h.chol_dense[size_t(i * n + lower.cols[k])] = lower.vals[k];
+  h.seconds_coarsest = since(tc);
   return h;
 }
 

@@ -52,6 +52,9 @@ struct Hierarchy {
   std::vector<double> upper_dense;  // coarsest upper LU factor (nonsymmetric), dense row major
   bool lu = false;                  // nonsymmetric coarsest matrix: LU instead of Cholesky
   bool spd = true;                  // the coarsest factorisation succeeded
+  // wall seconds of the setup phases, summed over levels: strength graph and
+  // criterion, aggregation, Galerkin products, coarsest factorisation
+  double seconds_criterion = 0, seconds_aggregation = 0, seconds_galerkin = 0, seconds_coarsest = 0;
 };
 
 constexpr uint32_t kUnaggregated = 0xFFFFFFFFu;
