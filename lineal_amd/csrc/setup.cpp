@@ -127,17 +127,22 @@ class Aggregator {
  public:
   Aggregator(const Graph& g, int64_t min_size, int64_t max_size, bool aggressive)
       : g_(g), min_(min_size), max_(max_size), aggressive_(aggressive),
-        agg_(size_t(g.n), kUnaggregated), in_front_(size_t(g.n), 0) {}
+        agg_(size_t(g.n), kUnaggregated), in_front_(size_t(g.n), 0), conn_(size_t(g.n) + 1, 0),
+        kind_(g.head.size()) {
+    // bit 0: the edge's head is a strong dependency; bit 1: it is influenced
+    for (size_t e = 0; e < kind_.size(); ++e) kind_[e] = uint8_t(g.depends[e] | (g.influences[e] << 1));
+  }
 
   Level run() {
     for (int64_t v = 0; v < g_.n; ++v) handle_vertex(v);
     Level lv;
-    const int64_t nagg = int64_t(members_.size()) - 1;  // the last block is the open one
+    const int64_t nagg = int64_t(size_.size());
     lv.c2f_ptr.assign(size_t(nagg + 1), 0);
-    for (int64_t c = 0; c < nagg; ++c) lv.c2f_ptr[c + 1] = lv.c2f_ptr[c] + int64_t(members_[c].size());
-    lv.c2f_idx.reserve(size_t(lv.c2f_ptr[nagg]));
-    for (int64_t c = 0; c < nagg; ++c)
-      for (int32_t v : members_[c]) lv.c2f_idx.push_back(v);
+    for (int64_t c = 0; c < nagg; ++c) lv.c2f_ptr[c + 1] = lv.c2f_ptr[c] + size_[c];
+    // the log in order, placed per aggregate: members in insertion order
+    lv.c2f_idx.resize(size_t(lv.c2f_ptr[nagg]));
+    std::vector<int64_t> pos(lv.c2f_ptr.begin(), lv.c2f_ptr.end() - 1);
+    for (size_t k = 0; k < log_agg_.size(); ++k) lv.c2f_idx[pos[log_agg_[k]]++] = log_vertex_[k];
     stats_.aggregates = nagg;
     lv.stats = stats_;
     lv.agg = std::move(agg_);
@@ -146,11 +151,11 @@ class Aggregator {
 
  private:
   bool is_aggregate(uint32_t a) const { return a < kIsolated; }
-  bool connected(uint32_t a) const {
-    return std::find(connected_.begin(), connected_.end(), a) != connected_.end();
-  }
-  bool two_way(int64_t e) const { return g_.depends[e] && g_.influences[e]; }
-  bool one_way(int64_t e) const { return g_.depends[e] && !g_.influences[e]; }
+  // membership of the aggregate's connected set: conn_[a] holds the stamp of
+  // the last vertex whose aggregate reached a
+  bool connected(uint32_t a) const { return conn_[a] == stamp_; }
+  bool two_way(int64_t e) const { return kind_[e] == 3; }
+  bool one_way(int64_t e) const { return kind_[e] == 1; }
   bool strong(int64_t e) const { return g_.depends[e] || g_.influences[e]; }
 
   void front_insert(int32_t v) {
@@ -166,11 +171,11 @@ class Aggregator {
   // AggregateManager::add_help: store, then scan every neighbour
   void add(int32_t u, uint32_t cur) {
     agg_[u] = cur;
-    members_[cur].push_back(u);
+    open_.push_back(u);
     for (int64_t e = g_.eptr[u]; e < g_.eptr[u + 1]; ++e) {
       const int32_t t = g_.head[e];
       const uint32_t ta = agg_[t];
-      if (is_aggregate(ta) && !connected(ta)) connected_.push_back(ta);
+      if (is_aggregate(ta)) conn_[ta] = stamp_;
       if (ta == kUnaggregated && !in_front_[t]) front_insert(t);
     }
   }
@@ -183,7 +188,7 @@ class Aggregator {
   }
 
   void grow(uint32_t cur) {
-    while (int64_t(members_[cur].size()) < min_) {
+    while (int64_t(open_.size()) < min_) {
       int64_t best0 = 0, best2 = 0;
       double best1 = 0.0;
       bool has_two_way = false;
@@ -193,7 +198,8 @@ class Aggregator {
         int64_t con = 0, nbs = 0, cons2 = 0, cons1 = 0;
         for (int64_t e = g_.eptr[v]; e < g_.eptr[v + 1]; ++e) {
           const uint32_t ha = agg_[g_.head[e]];
-          con += is_aggregate(ha) && connected(ha);
+          // conn_[n] is never stamped: non-aggregates count as unconnected
+          con += conn_[std::min<uint32_t>(ha, uint32_t(g_.n))] == stamp_;
           nbs += in_front_[g_.head[e]];
           cons2 += ha == cur && two_way(e);
           cons1 += ha == cur && one_way(e);
@@ -224,14 +230,14 @@ class Aggregator {
         }
       }
       if (cand_.empty()) break;
-      const size_t room = size_t(max_ - int64_t(members_[cur].size()));
+      const size_t room = size_t(max_ - int64_t(open_.size()));
       if (cand_.size() > room) cand_.resize(room);
       add_all(cur);
     }
   }
 
   void round(uint32_t cur) {
-    while (int64_t(members_[cur].size()) < max_) {
+    while (int64_t(open_.size()) < max_) {
       cand_.clear();
       for (int32_t v : front_) {
         if (g_.isolated[v]) continue;
@@ -252,41 +258,42 @@ class Aggregator {
   }
 
   void handle_vertex(int64_t v) {
-    if (members_.empty()) members_.emplace_back();
     if (agg_[v] != kUnaggregated) return;
     if (g_.isolated[v]) {
       agg_[v] = kIsolated;
       ++stats_.isolated;
       return;
     }
-    const uint32_t cur = uint32_t(members_.size() - 1);
+    const uint32_t cur = uint32_t(size_.size());  // the open aggregate's index
+    open_.clear();
     front_.clear();
-    connected_.clear();
+    stamp_ = uint32_t(v) + 1;  // a fresh, empty connected set
     add(int32_t(v), cur);
     grow(cur);
     round(cur);
     for (int32_t f : front_) in_front_[f] = 0;
     front_.clear();
-    if (members_[cur].size() == 1 && max_ > 1) {
+    if (open_.size() == 1 && max_ > 1) {
       // find_merge_aggregate: first strong neighbour with room in its aggregate
+      // (a closed one: only v itself is in the open aggregate)
       uint32_t target = kUnaggregated;
       for (int64_t e = g_.eptr[v]; e < g_.eptr[v + 1]; ++e) {
         if (!strong(e)) continue;
         const uint32_t ha = agg_[g_.head[e]];
-        if (is_aggregate(ha) && int64_t(members_[ha].size()) < max_) {
+        if (is_aggregate(ha) && size_[ha] < max_) {
           target = ha;
           break;
         }
       }
       if (target != kUnaggregated) {
-        members_[cur].clear();
         agg_[v] = target;
-        members_[target].push_back(int32_t(v));
+        ++size_[target];
+        log_agg_.push_back(target);
+        log_vertex_.push_back(int32_t(v));
         ++stats_.merged;
         return;
       }
       if (aggressive_) {
-        members_[cur].clear();
         agg_[v] = kIsolated;
         ++stats_.isolated;
         return;
@@ -295,7 +302,12 @@ class Aggregator {
     } else {
       ++stats_.multi;
     }
-    members_.emplace_back();  // finalize: open the next aggregate
+    // finalize: close the open aggregate
+    size_.push_back(int64_t(open_.size()));
+    for (int32_t u : open_) {
+      log_agg_.push_back(cur);
+      log_vertex_.push_back(u);
+    }
   }
 
   const Graph& g_;
@@ -304,9 +316,14 @@ class Aggregator {
   std::vector<uint32_t> agg_;
   std::vector<uint8_t> in_front_;
   std::vector<int32_t> front_;
-  std::vector<uint32_t> connected_;
+  std::vector<uint32_t> conn_;
+  std::vector<uint8_t> kind_;
+  uint32_t stamp_ = 0;
   std::vector<int32_t> cand_;
-  std::vector<std::vector<int32_t>> members_;
+  std::vector<int32_t> open_;        // members of the open aggregate, in order
+  std::vector<int64_t> size_;        // member count of each closed aggregate
+  std::vector<uint32_t> log_agg_;    // every membership in insertion order:
+  std::vector<int32_t> log_vertex_;  // aggregate and vertex
   AggregationStats stats_;
 };
 
