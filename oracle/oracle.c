@@ -18,10 +18,13 @@
  *    (test/python-reference/cholesky.py) and of coarsest AMG matrices.
  *  - Galerkin / restriction / prolongation: P^T A P and P^T r with dense numpy
  *    products for the oracle's own aggregate maps.
- *  - Aggregation and strength criterion: invariants only (every vertex in one
- *    aggregate or isolated, aggregate sizes <= max, strong-edge rules) plus the
- *    reference's convergence thresholds.  The exact aggregate maps are
- *    "parity unpinned": the reference cannot be built here.
+ *  - Aggregation and strength criterion: aggregates traced by hand from the
+ *    grow / round / single-vertex rules on chains, grids, a pendant vertex and
+ *    a weak coupling (tests/aggregation_cases.py: ranking keys, ties,
+ *    insertion order, merge); invariants on stencil hierarchies (every vertex
+ *    in one aggregate or isolated, sizes <= max, strong-edge rules); the
+ *    reference's convergence thresholds.  Maps of large graphs equal the
+ *    reference's by construction only: the reference cannot be built here.
  *  - CG / BiCGStab / PCG-AMG: exact termination on tiny SPD systems, the
  *    reference's thresholds (test/shared-stencil.hpp:157-224), scipy solves.
  *  - Effective diffusivity: the reference's known answer for diagonal lines

@@ -272,3 +272,18 @@ def test_bicgstab_nonsymmetric_amg_matches_oracle():
     assert res.converged and abs(res.iterations - it_ref) <= 1
     assert np.abs(x.numpy() - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
     assert np.linalg.norm(O.residual(S, b, x.numpy())) <= 1e-9 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in __import__("aggregation_cases").CASES])
+def test_hierarchy_known_answers(case):
+    """The library's host aggregation on the hand-traced graphs of
+    aggregation_cases.py (the same known answers that pin the oracle)."""
+    import aggregation_cases as AC
+
+    _, build, aggregates, _ = next(c for c in AC.CASES if c[0] == case)
+    rp, cols, vals = build()
+    A = la.CsrMatrix.from_arrays(rp, cols, vals)
+    wright = la.HierarchyWrightParams(max_level_num=2, min_coarsen_size=1, min_coarsen_factor=1.0)
+    M = la.MultiGridSolver(wright).instantiate(A)
+    assert M.level_size(1) == len(aggregates)
+    np.testing.assert_array_equal(M.aggregates(0), AC.expected_map(rp.size - 1, aggregates))

@@ -910,3 +910,23 @@ def test_reference_thresholds_ssor(ultra):
         x, it, hist = fn(S, b, None, precon=3, ultra=ultra, rel_tol=0.0, max_it=129)
         r = O.residual(S, b, x)
         assert hist[-1] <= 1e-13 and np.abs(r).max() <= 1e-13
+
+
+@pytest.mark.parametrize("case", [c[0] for c in __import__("aggregation_cases").CASES])
+def test_aggregation_known_answers(case):
+    """Hand-traced aggregates of chains and small grids (aggregation_cases.py):
+    the ranking keys, tie handling, insertion order, the round rule and the
+    single-vertex rule."""
+    import aggregation_cases as AC
+
+    _, build, aggregates, counts = next(c for c in AC.CASES if c[0] == case)
+    rp, cols, vals = build()
+    A = O.Csr(rp, cols, vals, symmetric=True)
+    H = O.Hierarchy(A, O.AmgParams(2, 1, 1.0, 1.0 / 3.0, 1.0 / 3.0, 6, 9, 0))
+    n = rp.size - 1
+    assert H.level_num >= 2 and H.size(1) == len(aggregates)
+    np.testing.assert_array_equal(H.agg_map(0), AC.expected_map(n, aggregates))
+    ptr, idx = H.c2f(0)
+    assert [idx[ptr[a]:ptr[a + 1]].tolist() for a in range(len(aggregates))] == aggregates
+    stats = H.stats(0)
+    assert stats[0] == len(aggregates) and tuple(stats[1:4]) == counts and stats[4] == 0
