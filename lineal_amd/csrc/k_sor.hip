@@ -124,6 +124,77 @@ __device__ __forceinline__ void rstore(T v, Rsrc r, uint32_t off) {
   }
 }
 
+// Runs of consecutive elements with 16-byte loads and stores: a lane's K-step
+// window of one line is contiguous, so one instruction moves four dwords of
+// it.  Element loads touch one cache line per lane and instruction, and these
+// sweeps are bound by that line rate of the vector memory pipeline, not by
+// bytes.  The lowest element's byte offset must be valid (the run may reach
+// Callers keep every run they use inside its resource; kRunOff moves nothing
+// (2^31 and the run's length stay beyond any column resource, which holds
+// fewer than 2^31 bytes, with or without 32-bit wrap-around).
+constexpr uint32_t kRunOff = 0x80000000u;
+template <int NDW>
+__device__ __forceinline__ void load_dwords(Rsrc r, uint32_t off, uint32_t (&d)[NDW]) {
+  using V4 = unsigned int __attribute__((ext_vector_type(4)));
+  using V2 = unsigned int __attribute__((ext_vector_type(2)));
+  constexpr int n4 = NDW / 4, rem = NDW % 4;
+#pragma unroll
+  for (int i = 0; i < n4; ++i) {
+    const V4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u * i, 0u, 0);
+    d[4 * i] = v[0];
+    d[4 * i + 1] = v[1];
+    d[4 * i + 2] = v[2];
+    d[4 * i + 3] = v[3];
+  }
+  if constexpr (rem >= 2) {
+    const V2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16u * n4, 0u, 0);
+    d[4 * n4] = v[0];
+    d[4 * n4 + 1] = v[1];
+  }
+  if constexpr (rem % 2) d[NDW - 1] = __builtin_amdgcn_raw_buffer_load_b32(r, off + 4u * (NDW - 1), 0u, 0);
+}
+
+// out[j] = element lo + j (REV: lo + N - 1 - j)
+template <typename E, int N, bool REV>
+__device__ __forceinline__ void load_run(Rsrc r, uint32_t lo_off, E (&out)[N]) {
+  static_assert(sizeof(E) == 4 || sizeof(E) == 8, "runs of dword elements");
+  constexpr int W = int(sizeof(E) / 4);
+  uint32_t d[N * W];
+  load_dwords<N * W>(r, lo_off, d);
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    E v;
+    if constexpr (W == 1) v = __builtin_bit_cast(E, d[j]);
+    else v = __builtin_bit_cast(E, uint64_t(d[2 * j]) | (uint64_t(d[2 * j + 1]) << 32));
+    out[REV ? N - 1 - j : j] = v;
+  }
+}
+
+// element lo + j <- v[j] (REV: lo + N - 1 - j <- v[j])
+template <typename T, int N, bool REV>
+__device__ __forceinline__ void store_run(const T (&v)[N], Rsrc r, uint32_t lo_off) {
+  using V4 = unsigned int __attribute__((ext_vector_type(4)));
+  constexpr int W = int(sizeof(T) / 4), NDW = N * W;
+  static_assert(NDW % 4 == 0, "whole 16-byte stores");
+  uint32_t d[NDW];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const T x = v[REV ? N - 1 - j : j];
+    if constexpr (W == 1) {
+      d[j] = __builtin_bit_cast(uint32_t, x);
+    } else {
+      const uint64_t u = __builtin_bit_cast(uint64_t, x);
+      d[2 * j] = uint32_t(u);
+      d[2 * j + 1] = uint32_t(u >> 32);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NDW / 4; ++i) {
+    const V4 q = {d[4 * i], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]};
+    __builtin_amdgcn_raw_buffer_store_b128(q, r, lo_off + 16u * i, 0u, 0);
+  }
+}
+
 // The per-cell array a valuator reads: coefficients or material ids.
 template <typename T, int KIND>
 __device__ __forceinline__ const typename CellInfo<T, KIND>::Raw* cell_ptr(const StencilDesc& s) {
@@ -184,6 +255,11 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   // halo lines: the +x line for a = 7, the -x line for a = 0 (+y / -y for b)
   const bool hx_on = a == CB - 1 || a == 0, hy_on = b == CB - 1 || b == 0;
   const int32_t hxo = a == CB - 1 ? hsx : -hsx, hyo = b == CB - 1 ? hsy : -hsy;
+  // the halo lines a lane's steps read
+  const bool hx_need = (a == 0 && has_px) || (a == CB - 1 && has_sx);
+  const bool hy_need = (b == 0 && has_py) || (b == CB - 1 && has_sy);
+  // 16-byte runs for dword elements (the u8 lookup ids keep element loads)
+  constexpr bool RUNS = sizeof(Raw) >= 4;
   // byte offset of element e, clamped into the resource: the clamp also keeps
   // the compiler from splitting a constant out of the offset into the
   // instruction's immediate, which the hardware would add without wrapping
@@ -213,22 +289,58 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
     // ---- chunk windows: unconditional loads, one round trip ----
     Raw cw[K + 2], hx[K + 2], hy[K + 2];
     T xw[K + 1], exx[K + 1], exy[K + 1], bw[K];
-#pragma unroll
-    for (int j = 0; j < K + 2; ++j) {
-      const int32_t e = zorg + dz * (z0 - 1 + j);
-      cw[j] = rload<Raw>(rc, boff(e, SR));
-      hx[j] = rload<Raw>(rc, hx_on ? boff(e + hxo, SR) : kOff);
-      hy[j] = rload<Raw>(rc, hy_on ? boff(e + hyo, SR) : kOff);
+    // runs: the lowest element of each window (forward: the first step's;
+    // backward: the last step's), for 16-byte loads when every window a lane
+    // uses starts inside its resource
+    const int32_t zc = FWD ? zorg + z0 - 1 : zorg - z0 - K;      // cw: K + 2 elements
+    const int32_t zx = FWD ? zorg + z0 : zorg - z0 - K;          // xw: K + 1
+    const int32_t zb = FWD ? zorg + z0 : zorg - z0 - (K - 1);    // bw: K
+    bool runs = false;
+    if constexpr (RUNS) {
+      auto out = [emax](int32_t lo, int n) { return lo < 0 || lo + n - 1 > emax; };
+      const bool bad = line && (out(zc, K + 2) || out(zb, K) || (HAS_X && out(zx, K + 1)) ||
+                                (hx_need && out(zc + hxo, K + 2)) ||
+                                (hy_need && out(zc + hyo, K + 2)) ||
+                                (HAS_X && give_x && has_sx && out(zx + hsx, K + 1)) ||
+                                (HAS_X && give_y && has_sy && out(zx + hsy, K + 1)));
+      runs = __builtin_amdgcn_ballot_w64(bad) == 0;
     }
+    if (runs) {
+      if constexpr (RUNS) {
+        load_run<Raw, K + 2, !FWD>(rc, line ? uint32_t(zc) * SR : kRunOff, cw);
+        load_run<Raw, K + 2, !FWD>(rc, line && hx_need ? uint32_t(zc + hxo) * SR : kRunOff, hx);
+        load_run<Raw, K + 2, !FWD>(rc, line && hy_need ? uint32_t(zc + hyo) * SR : kRunOff, hy);
+        load_run<T, K, !FWD>(rbv, line ? uint32_t(zb) * ST : kRunOff, bw);
+        if constexpr (HAS_X) {
+          load_run<T, K + 1, !FWD>(rxi, line ? uint32_t(zx) * ST : kRunOff, xw);
+          load_run<T, K + 1, !FWD>(rxi, line && give_x && has_sx ? uint32_t(zx + hsx) * ST : kRunOff, exx);
+          load_run<T, K + 1, !FWD>(rxi, line && give_y && has_sy ? uint32_t(zx + hsy) * ST : kRunOff, exy);
+        } else {
 #pragma unroll
-    for (int j = 0; j < K + 1; ++j) {
-      const int32_t e = zorg + dz * (z0 + j);
-      xw[j] = HAS_X ? rload<T>(rxi, boff(e, ST)) : T(0);
-      exx[j] = HAS_X ? rload<T>(rxi, give_x ? boff(e + hsx, ST) : kOff) : T(0);
-      exy[j] = HAS_X ? rload<T>(rxi, give_y ? boff(e + hsy, ST) : kOff) : T(0);
+          for (int j = 0; j < K + 1; ++j) xw[j] = exx[j] = exy[j] = T(0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < K + 2; ++j) {
+        const int32_t e = zorg + dz * (z0 - 1 + j);
+        cw[j] = rload<Raw>(rc, boff(e, SR));
+        hx[j] = rload<Raw>(rc, hx_on ? boff(e + hxo, SR) : kOff);
+        hy[j] = rload<Raw>(rc, hy_on ? boff(e + hyo, SR) : kOff);
+      }
+#pragma unroll
+      for (int j = 0; j < K + 1; ++j) {
+        const int32_t e = zorg + dz * (z0 + j);
+        xw[j] = HAS_X ? rload<T>(rxi, boff(e, ST)) : T(0);
+        exx[j] = HAS_X ? rload<T>(rxi, give_x ? boff(e + hsx, ST) : kOff) : T(0);
+        exy[j] = HAS_X ? rload<T>(rxi, give_y ? boff(e + hsy, ST) : kOff) : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < K; ++j) bw[j] = rload<T>(rbv, boff(zorg + dz * (z0 + j), ST));
     }
-#pragma unroll
-    for (int j = 0; j < K; ++j) bw[j] = rload<T>(rbv, boff(zorg + dz * (z0 + j), ST));
+    // the chunk's results leave as one run when every line's K steps are valid
+    const bool run_store = RUNS && __builtin_amdgcn_ballot_w64(line && (z0 < 0 || z0 + K > nz)) == 0;
+    T xo[K];
     // ---- new border values of the upstream columns: prefetched from their
     // mailboxes now (every lane reads a word of its upstream line, clamped
     // into it, and lanes that take none discard it), polled again at the step
@@ -343,7 +455,8 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
         const T quot = acc / d;
         xn = fma(omega, quot, m1w * xi);
       }
-      rstore<T>(xn, rxo, valid ? boff(zorg + dz * lz, ST) : kOff);
+      xo[k] = xn;
+      if (!run_store) rstore<T>(xn, rxo, valid ? boff(zorg + dz * lz, ST) : kOff);
       // border lines publish their values to the downstream columns
       if (valid && give_x) __hip_atomic_store(out_x + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (valid && give_y) __hip_atomic_store(out_y + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -353,6 +466,9 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
       wpx_prev = w_sx;
       wpy_prev = w_sy;
       wpz_prev = w_sz;
+    }
+    if constexpr (RUNS) {
+      if (run_store) store_run<T, K, !FWD>(xo, rxo, line ? uint32_t(zb) * ST : kRunOff);
     }
   }
 }
