@@ -35,10 +35,13 @@ namespace {
 constexpr int CB = 8;        // lines per column side
 // steps per chunk (one window load and one mailbox prefetch each): shorter
 // chunks let a column follow its upstream columns more closely, longer ones
-// amortise the loads; 8 measured best at 256^3 for f32 (4, 16 and 32 slower)
-// and for f64 (4 slower)
+// amortise the loads.  With 16-byte window runs, 4 measured best at 256^3 for
+// f32 (the forward sweep from zero 7% faster than 8; 16 slower) and 8 for f64
+// (4 and 16 slower)
 template <typename T>
 constexpr int kChunk = 8;
+template <>
+constexpr int kChunk<float> = 4;
 constexpr int SPIN_LIMIT = 1 << 24;
 
 // Mailbox words start as a NaN with a payload that no arithmetic produces
