@@ -11,11 +11,11 @@
 // is the +x / +y / +z weight its predecessor computed one step earlier (the
 // harmonic mean is symmetric bit for bit), so each cell evaluates three faces.
 //
-// Steps run in chunks of K.  At the start of a chunk every lane loads the
-// coefficient, right-hand-side and old-value windows of its own line for the
-// chunk's K cells (lanes on the column border also load the neighbouring
-// column's lines), so the K steps run from registers and shuffles with one
-// memory round trip per chunk.  The lines on a column's downstream borders
+// Steps run in chunks of K.  Every lane loads the coefficient,
+// right-hand-side and old-value windows of its own line for a chunk's K cells
+// (lanes on the column border also load the neighbouring column's lines) as
+// 16-byte runs, one chunk ahead, so the K steps run from registers and
+// shuffles while the next chunk's windows are in flight.  The lines on a column's downstream borders
 // publish every new value into a mailbox (a word initialised to a sentinel NaN
 // and written once with a single-copy-atomic store); the border lines of the
 // downstream columns (I+1, J) and (I, J+1) poll those words, so no memory
@@ -287,11 +287,14 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   const bool take_y = line && b == 0 && has_py;
   const bool give_x = a == CB - 1, give_y = b == CB - 1;
   T prev = T(0), wpx_prev = T(0), wpy_prev = T(0), wpz_prev = T(0);
-  for (int64_t t0 = 0; t0 < nsteps; t0 += K) {
-    const int32_t z0 = int32_t(t0) - a - b;
-    // ---- chunk windows: unconditional loads, one round trip ----
-    Raw cw[K + 2], hx[K + 2], hy[K + 2];
-    T xw[K + 1], exx[K + 1], exy[K + 1], bw[K];
+  // a chunk's windows; the next chunk's are issued while the current one's
+  // steps run (after its mailbox prefetch, so waiting for those words does
+  // not wait for the windows).  PF = false loads them at the chunk start.
+  constexpr bool PF = true;
+  auto load_windows = [&](Raw(&w_cw)[K + 2], Raw(&w_hx)[K + 2], Raw(&w_hy)[K + 2],
+                          T(&w_xw)[K + 1], T(&w_exx)[K + 1], T(&w_exy)[K + 1], T(&w_bw)[K],
+                          int64_t tc) {
+    const int32_t z0 = int32_t(tc) - a - b;
     // runs: the lowest element of each window (forward: the first step's;
     // backward: the last step's), for 16-byte loads when every window a lane
     // uses starts inside its resource
@@ -310,37 +313,65 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
     }
     if (runs) {
       if constexpr (RUNS) {
-        load_run<Raw, K + 2, !FWD>(rc, line ? uint32_t(zc) * SR : kRunOff, cw);
-        load_run<Raw, K + 2, !FWD>(rc, line && hx_need ? uint32_t(zc + hxo) * SR : kRunOff, hx);
-        load_run<Raw, K + 2, !FWD>(rc, line && hy_need ? uint32_t(zc + hyo) * SR : kRunOff, hy);
-        load_run<T, K, !FWD>(rbv, line ? uint32_t(zb) * ST : kRunOff, bw);
+        load_run<Raw, K + 2, !FWD>(rc, line ? uint32_t(zc) * SR : kRunOff, w_cw);
+        load_run<Raw, K + 2, !FWD>(rc, line && hx_need ? uint32_t(zc + hxo) * SR : kRunOff, w_hx);
+        load_run<Raw, K + 2, !FWD>(rc, line && hy_need ? uint32_t(zc + hyo) * SR : kRunOff, w_hy);
+        load_run<T, K, !FWD>(rbv, line ? uint32_t(zb) * ST : kRunOff, w_bw);
         if constexpr (HAS_X) {
-          load_run<T, K + 1, !FWD>(rxi, line ? uint32_t(zx) * ST : kRunOff, xw);
-          load_run<T, K + 1, !FWD>(rxi, line && give_x && has_sx ? uint32_t(zx + hsx) * ST : kRunOff, exx);
-          load_run<T, K + 1, !FWD>(rxi, line && give_y && has_sy ? uint32_t(zx + hsy) * ST : kRunOff, exy);
+          load_run<T, K + 1, !FWD>(rxi, line ? uint32_t(zx) * ST : kRunOff, w_xw);
+          load_run<T, K + 1, !FWD>(rxi, line && give_x && has_sx ? uint32_t(zx + hsx) * ST : kRunOff, w_exx);
+          load_run<T, K + 1, !FWD>(rxi, line && give_y && has_sy ? uint32_t(zx + hsy) * ST : kRunOff, w_exy);
         } else {
 #pragma unroll
-          for (int j = 0; j < K + 1; ++j) xw[j] = exx[j] = exy[j] = T(0);
+          for (int j = 0; j < K + 1; ++j) w_xw[j] = w_exx[j] = w_exy[j] = T(0);
         }
       }
     } else {
 #pragma unroll
       for (int j = 0; j < K + 2; ++j) {
         const int32_t e = zorg + dz * (z0 - 1 + j);
-        cw[j] = rload<Raw>(rc, boff(e, SR));
-        hx[j] = rload<Raw>(rc, hx_on ? boff(e + hxo, SR) : kOff);
-        hy[j] = rload<Raw>(rc, hy_on ? boff(e + hyo, SR) : kOff);
+        w_cw[j] = rload<Raw>(rc, boff(e, SR));
+        w_hx[j] = rload<Raw>(rc, hx_on ? boff(e + hxo, SR) : kOff);
+        w_hy[j] = rload<Raw>(rc, hy_on ? boff(e + hyo, SR) : kOff);
       }
 #pragma unroll
       for (int j = 0; j < K + 1; ++j) {
         const int32_t e = zorg + dz * (z0 + j);
-        xw[j] = HAS_X ? rload<T>(rxi, boff(e, ST)) : T(0);
-        exx[j] = HAS_X ? rload<T>(rxi, give_x ? boff(e + hsx, ST) : kOff) : T(0);
-        exy[j] = HAS_X ? rload<T>(rxi, give_y ? boff(e + hsy, ST) : kOff) : T(0);
+        w_xw[j] = HAS_X ? rload<T>(rxi, boff(e, ST)) : T(0);
+        w_exx[j] = HAS_X ? rload<T>(rxi, give_x ? boff(e + hsx, ST) : kOff) : T(0);
+        w_exy[j] = HAS_X ? rload<T>(rxi, give_y ? boff(e + hsy, ST) : kOff) : T(0);
       }
 #pragma unroll
-      for (int j = 0; j < K; ++j) bw[j] = rload<T>(rbv, boff(zorg + dz * (z0 + j), ST));
+      for (int j = 0; j < K; ++j) w_bw[j] = rload<T>(rbv, boff(zorg + dz * (z0 + j), ST));
     }
+  };
+  // the next chunk's windows (PF)
+  Raw pcw[K + 2], phx[K + 2], phy[K + 2];
+  T pxw[K + 1], pexx[K + 1], pexy[K + 1], pbw[K];
+  if constexpr (PF) load_windows(pcw, phx, phy, pxw, pexx, pexy, pbw, 0);
+  for (int64_t t0 = 0; t0 < nsteps; t0 += K) {
+    Raw cw[K + 2], hx[K + 2], hy[K + 2];
+    T xw[K + 1], exx[K + 1], exy[K + 1], bw[K];
+    if constexpr (PF) {
+#pragma unroll
+      for (int j = 0; j < K + 2; ++j) {
+        cw[j] = pcw[j];
+        hx[j] = phx[j];
+        hy[j] = phy[j];
+      }
+#pragma unroll
+      for (int j = 0; j < K + 1; ++j) {
+        xw[j] = pxw[j];
+        exx[j] = pexx[j];
+        exy[j] = pexy[j];
+      }
+#pragma unroll
+      for (int j = 0; j < K; ++j) bw[j] = pbw[j];
+    } else {
+      load_windows(cw, hx, hy, xw, exx, exy, bw, t0);
+    }
+    const int32_t z0 = int32_t(t0) - a - b;
+    const int32_t zb = FWD ? zorg + z0 : zorg - z0 - (K - 1);  // lowest result element
     // the chunk's results leave as one run when every line's K steps are valid
     const bool run_store = RUNS && __builtin_amdgcn_ballot_w64(line && (z0 < 0 || z0 + K > nz)) == 0;
     T xo[K];
@@ -358,6 +389,9 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
       const T vy = __hip_atomic_load(in_y + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       nwx[j] = take_x && ok ? vx : T(0);
       nwy[j] = take_y && ok ? vy : T(0);
+    }
+    if constexpr (PF) {
+      if (t0 + K < nsteps) load_windows(pcw, phx, phy, pxw, pexx, pexy, pbw, t0 + K);
     }
     // ---- K steps from registers ----
 #pragma unroll
