@@ -136,34 +136,36 @@ __device__ __forceinline__ void rstore(T v, Rsrc r, uint32_t off) {
 // (2^31 and the run's length stay beyond any column resource, which holds
 // fewer than 2^31 bytes, with or without 32-bit wrap-around).
 constexpr uint32_t kRunOff = 0x80000000u;
-template <int NDW>
+// POL: cache policy bits of the loads (16: sc1, the coherence that relaxed
+// agent-scope atomic loads use)
+template <int NDW, int POL = 0>
 __device__ __forceinline__ void load_dwords(Rsrc r, uint32_t off, uint32_t (&d)[NDW]) {
   using V4 = unsigned int __attribute__((ext_vector_type(4)));
   using V2 = unsigned int __attribute__((ext_vector_type(2)));
   constexpr int n4 = NDW / 4, rem = NDW % 4;
 #pragma unroll
   for (int i = 0; i < n4; ++i) {
-    const V4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u * i, 0u, 0);
+    const V4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u * i, 0u, POL);
     d[4 * i] = v[0];
     d[4 * i + 1] = v[1];
     d[4 * i + 2] = v[2];
     d[4 * i + 3] = v[3];
   }
   if constexpr (rem >= 2) {
-    const V2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16u * n4, 0u, 0);
+    const V2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16u * n4, 0u, POL);
     d[4 * n4] = v[0];
     d[4 * n4 + 1] = v[1];
   }
-  if constexpr (rem % 2) d[NDW - 1] = __builtin_amdgcn_raw_buffer_load_b32(r, off + 4u * (NDW - 1), 0u, 0);
+  if constexpr (rem % 2) d[NDW - 1] = __builtin_amdgcn_raw_buffer_load_b32(r, off + 4u * (NDW - 1), 0u, POL);
 }
 
 // out[j] = element lo + j (REV: lo + N - 1 - j)
-template <typename E, int N, bool REV>
+template <typename E, int N, bool REV, int POL = 0>
 __device__ __forceinline__ void load_run(Rsrc r, uint32_t lo_off, E (&out)[N]) {
   static_assert(sizeof(E) == 4 || sizeof(E) == 8, "runs of dword elements");
   constexpr int W = int(sizeof(E) / 4);
   uint32_t d[N * W];
-  load_dwords<N * W>(r, lo_off, d);
+  load_dwords<N * W, POL>(r, lo_off, d);
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     E v;
@@ -286,6 +288,8 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   const bool take_x = line && a == 0 && has_px;
   const bool take_y = line && b == 0 && has_py;
   const bool give_x = a == CB - 1, give_y = b == CB - 1;
+  // the whole mailbox (both parts) as a resource, for the prefetch runs
+  const Rsrc rmb = col_rsrc(static_cast<const T*>(mbox), 0, 2 * ncols * CB * nz);
   T prev = T(0), wpx_prev = T(0), wpy_prev = T(0), wpz_prev = T(0);
   // a chunk's windows; the next chunk's are issued while the current one's
   // steps run (after its mailbox prefetch, so waiting for those words does
@@ -379,16 +383,28 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
     // mailboxes now (every lane reads a word of its upstream line, clamped
     // into it, and lanes that take none discard it), polled again at the step
     // that needs them if still unset ----
+    // f32 words inside every taking line's range come as 16-byte coherent
+    // runs (each 4-byte word is read whole; a word still unset reads as the
+    // sentinel and is polled at its step)
     T nwx[K], nwy[K];
+    const bool mruns = sizeof(T) == 4 &&
+                       __builtin_amdgcn_ballot_w64((take_x || take_y) && (z0 < 0 || z0 + K > nz)) == 0;
+    if (mruns) {
+      if constexpr (sizeof(T) == 4) {
+        load_run<T, K, false, 16>(rmb, take_x ? uint32_t(in_x - mbox + z0) * ST : kRunOff, nwx);
+        load_run<T, K, false, 16>(rmb, take_y ? uint32_t(in_y - mbox + z0) * ST : kRunOff, nwy);
+      }
+    } else {
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const int32_t lz = z0 + j;
-      const bool ok = lz >= 0 && lz < nz;
-      const int32_t cz = lz < 0 ? 0 : (lz >= nz ? int32_t(nz - 1) : lz);
-      const T vx = __hip_atomic_load(in_x + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const T vy = __hip_atomic_load(in_y + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      nwx[j] = take_x && ok ? vx : T(0);
-      nwy[j] = take_y && ok ? vy : T(0);
+      for (int j = 0; j < K; ++j) {
+        const int32_t lz = z0 + j;
+        const bool ok = lz >= 0 && lz < nz;
+        const int32_t cz = lz < 0 ? 0 : (lz >= nz ? int32_t(nz - 1) : lz);
+        const T vx = __hip_atomic_load(in_x + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const T vy = __hip_atomic_load(in_y + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        nwx[j] = take_x && ok ? vx : T(0);
+        nwy[j] = take_y && ok ? vy : T(0);
+      }
     }
     if constexpr (PF) {
       if (t0 + K < nsteps) load_windows(pcw, phx, phy, pxw, pexx, pexy, pbw, t0 + K);
