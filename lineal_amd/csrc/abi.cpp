@@ -58,6 +58,7 @@ struct lgpu_ctx_s {
   int64_t partial_cap = 0;
   DevBuf scal;            // SC_COUNT device scalars
   double* host_scal = nullptr;  // pinned mirror
+  hipEvent_t scal_ev = nullptr;  // marks the latest mirror copy (post_scalars)
   // opt-in timing of kernel classes (lgpu_ctx_kernel_timing): start/stop
   // event pairs recorded around every launch of a class whose bit is set
   int ktime_mask = 0;
@@ -417,6 +418,14 @@ double read_scalar(lgpu_ctx ctx, int slot) {
   return ctx->host_scal[0];
 }
 
+// All device scalars into the pinned mirror, without waiting: work queued
+// after this call overlaps the copy and the host's wait_scalars().
+void post_scalars(lgpu_ctx ctx) {
+  LGPU_HIP_CHECK(hipMemcpyAsync(ctx->host_scal, ctx->scal.ptr, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LGPU_HIP_CHECK(hipEventRecord(ctx->scal_ev, ctx->stream));
+}
+void wait_scalars(lgpu_ctx ctx) { LGPU_HIP_CHECK(hipEventSynchronize(ctx->scal_ev)); }
+
 // r = b - A x (f64), ||r||^2 into scal[slot]
 void residual_norm(lgpu_op op, const double* b, const double* x, double* r, int slot) {
   lgpu_ctx ctx = op->ctx;
@@ -556,6 +565,7 @@ extern "C" int lgpu_ctx_create(int device, lgpu_ctx* out) {
   ctx->scal.alloc(SC_COUNT * sizeof(double));
   LGPU_HIP_CHECK(hipMemset(ctx->scal.ptr, 0, SC_COUNT * sizeof(double)));
   LGPU_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_scal), SC_COUNT * sizeof(double), hipHostMallocDefault));
+  LGPU_HIP_CHECK(hipEventCreateWithFlags(&ctx->scal_ev, hipEventDisableTiming));
   ctx->partials_for(4096);
   *out = ctx.release();
   API_END
@@ -570,6 +580,7 @@ extern "C" int lgpu_ctx_destroy(lgpu_ctx ctx) {
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
+  if (ctx->scal_ev) (void)hipEventDestroy(ctx->scal_ev);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   API_END
@@ -1292,12 +1303,19 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
   int it = 0, status = LGPU_NOT_CONVERGED;
   if (!std::isfinite(r0)) status = LGPU_NONFINITE;
   else if (r0 == 0.0) status = LGPU_OK;
-  while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
-    // conjugate-gradients/solver.hpp:75-115
+  // q = A p and <p, q> of an iteration; it reads p only, so it is queued
+  // before the host has read the previous iteration's residual norm
+  auto spmv_pq = [&] {
     const int64_t np = op_partials(op);
     double* part = ctx->partials_for(np);
     op_apply(op, SM_SPMV_DOT, false, p, nullptr, q, 0.0, part);
     reduce_partials(part, np, scal + SC_PQ, st);
+  };
+  bool pq_queued = false;
+  while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
+    // conjugate-gradients/solver.hpp:75-115
+    if (!pq_queued) spmv_pq();
+    pq_queued = false;
     // FP32 V-cycle: the update also rounds r into the cycle's input, and the
     // dot and the direction update read the cycle's f32 output directly (it
     // widens exactly), so no conversion pass runs in between
@@ -1333,11 +1351,19 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
     }
     std::swap(cur, nxt);
     ++it;
-    const double rn = std::sqrt(read_scalar(ctx, SC_RR));
+    // the next iteration's product runs while the host checks this one (a
+    // final extra product only overwrites q and the <p, q> slot)
+    post_scalars(ctx);
+    if (it < params->max_it) {
+      spmv_pq();
+      pq_queued = true;
+    }
+    wait_scalars(ctx);
+    const double rn = std::sqrt(ctx->host_scal[SC_RR]);
     if (res_hist) res_hist[it] = rn;
     if (!std::isfinite(rn)) status = LGPU_NONFINITE;
     else if (rn <= params->rel_tol * r0) status = LGPU_OK;
-    else if (read_scalar(ctx, cur) == 0.0) status = LGPU_BREAKDOWN;
+    else if (ctx->host_scal[cur] == 0.0) status = LGPU_BREAKDOWN;
   }
   if (iterations) *iterations = it;
   return finish(op, precon, status);

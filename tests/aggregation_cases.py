@@ -13,8 +13,8 @@ small graphs, traced by hand from its rules:
   neighbour with room, or else stays on its own.
 
 Every off-diagonal entry is -1 (one case has a weak coupling) and every
-diagonal the same, so every other edge is strong in both directions.  Each case lists the aggregates as the vertices in
-insertion order.
+diagonal the same, so every other edge is strong in both directions.  Each
+case lists the aggregates as the vertices in insertion order.
 """
 
 import numpy as np
