@@ -1400,14 +1400,18 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
   if (!std::isfinite(rn0)) status = LGPU_NONFINITE;
   else if (rn0 == 0.0) status = LGPU_OK;
   const int64_t nd = dot_partials(n);
+  // y = M p and v = A y of an iteration; they read p only, so they are
+  // queued before the host has read the previous iteration's residual norm
+  double* const y = precon ? px : p;
+  auto head = [&] {
+    if (precon) precondition(precon, p, px, op);
+    op_apply(op, SM_SPMV, false, y, nullptr, v, 0.0, nullptr);
+  };
+  bool head_queued = false;
   while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
     // bicgstab/solver.hpp:71-151
-    double* y = p;
-    if (precon) {
-      precondition(precon, p, px, op);
-      y = px;
-    }
-    op_apply(op, SM_SPMV, false, y, nullptr, v, 0.0, nullptr);
+    if (!head_queued) head();
+    head_queued = false;
     dot_into(ctx, r0, v, false, n, SC_R0V);
     axpy_scal(xv, y, scal, cur, SC_R0V, 1.0, n, st);
     double* pd = ctx->partials_for(nd);
@@ -1439,11 +1443,19 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
     bicg_p_update(p, r, v, scal, nxt, cur, n, st);
     std::swap(cur, nxt);
     ++it;
-    const double rn = std::sqrt(read_scalar(ctx, SC_RR));
+    // the next iteration's preconditioner and product run while the host
+    // checks this one (a final extra pair only overwrites px and v)
+    post_scalars(ctx);
+    if (it < params->max_it) {
+      head();
+      head_queued = true;
+    }
+    wait_scalars(ctx);
+    const double rn = std::sqrt(ctx->host_scal[SC_RR]);
     if (res_hist) res_hist[it] = rn;
     if (!std::isfinite(rn)) status = LGPU_NONFINITE;
     else if (rn <= params->rel_tol * rn0) status = LGPU_OK;
-    else if (read_scalar(ctx, cur) == 0.0) status = LGPU_BREAKDOWN;
+    else if (ctx->host_scal[cur] == 0.0) status = LGPU_BREAKDOWN;
   }
   if (iterations) *iterations = it;
   return finish(op, precon, status);
