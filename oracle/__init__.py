@@ -111,6 +111,7 @@ def lib():
             "or_dot_f64": (C.c_double, [f64p, f64p, C.c_int64]),
             "or_csr_from_stencil": (C.c_int64, [sp, C.c_int, i64p, i64p, i64p, f64p, f64p, i64p]),
             "or_amg_setup": (hp, [mp, C.POINTER(AmgParams)]),
+            "or_strength_graph": (C.c_int64, [mp, C.c_double, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]),
             "or_amg_free": (None, [hp]),
             "or_amg_level_num": (C.c_int, [hp]),
             "or_amg_level_size": (C.c_int64, [hp, C.c_int]),
@@ -310,6 +311,18 @@ def ssor(A, x_in, b, omega=1.0, ultra=False, iterations=1):
 
 def dot(a, b):
     return lib().or_dot_f64(_pf(a), _pf(b), a.size)
+
+
+def strength_graph(A, theta=1.0 / 3.0):
+    """Edge flags (bit 0 depends, bit 1 influences) of the off-diagonal
+    entries in row order and the isolated flag of every vertex."""
+    L = lib()
+    ne = L.or_strength_graph(C.byref(A.mat), theta, None, None)
+    props = np.zeros(max(ne, 1), np.uint8)
+    iso = np.zeros(max(A.N, 1), np.uint8)
+    u8 = C.POINTER(C.c_uint8)
+    L.or_strength_graph(C.byref(A.mat), theta, props.ctypes.data_as(u8), iso.ctypes.data_as(u8))
+    return props[:ne], iso[: A.N]
 
 
 def default_params(coarse_f32=False):

@@ -21,10 +21,18 @@
  *  - Aggregation and strength criterion: aggregates traced by hand from the
  *    grow / round / single-vertex rules on chains, grids, a pendant vertex and
  *    a weak coupling (tests/aggregation_cases.py: ranking keys, ties,
- *    insertion order, merge); invariants on stencil hierarchies (every vertex
- *    in one aggregate or isolated, sizes <= max, strong-edge rules); the
- *    reference's convergence thresholds.  Maps of large graphs equal the
- *    reference's by construction only: the reference cannot be built here.
+ *    insertion order, merge); the nonsymmetric criterion by a hand-traced
+ *    strength graph (reverse-entry weights, missing reverse entries, negative
+ *    weights, the head's diagonal, one-way and influences-only edges, an
+ *    isolated vertex) and nonsymmetric aggregates traced by hand (one-way
+ *    ranking in grow, two-way candidates displacing one-way ones, the round
+ *    rule counting dependencies only, the merge through an influences-only
+ *    coupling); invariants on stencil hierarchies (every vertex in one
+ *    aggregate or isolated, sizes <= max, strong-edge rules); the reference's
+ *    convergence thresholds.  Maps of large graphs equal the reference's by
+ *    construction only: the reference cannot be built here.
+ *  - Galerkin products of nonsymmetric (upwind advection) levels: dense
+ *    P^T A P on every level, with A_c != (P^T A^T P).
  *  - CG / BiCGStab / PCG-AMG: exact termination on tiny SPD systems, the
  *    reference's thresholds (test/shared-stencil.hpp:157-224), scipy solves.
  *  - Effective diffusivity: the reference's known answer for diagonal lines
@@ -327,6 +335,23 @@ static void criterion(graph_t* g, double theta_d) {
   }
   free(st);
   free(stf);
+}
+
+/* the strength graph of one level on its own (criterion.hpp:35-78): edge
+ * flags in row order (off-diagonal entries in ascending column), isolated
+ * flags per vertex; returns the edge count (props may be NULL to query it) */
+int64_t or_strength_graph(const or_mat* m, double theta, uint8_t* props, uint8_t* iso) {
+  graph_t g;
+  memset(&g, 0, sizeof g);
+  graph_build(m, &g);
+  const int64_t ne = g.eptr[g.n];
+  if (props) {
+    criterion(&g, theta);
+    memcpy(props, g.props, (size_t)ne);
+    memcpy(iso, g.iso, (size_t)g.n);
+  }
+  graph_free(&g);
+  return ne;
 }
 
 #define DEPENDS(p) (((p)&1u) != 0)

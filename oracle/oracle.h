@@ -127,6 +127,11 @@ typedef struct {
 typedef struct or_hierarchy or_hierarchy;
 
 or_hierarchy* or_amg_setup(const or_mat* finest, const or_amg_params* p);
+/* strength graph of one level (dependency/criterion.hpp:35-78 with
+ * graph/weight-transform.hpp:28-39): props[e] for the off-diagonal entries in
+ * row order (bit 0 depends, bit 1 influences), iso[i] for each vertex;
+ * returns the edge count (pass props == NULL to query it only) */
+int64_t or_strength_graph(const or_mat* m, double theta, uint8_t* props, uint8_t* iso);
 /* 1 if the coarsest level is solved by LU (nonsymmetric hierarchy) */
 int or_amg_is_lu(const or_hierarchy* h);
 /* dense row-wise LU without pivoting of a CSR matrix (unit lower factor) and

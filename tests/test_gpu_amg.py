@@ -287,3 +287,18 @@ def test_hierarchy_known_answers(case):
     M = la.MultiGridSolver(wright).instantiate(A)
     assert M.level_size(1) == len(aggregates)
     np.testing.assert_array_equal(M.aggregates(0), AC.expected_map(rp.size - 1, aggregates))
+
+
+@pytest.mark.parametrize("case", [c[0] for c in __import__("aggregation_cases").NONSYMMETRIC_CASES])
+def test_hierarchy_nonsymmetric_known_answers(case):
+    """The library's aggregation on the hand-traced nonsymmetric graphs of
+    aggregation_cases.py (one-way and influences-only couplings)."""
+    import aggregation_cases as AC
+
+    _, build, aggregates, _ = next(c for c in AC.NONSYMMETRIC_CASES if c[0] == case)
+    rp, cols, vals = build()
+    A = la.CsrMatrix.from_arrays(rp, cols, vals, symmetric=False)
+    wright = la.HierarchyWrightParams(max_level_num=2, min_coarsen_size=1, min_coarsen_factor=1.0)
+    M = la.MultiGridSolver(wright).instantiate(A)
+    assert M.level_size(1) == len(aggregates)
+    np.testing.assert_array_equal(M.aggregates(0), AC.expected_map(rp.size - 1, aggregates))

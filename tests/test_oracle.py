@@ -930,3 +930,61 @@ def test_aggregation_known_answers(case):
     assert [idx[ptr[a]:ptr[a + 1]].tolist() for a in range(len(aggregates))] == aggregates
     stats = H.stats(0)
     assert stats[0] == len(aggregates) and tuple(stats[1:4]) == counts and stats[4] == 0
+
+
+# ---------------------------------------------------------------------------
+# nonsymmetric strength graph, aggregation and Galerkin products
+
+
+def test_strength_criterion_nonsymmetric_known_flags():
+    """The strength graph of aggregation_cases.STRENGTH_CASE, traced by hand
+    (dependency/criterion.hpp:35-78): w_ji taken from the reverse entry, 0 for
+    a missing one or a negative weight, the head's diagonal, one-way and
+    influences-only edges, an isolated vertex."""
+    import aggregation_cases as AC
+
+    build, props, iso = AC.STRENGTH_CASE
+    rp, cols, vals = build()
+    got_props, got_iso = O.strength_graph(O.Csr(rp, cols, vals, symmetric=False))
+    np.testing.assert_array_equal(got_props, props)
+    np.testing.assert_array_equal(got_iso, iso)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in __import__("aggregation_cases").NONSYMMETRIC_CASES])
+def test_aggregation_nonsymmetric_known_answers(case):
+    """Hand-traced aggregates of nonsymmetric graphs (aggregation_cases.py):
+    the one-way ranking of grow, two-way candidates displacing one-way ones,
+    the round rule counting dependencies only, and the merge through an
+    influences-only coupling (aggregation/base.hpp:495-677)."""
+    import aggregation_cases as AC
+
+    _, build, aggregates, counts = next(c for c in AC.NONSYMMETRIC_CASES if c[0] == case)
+    rp, cols, vals = build()
+    A = O.Csr(rp, cols, vals, symmetric=False)
+    H = O.Hierarchy(A, O.AmgParams(2, 1, 1.0, 1.0 / 3.0, 1.0 / 3.0, 6, 9, 0))
+    assert H.level_num >= 2 and H.size(1) == len(aggregates)
+    np.testing.assert_array_equal(H.agg_map(0), AC.expected_map(rp.size - 1, aggregates))
+    ptr, idx = H.c2f(0)
+    assert [idx[ptr[a]:ptr[a + 1]].tolist() for a in range(len(aggregates))] == aggregates
+    stats = H.stats(0)
+    assert stats[0] == len(aggregates) and tuple(stats[1:4]) == counts and stats[4] == 0
+
+
+def test_galerkin_nonsymmetric_dense():
+    """P^T A P (transport/matrix-shared.hpp:26-291) on an upwind-advection
+    operator (A != A^T) against dense numpy products on every level; the coarse
+    matrices are themselves nonsymmetric, so a transposed operand fails."""
+    dims = (6, 5, 4)
+    c = problems.lognormal(dims, 1.0, 7)
+    v = 2.5
+    S = O.Stencil(dims, coeff=c, velocity=v)
+    H = O.Hierarchy(S, O.AmgParams(16, 2, 1.2, 1.0 / 3.0, 1.0 / 3.0, 6, 9, 0))
+    assert H.level_num >= 3 and H.is_lu
+    A, _ = dense_advection(dims, c, v)
+    for l in range(H.level_num - 1):
+        P = prolongator(H.agg_map(l), H.size(l + 1))
+        Ac = H.matrix(l + 1).dense()
+        np.testing.assert_allclose(Ac, P.T @ A @ P, rtol=1e-13, atol=1e-13)
+        if Ac.shape[0] > 1:
+            assert np.abs(Ac - P.T @ A.T @ P).max() > 1e-3 * np.abs(Ac).max()
+        A = Ac
