@@ -53,6 +53,17 @@ int lgpu_ctx_synchronize(lgpu_ctx ctx);
 /* HIP events on the context stream: elapsed milliseconds between start and stop */
 int lgpu_timer_start(lgpu_ctx ctx);
 int lgpu_timer_stop(lgpu_ctx ctx, double* ms);
+/* The context's sticky wait-error word (no counterpart in the reference:
+ * the reference's sweeps run on host threads and cannot time out).  The
+ * wavefront sweeps (SOR on a stencil, one-segment SOR on a CSR operator, and
+ * the V-cycle's sweeps) wait on each other through bounded waits; one that
+ * expires sets this word, after which every wait of the context's sweeps
+ * gives up at once, so that every sweep still ends.  The entry points that
+ * run such sweeps (lgpu_op_sor, lgpu_op_ssor, lgpu_amg_vcycle, the Krylov
+ * solves) then return LGPU_SYNC_TIMEOUT and clear the word; the output of a
+ * sweep whose waits gave up is undefined.  lgpu_ctx_set_wait_error sets the
+ * word from the host, for tests of that path. */
+int lgpu_ctx_set_wait_error(lgpu_ctx ctx);
 /* Kernel-class timing (no counterpart in the reference; used by the
  * benchmark to time the dominant kernel inside a timed solve).  Every launch
  * of a class whose bit (1 << class) is set in the mask is bracketed by HIP
@@ -149,7 +160,9 @@ int lgpu_op_jacobi(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double
 /* One SOR sweep x_in -> x_out in lexicographic order (one thread segment),
  * SorSweepSink (linear-solver/iterative-stationary/gauss-seidel-variant/sweep.hpp:134-200);
  * forward 1/0, ultra 0 (regular) / 1 (l1 variant); x_in may be NULL (= 0) but
- * must not alias x_out. */
+ * must not alias x_out.  Synchronises the context's stream to read the
+ * wait-error word (LGPU_SYNC_TIMEOUT, see lgpu_ctx_set_wait_error); so do
+ * lgpu_op_ssor and lgpu_amg_vcycle. */
 int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
                 int forward, int ultra);
 /* One SSOR step, SsorBroom::sweep

@@ -143,6 +143,7 @@ SIGNATURES = {
     "lgpu_cg_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
     "lgpu_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
     "lgpu_ctx_kernel_timing": (C.c_int, [_vp, C.c_int]),
+    "lgpu_ctx_set_wait_error": (C.c_int, [_vp]),
     "lgpu_ctx_kernel_time": (C.c_int, [_vp, C.c_int, _f64p, _i64p]),
     "lgpu_outlet_flux": (C.c_int, [_vp, _vp, _f64p]),
     "lgpu_outlet_flux_adjoint_rhs": (C.c_int, [_vp, _vp]),
@@ -212,6 +213,12 @@ class Context:
         ms = C.c_double()
         _check(lib().lgpu_timer_stop(self.h, C.byref(ms)))
         return ms.value
+
+    def set_wait_error(self):
+        """Set the sticky wait-error word: every bounded wait of the next
+        wavefront sweeps gives up at once (tests of the timeout path); the
+        entry point that ran them raises SYNC_TIMEOUT and clears the word."""
+        _check(lib().lgpu_ctx_set_wait_error(self.h))
 
     def kernel_timing(self, enable):
         """Bracket every launch of the chosen kernel classes with HIP events

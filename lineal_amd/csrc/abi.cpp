@@ -64,6 +64,22 @@ struct lgpu_ctx_s {
   int ktime_mask = 0;
   std::vector<hipEvent_t> kev[LGPU_KT_COUNT];
   size_t kused[LGPU_KT_COUNT] = {};
+  // Krylov work vectors (f64), kept between solves and grown on demand, so
+  // that a repeated solve allocates nothing
+  std::vector<DevBuf> work;
+  int64_t work_n = 0;
+  // the sticky wait-error word of the wavefront sweeps: an int in a scalar
+  // slot, so that every mirror copy of the scalars carries it
+  int* err() const { return reinterpret_cast<int*>(scal.as<double>() + SC_ERR); }
+  bool host_err() const { return reinterpret_cast<const int*>(host_scal + SC_ERR)[0] != 0; }
+  double* work_vec(int i, int64_t n) {
+    if (n > work_n) {
+      work.clear();
+      work_n = n;
+    }
+    while (int(work.size()) <= i) work.emplace_back(size_t(work_n) * sizeof(double));
+    return work[size_t(i)].as<double>();
+  }
   double* partials_for(int64_t need) {
     need *= 2;  // dual reductions
     if (need > partial_cap) {
@@ -370,13 +386,15 @@ int64_t op_partials(lgpu_op op) {
   return op->is_csr ? csr_partials(op->csr.desc()) : stencil_partials(op->desc, false);
 }
 
-void csr_sweep(CsrDev& a, const void* x_in, void* x_out, const void* b, double omega, bool fwd,
-               bool ultra, hipStream_t st) {
+void csr_sweep(lgpu_ctx ctx, CsrDev& a, const void* x_in, void* x_out, const void* b, double omega,
+               bool fwd, bool ultra) {
+  hipStream_t st = ctx->stream;
   if (++a.epoch == 0x7fffffff) {
     vec_fill_int(a.flags.as<int>(), a.n, 0, st);
     a.epoch = 1;
   }
-  csr_sor(a.desc(), x_in, x_out, b, omega, fwd, ultra, a.sched.as<int>(), a.flags.as<int>(), a.epoch, st);
+  csr_sor(a.desc(), x_in, x_out, b, omega, fwd, ultra, a.sched.as<int>(), ctx->err(),
+          a.flags.as<int>(), a.epoch, st);
 }
 
 void op_sor(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, double omega,
@@ -385,11 +403,12 @@ void op_sor(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, 
   if (op->is_csr) {
     LGPU_REQUIRE(op->csr.f32 == f32, "CSR value type and vector type differ");
     KTimed kt(op->ctx, LGPU_KT_CSR_SOR);
-    csr_sweep(op->csr, x_in, x_out, b, omega, fwd, ultra, st);
+    csr_sweep(op->ctx, op->csr, x_in, x_out, b, omega, fwd, ultra);
   } else {
     if (f32) op->ensure_f32();
     SorSched s;
     s.state = op->sor_state.as<int>();
+    s.err = op->ctx->err();
     s.order = op->sor_order.as<uint32_t>();
     s.ncols = op->sor_cols;
     s.mailbox = op->sor_mailbox.ptr;
@@ -406,10 +425,20 @@ int read_int(hipStream_t st, const int* dev) {
   return v;
 }
 
-bool op_sync_error(lgpu_op op) {
-  hipStream_t st = op->ctx->stream;
-  if (op->is_csr) return read_int(st, op->csr.sched.as<int>() + 1) != 0;
-  return read_int(st, op->sor_state.as<int>() + 1) != 0;
+// Reads the context's sticky wait-error word after the queued work and
+// clears it: true if a bounded wait of a wavefront sweep expired since the
+// last clear (the sweeps after it gave up their waits at once and their
+// output is undefined).
+bool take_wait_error(lgpu_ctx ctx) {
+  if (read_int(ctx->stream, ctx->err()) == 0) return false;
+  vec_fill_int(ctx->err(), 1, 0, ctx->stream);
+  LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return true;
+}
+
+int wait_timeout() {
+  set_error("a wavefront kernel's bounded wait expired");
+  return LGPU_SYNC_TIMEOUT;
 }
 
 double read_scalar(lgpu_ctx ctx, int slot) {
@@ -449,7 +478,7 @@ void coarse_sweep(lgpu_amg amg, CsrDev& a, const void* x_in, void* x_out, const 
   const int64_t seg = amg->params.coarse_segment_rows;
   KTimed kt(amg->ctx, LGPU_KT_CSR_SOR);
   if (seg > 0) csr_sor_segmented(a.desc(), x_in, x_out, b, omega, fwd, ultra, int(seg), amg->ctx->stream);
-  else csr_sweep(a, x_in, x_out, b, omega, fwd, ultra, amg->ctx->stream);
+  else csr_sweep(amg->ctx, a, x_in, x_out, b, omega, fwd, ultra);
 }
 
 // One V-cycle in precision T from zero: z = M r (multigrid/solver.hpp:56-176).
@@ -522,13 +551,6 @@ void precondition(lgpu_amg amg, const double* r, double* z, lgpu_op fine) {
   vec_convert(r, false, amg->r32.ptr, true, n, st);
   vcycle(amg, amg->r32.ptr, amg->z32.ptr, fine);
   vec_convert(amg->z32.ptr, true, z, false, n, st);
-}
-
-bool amg_sync_error(lgpu_amg amg, lgpu_op fine) {
-  bool e = op_sync_error(amg->finest) || (fine != amg->finest && op_sync_error(fine));
-  for (size_t l = 1; l < amg->lv.size(); ++l)
-    e = e || read_int(amg->ctx->stream, amg->lv[l].a.sched.as<int>() + 1) != 0;
-  return e;
 }
 
 }  // namespace
@@ -608,6 +630,13 @@ extern "C" int lgpu_timer_stop(lgpu_ctx ctx, double* ms) {
   float f = 0;
   LGPU_HIP_CHECK(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
   *ms = f;
+  API_END
+}
+
+extern "C" int lgpu_ctx_set_wait_error(lgpu_ctx ctx) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx, "ctx is NULL");
+  vec_fill_int(ctx->err(), 1, 1, ctx->stream);
   API_END
 }
 
@@ -1013,6 +1042,7 @@ extern "C" int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b
   }
   LGPU_REQUIRE(b->f32 == x_out->f32, "vectors must share a dtype");
   op_sor(op, x_out->f32, x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega, forward != 0, ultra != 0);
+  if (take_wait_error(op->ctx)) return wait_timeout();
   API_END
 }
 
@@ -1028,6 +1058,7 @@ extern "C" int lgpu_op_ssor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec 
   }
   LGPU_REQUIRE(b->f32 == x_out->f32, "vectors must share a dtype");
   ssor_sweep(op, x_out->f32, x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega, ultra != 0);
+  if (take_wait_error(op->ctx)) return wait_timeout();
   API_END
 }
 
@@ -1231,6 +1262,7 @@ extern "C" int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z) {
     LGPU_REQUIRE(r->ptr != z->ptr, "r and z must not alias");
     precondition(amg, static_cast<const double*>(r->ptr), static_cast<double*>(z->ptr),
                  amg->finest);
+    if (take_wait_error(amg->ctx)) return wait_timeout();
     return LGPU_OK;
   }
   check_vec(r, amg->lv[0].n, "r");
@@ -1243,6 +1275,7 @@ extern "C" int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z) {
     precondition(amg, static_cast<const double*>(r->ptr), static_cast<double*>(z->ptr),
                  amg->finest);
   }
+  if (take_wait_error(amg->ctx)) return wait_timeout();
   API_END
 }
 
@@ -1251,21 +1284,18 @@ extern "C" int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z) {
 
 namespace {
 
-struct Work {
-  std::vector<DevBuf> v;
-  Work(int k, int64_t n) {
-    for (int i = 0; i < k; ++i) v.emplace_back(size_t(n) * sizeof(double));
-  }
-  double* operator[](int i) { return v[size_t(i)].as<double>(); }
-};
-
 bool rere(int iteration, int every) { return every > 0 && iteration > 0 && iteration % every == 0; }
 
-int finish(lgpu_op op, lgpu_amg amg, int status) {
-  if (op_sync_error(op) || (amg && amg_sync_error(amg, op))) {
-    set_error("a wavefront kernel's bounded wait expired");
-    return LGPU_SYNC_TIMEOUT;
-  }
+// End of a Krylov solve.  Every convergence check reads the wait-error word
+// with the scalars, so an expiry in the work an iteration used has already
+// stopped the loop (LGPU_SYNC_TIMEOUT).  Work queued after the last check
+// is the next iteration's speculative head, whose results are discarded: an
+// expiry there only clears the word.  A solve that never reached a check
+// (a zero or non-finite initial residual) takes the word as it stands.
+int finish(lgpu_ctx ctx, int status, bool checked) {
+  const bool late = take_wait_error(ctx);
+  if (status == LGPU_SYNC_TIMEOUT) return wait_timeout();
+  if (late && !checked) return wait_timeout();
   return status;
 }
 
@@ -1284,8 +1314,7 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
   lgpu_ctx ctx = op->ctx;
   hipStream_t st = ctx->stream;
   double* scal = ctx->scal.as<double>();
-  Work w(3, n);
-  double *r = w[0], *p = w[1], *q = w[2];
+  double *r = ctx->work_vec(0, n), *p = ctx->work_vec(1, n), *q = ctx->work_vec(2, n);
   double* xv = static_cast<double*>(x->ptr);
   const double* bv = static_cast<const double*>(b->ptr);
   // conjugate-gradients/solver.hpp:58-71
@@ -1311,7 +1340,7 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
     op_apply(op, SM_SPMV_DOT, false, p, nullptr, q, 0.0, part);
     reduce_partials(part, np, scal + SC_PQ, st);
   };
-  bool pq_queued = false;
+  bool pq_queued = false, checked = false;
   while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
     // conjugate-gradients/solver.hpp:75-115
     if (!pq_queued) spmv_pq();
@@ -1359,14 +1388,16 @@ extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x
       pq_queued = true;
     }
     wait_scalars(ctx);
+    checked = true;
     const double rn = std::sqrt(ctx->host_scal[SC_RR]);
     if (res_hist) res_hist[it] = rn;
-    if (!std::isfinite(rn)) status = LGPU_NONFINITE;
+    if (ctx->host_err()) status = LGPU_SYNC_TIMEOUT;
+    else if (!std::isfinite(rn)) status = LGPU_NONFINITE;
     else if (rn <= params->rel_tol * r0) status = LGPU_OK;
     else if (ctx->host_scal[cur] == 0.0) status = LGPU_BREAKDOWN;
   }
   if (iterations) *iterations = it;
-  return finish(op, precon, status);
+  return finish(ctx, status, checked);
   API_END
 }
 
@@ -1384,8 +1415,8 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
   lgpu_ctx ctx = op->ctx;
   hipStream_t st = ctx->stream;
   double* scal = ctx->scal.as<double>();
-  Work w(6, n);
-  double *r = w[0], *r0 = w[1], *p = w[2], *v = w[3], *t = w[4], *px = w[5];
+  double *r = ctx->work_vec(0, n), *r0 = ctx->work_vec(1, n), *p = ctx->work_vec(2, n),
+         *v = ctx->work_vec(3, n), *t = ctx->work_vec(4, n), *px = ctx->work_vec(5, n);
   double* xv = static_cast<double*>(x->ptr);
   const double* bv = static_cast<const double*>(b->ptr);
   // bicgstab/solver.hpp:64-68
@@ -1407,7 +1438,7 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
     if (precon) precondition(precon, p, px, op);
     op_apply(op, SM_SPMV, false, y, nullptr, v, 0.0, nullptr);
   };
-  bool head_queued = false;
+  bool head_queued = false, checked = false;
   while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
     // bicgstab/solver.hpp:71-151
     if (!head_queued) head();
@@ -1451,14 +1482,16 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
       head_queued = true;
     }
     wait_scalars(ctx);
+    checked = true;
     const double rn = std::sqrt(ctx->host_scal[SC_RR]);
     if (res_hist) res_hist[it] = rn;
-    if (!std::isfinite(rn)) status = LGPU_NONFINITE;
+    if (ctx->host_err()) status = LGPU_SYNC_TIMEOUT;
+    else if (!std::isfinite(rn)) status = LGPU_NONFINITE;
     else if (rn <= params->rel_tol * rn0) status = LGPU_OK;
     else if (ctx->host_scal[cur] == 0.0) status = LGPU_BREAKDOWN;
   }
   if (iterations) *iterations = it;
-  return finish(op, precon, status);
+  return finish(ctx, status, checked);
   API_END
 }
 
@@ -1709,16 +1742,17 @@ extern "C" int lgpu_adjoint_grad(lgpu_op op, lgpu_amg precon, const lgpu_krylov_
                "p, lambda and grad must not alias");
   lgpu_ctx ctx = op->ctx;
   lgpu_vec_s rhs_v{};
-  DevBuf rhs(size_t(n) * sizeof(double));
+  // the right-hand sides live in a work vector the CG solves do not use
+  double* rhs = ctx->work_vec(6, n);
   rhs_v.ctx = ctx;
   rhs_v.n = n;
   rhs_v.f32 = false;
-  rhs_v.ptr = rhs.ptr;
+  rhs_v.ptr = rhs;
   int it[2] = {0, 0};
-  stencil_rhs(op->desc, false, rhs.ptr, ctx->stream);
+  stencil_rhs(op->desc, false, rhs, ctx->stream);
   int status = lgpu_cg_solve(op, precon, &rhs_v, p, params, &it[0], nullptr);
   if (status == LGPU_OK || status == LGPU_NOT_CONVERGED) {
-    outlet_flux_adjoint_rhs(op->desc, rhs.as<double>(), ctx->stream);
+    outlet_flux_adjoint_rhs(op->desc, rhs, ctx->stream);
     const int s2 = lgpu_cg_solve(op, precon, &rhs_v, lambda, params, &it[1], nullptr);
     if (status == LGPU_OK) status = s2;
     outlet_flux_gradient(op->desc, static_cast<const double*>(p->ptr),

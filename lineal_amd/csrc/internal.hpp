@@ -130,7 +130,8 @@ void stencil_rhs(const StencilDesc& s, bool f32, void* b, hipStream_t st);
 
 // lexicographic SOR sweep on the stencil (tile-column wavefront)
 struct SorSched {
-  int* state = nullptr;        // [0] ticket, [1] error (sticky)
+  int* state = nullptr;        // [0] ticket
+  int* err = nullptr;          // sticky wait-error word (the context's)
   const uint32_t* order = nullptr;  // ticket -> column id (forward topological order)
   int64_t ncols = 0;
   void* mailbox = nullptr;     // border-line values passed between columns
@@ -146,7 +147,7 @@ int64_t csr_partials(const CsrDesc& a);
 void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y, double omega,
                double* partials, hipStream_t st);
 void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
-             bool forward, bool ultra, int* sched, int* flags, int epoch, hipStream_t st);
+             bool forward, bool ultra, int* sched, int* err, int* flags, int epoch, hipStream_t st);
 // thread-segment SOR: segments of `seg` consecutive rows, 1 <= seg <= kMaxSegmentRows
 constexpr int kMaxSegmentRows = 64;
 void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
@@ -231,6 +232,8 @@ enum : int {
   SC_OMEGA = 8,
   SC_FLUX = 9,    // outlet flux
   SC_EFFDIFF = 10, // mid-plane filled count / flux sum
+  SC_ERR = 11,     // the int at this slot: sticky wait-error word of the
+                   // context's wavefront sweeps (copied with the scalars)
   SC_COUNT = 16,
 };
 // x += (num/den) p, r -= (num/den) q, partial += r^2; r32 (may be NULL)

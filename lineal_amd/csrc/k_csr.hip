@@ -156,7 +156,7 @@ __global__ __launch_bounds__(NT) void csr_sor_kernel(int64_t n, const int64_t* _
                                                      const V* __restrict__ va,
                                                      const T* __restrict__ xin, T* xout,
                                                      const T* __restrict__ b, T omega, int* sched,
-                                                     int* flags, int epoch) {
+                                                     int* err, int* flags, int epoch) {
   __shared__ int s_ticket;
   if (threadIdx.x == 0) s_ticket = atomicAdd(sched, 1);
   __syncthreads();
@@ -179,10 +179,10 @@ __global__ __launch_bounds__(NT) void csr_sor_kernel(int64_t n, const int64_t* _
       }
       if (!ready) {
         // after any expired wait every row proceeds, so the sweep still ends
-        if (__hip_atomic_load(sched + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
           ready = true;
         } else if (++spins > SPIN_LIMIT) {
-          __hip_atomic_store(sched + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ready = true;
         }
       }
@@ -768,31 +768,31 @@ void dispatch_apply(const CsrDesc& a, int mode, const void* x, const void* b, vo
 
 template <typename T, typename V, bool FWD, bool ULTRA>
 void launch_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
-                int* sched, int* flags, int epoch, hipStream_t st) {
+                int* sched, int* err, int* flags, int epoch, hipStream_t st) {
   const int64_t blocks = (a.n + NT - 1) / NT;
   if (x_in) {
     hipLaunchKernelGGL((csr_sor_kernel<T, V, FWD, ULTRA, true>), dim3(blocks), dim3(NT), 0, st,
                        a.n, a.rowptr, a.cols, static_cast<const V*>(a.vals),
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched, flags, epoch);
+                       static_cast<const T*>(b), T(omega), sched, err, flags, epoch);
   } else {
     hipLaunchKernelGGL((csr_sor_kernel<T, V, FWD, ULTRA, false>), dim3(blocks), dim3(NT), 0, st,
                        a.n, a.rowptr, a.cols, static_cast<const V*>(a.vals),
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched, flags, epoch);
+                       static_cast<const T*>(b), T(omega), sched, err, flags, epoch);
   }
   LGPU_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T, typename V>
 void dispatch_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
-                  bool fwd, bool ultra, int* sched, int* flags, int epoch, hipStream_t st) {
+                  bool fwd, bool ultra, int* sched, int* err, int* flags, int epoch, hipStream_t st) {
   if (fwd) {
-    if (ultra) launch_sor<T, V, true, true>(a, x_in, x_out, b, omega, sched, flags, epoch, st);
-    else launch_sor<T, V, true, false>(a, x_in, x_out, b, omega, sched, flags, epoch, st);
+    if (ultra) launch_sor<T, V, true, true>(a, x_in, x_out, b, omega, sched, err, flags, epoch, st);
+    else launch_sor<T, V, true, false>(a, x_in, x_out, b, omega, sched, err, flags, epoch, st);
   } else {
-    if (ultra) launch_sor<T, V, false, true>(a, x_in, x_out, b, omega, sched, flags, epoch, st);
-    else launch_sor<T, V, false, false>(a, x_in, x_out, b, omega, sched, flags, epoch, st);
+    if (ultra) launch_sor<T, V, false, true>(a, x_in, x_out, b, omega, sched, err, flags, epoch, st);
+    else launch_sor<T, V, false, false>(a, x_in, x_out, b, omega, sched, err, flags, epoch, st);
   }
 }
 
@@ -819,12 +819,12 @@ void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const vo
 }
 
 void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
-             bool forward, bool ultra, int* sched, int* flags, int epoch, hipStream_t st) {
+             bool forward, bool ultra, int* sched, int* err, int* flags, int epoch, hipStream_t st) {
   LGPU_REQUIRE(x_in != x_out, "SOR sweep needs distinct input and output vectors");
   if (a.n == 0) return;
   vec_fill_int(sched, 1, 0, st);
-  if (a.f32) dispatch_sor<float, float>(a, x_in, x_out, b, omega, forward, ultra, sched, flags, epoch, st);
-  else dispatch_sor<double, double>(a, x_in, x_out, b, omega, forward, ultra, sched, flags, epoch, st);
+  if (a.f32) dispatch_sor<float, float>(a, x_in, x_out, b, omega, forward, ultra, sched, err, flags, epoch, st);
+  else dispatch_sor<double, double>(a, x_in, x_out, b, omega, forward, ultra, sched, err, flags, epoch, st);
 }
 
 }  // namespace lgpu

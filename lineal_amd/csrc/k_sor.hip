@@ -69,15 +69,15 @@ __device__ __forceinline__ bool is_sentinel(T v) {
 // bounded wait for a mailbox word; an expired wait sets the sticky error word
 // and every later wait returns at once, so the sweep always ends
 template <typename T>
-__device__ __forceinline__ T mailbox_poll(const T* p, int* state) {
+__device__ __forceinline__ T mailbox_poll(const T* p, int* err) {
   T v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   int it = 0;
 #pragma clang loop unroll(disable)
   while (is_sentinel(v)) {
-    if (__hip_atomic_load(state + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
     __builtin_amdgcn_s_sleep(1);
     if (++it > SPIN_LIMIT) {
-      __hip_atomic_store(state + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       break;
     }
     v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -220,8 +220,8 @@ template <typename T, int KIND, bool FWD, bool ULTRA, bool HAS_X, bool FD>
 __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __restrict__ xin,
                                                     T* __restrict__ xout,
                                                     const T* __restrict__ bv, T omega,
-                                                    int* state, const uint32_t* order,
-                                                    int64_t cy, T* mbox) {
+                                                    int* state, int* err,
+                                                    const uint32_t* order, int64_t cy, T* mbox) {
   using CI = CellInfo<T, KIND>;
   using Raw = typename CI::Raw;
   constexpr int K = kChunk<T>;
@@ -427,8 +427,8 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
           if (take_x && okj && is_sentinel(nwx[j])) nwx[j] = vx;
           if (take_y && okj && is_sentinel(nwy[j])) nwy[j] = vy;
         }
-        if (take_x) nwx[k] = mailbox_poll(in_x + lz, state);
-        if (take_y) nwy[k] = mailbox_poll(in_y + lz, state);
+        if (take_x) nwx[k] = mailbox_poll(in_x + lz, err);
+        if (take_y) nwy[k] = mailbox_poll(in_y + lz, err);
       }
       // successor data of the neighbouring lanes (same logical z)
       // (lane +-1 by DPP whole-wave shifts, lane +-8 by permutes)
@@ -534,12 +534,12 @@ void launch5(const StencilDesc& s, const void* x_in, void* x_out, const void* b,
   if (x_in) {
     hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, true, FD>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched.state, sched.order, cy,
+                       static_cast<const T*>(b), T(omega), sched.state, sched.err, sched.order, cy,
                        static_cast<T*>(sched.mailbox));
   } else {
     hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, false, FD>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched.state, sched.order, cy,
+                       static_cast<const T*>(b), T(omega), sched.state, sched.err, sched.order, cy,
                        static_cast<T*>(sched.mailbox));
   }
   LGPU_HIP_CHECK(hipGetLastError());
@@ -594,8 +594,8 @@ void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, 
   // a column's ten planes are addressed with 32-bit byte offsets below 2^31
   LGPU_REQUIRE(10 * s.ny * s.nz * int64_t(f32 ? 4 : 8) < (int64_t(1) << 31),
                "SOR sweep: the y-z plane is too large for 32-bit column offsets");
-  // reset the ticket and the progress counters; state[1] (the error word) is
-  // sticky until the host clears it
+  // reset the ticket; the error word (sched.err) is sticky until the host
+  // clears it
   vec_fill_int(sched.state, 1, 0, st);
   const int64_t words = int64_t(sor_mailbox_bytes(s) / sizeof(double));
   const int64_t fw = f32 ? 2 * words : words;
