@@ -208,6 +208,20 @@ void lgpu_amg_default_params(lgpu_amg_params* p);
  * (direct/lu/decomposition.hpp:30-111) for a nonsymmetric one (upwind
  * advection, the BiCgStab preconditioner). */
 int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out);
+/* lgpu_op_ssor for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same sweeps and the same result bit for
+ * bit; the forward sweep's output, which lgpu_op_ssor keeps in a buffer the
+ * library allocates on the operator's first SSOR call, lives in a static
+ * per-GPU device scratch of 2^22 f64 (32 MiB), so operators with more than
+ * 4194304 rows are rejected (LGPU_INVALID_ARG).  Nothing is allocated and no
+ * kernel-class timing is recorded.  f32 sweeps on a per-cell coefficient
+ * stencil need the operator's f32 coefficient copy, which the first f32 pass
+ * of another entry point makes (otherwise LGPU_INVALID_ARG).  Every call on
+ * one GPU shares that GPU's scratch: calls that overlap on one GPU (other
+ * contexts or host threads) race.  Synchronises the stream like lgpu_op_ssor. */
+int lgpu_op_ssor_noalloc(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
+                         int ultra);
 /* SsorSolver<Real, FixedIterationManager, variant>{omega, IterMan{iterations}}
  * as a Krylov preconditioner (gauss-seidel-variant/solver.hpp:94-96,
  * broom.hpp:159-171; test/aux/iterative.hpp:143-165): a preconditioner handle

@@ -129,6 +129,7 @@ SIGNATURES = {
     "lgpu_op_sor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int]),
     "lgpu_op_sor_segmented": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _i64]),
     "lgpu_op_ssor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int]),
+    "lgpu_op_ssor_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int]),
     "lgpu_ssor_precon_create": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int, C.POINTER(_vp)]),
     "lgpu_amg_default_params": (None, [C.POINTER(_AmgParams)]),
     "lgpu_amg_create": (C.c_int, [_vp, C.POINTER(_AmgParams), C.POINTER(_vp)]),

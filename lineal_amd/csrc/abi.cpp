@@ -534,6 +534,29 @@ void ssor_sweep(lgpu_op op, bool f32, const void* x_in, void* x_out, const void*
   op_sor(op, f32, op->ssor_aux.ptr, x_out, b, omega, false, ultra);
 }
 
+// The same two sweeps with the intermediate vector in `aux` (device memory
+// of at least rows() elements), and no kernel-class timing (whose events are
+// created on demand): nothing here allocates.
+void ssor_sweep_into(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b,
+                     double omega, bool ultra, void* aux) {
+  hipStream_t st = op->ctx->stream;
+  for (int fwd = 1; fwd >= 0; --fwd) {
+    const void* in = fwd ? x_in : aux;
+    void* out = fwd ? aux : x_out;
+    if (op->is_csr) {
+      csr_sweep(op->ctx, op->csr, in, out, b, omega, fwd != 0, ultra);
+    } else {
+      SorSched s;
+      s.state = op->sor_state.as<int>();
+      s.err = op->ctx->err();
+      s.order = op->sor_order.as<uint32_t>();
+      s.ncols = op->sor_cols;
+      s.mailbox = op->sor_mailbox.ptr;
+      stencil_sor(op->desc, f32, in, out, b, omega, fwd != 0, ultra, s, st);
+    }
+  }
+}
+
 void precondition(lgpu_amg amg, const double* r, double* z, lgpu_op fine) {
   if (amg->ssor_iters > 0) {
     // DoubleSweepImpl with a fixed iteration count (broom.hpp:159-171), from zero
@@ -589,6 +612,7 @@ extern "C" int lgpu_ctx_create(int device, lgpu_ctx* out) {
   LGPU_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_scal), SC_COUNT * sizeof(double), hipHostMallocDefault));
   LGPU_HIP_CHECK(hipEventCreateWithFlags(&ctx->scal_ev, hipEventDisableTiming));
   ctx->partials_for(4096);
+  scratch_init(device);
   *out = ctx.release();
   API_END
 }
@@ -1059,6 +1083,40 @@ extern "C" int lgpu_op_ssor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec 
   LGPU_REQUIRE(b->f32 == x_out->f32, "vectors must share a dtype");
   ssor_sweep(op, x_out->f32, x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega, ultra != 0);
   if (take_wait_error(op->ctx)) return wait_timeout();
+  API_END
+}
+
+extern "C" int lgpu_op_ssor_noalloc(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b,
+                                     double omega, int ultra) {
+  API_BEGIN
+  LGPU_REQUIRE(op, "NULL operator");
+  check_vec(x_out, op->rows(), "x_out");
+  check_vec(b, op->rows(), "b");
+  if (x_in) {
+    check_vec(x_in, op->rows(), "x_in");
+    LGPU_REQUIRE(x_in->f32 == x_out->f32, "x_in and x_out must share a dtype");
+  }
+  LGPU_REQUIRE(b->f32 == x_out->f32, "vectors must share a dtype");
+  LGPU_REQUIRE(!op->is_csr || op->csr.f32 == x_out->f32, "CSR value type and vector type differ");
+  LGPU_REQUIRE(op->rows() <= kScratchDoubles,
+               "lgpu_op_ssor_noalloc: the operator has more rows than the static scratch holds "
+               "(4194304); use lgpu_op_ssor");
+  // the f32 coefficient copy of a per-cell stencil is made by the first f32
+  // pass; making it here would allocate
+  LGPU_REQUIRE(!x_out->f32 || op->is_csr || op->desc.kind != 0 || op->c32.ptr,
+               "lgpu_op_ssor_noalloc: f32 sweeps need the operator's f32 coefficient copy, "
+               "which the first f32 pass of another entry point makes");
+  lgpu_ctx ctx = op->ctx;
+  ssor_sweep_into(op, x_out->f32, x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega,
+                  ultra != 0, scratch_ptr(ctx->device));
+  // the wait-error word through the context's pinned mirror
+  LGPU_HIP_CHECK(hipMemcpyAsync(ctx->host_scal + SC_ERR, ctx->err(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (ctx->host_err()) {
+    vec_fill_int(ctx->err(), 1, 0, ctx->stream);
+    LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return wait_timeout();
+  }
   API_END
 }
 

@@ -194,6 +194,13 @@ void dvf_read(const char* path, std::vector<double>& v);
 void dvf_write(const char* path, const double* v, uint64_t n);
 }  // namespace io
 
+// Per-device static scratch of the allocation-free entry points: 2^22 f64
+// (32 MiB) per GPU.  scratch_init() runs once per device with that device
+// current (at context creation); scratch_ptr() is that device's address.
+constexpr int64_t kScratchDoubles = int64_t(1) << 22;
+void scratch_init(int device);
+double* scratch_ptr(int device);
+
 // vector kernels
 void vec_fill(void* x, int64_t n, bool f32, double v, hipStream_t st);
 void vec_fill_int(int* x, int64_t n, int v, hipStream_t st);
