@@ -122,6 +122,7 @@ def lib():
             "or_amg_cholesky": (cp, [hp]),
             "or_amg_stats": (None, [hp, C.c_int, i64p]),
             "or_amg_set_coarse_segment": (None, [hp, C.c_int64]),
+            "or_amg_set_cycle": (None, [hp, C.c_double, C.c_int, C.c_int]),
             "or_cholesky_solve": (None, [cp, f64p, f64p]),
             "or_cholesky_decompose": (C.c_int64, [cp, i64p, i64p, f64p]),
             "or_vcycle_f64": (None, [hp, f64p, f64p, C.c_int]),
@@ -331,13 +332,17 @@ def default_params(coarse_f32=False):
 
 
 class Hierarchy:
-    def __init__(self, A, params: AmgParams | None = None, coarse_f32=False, coarse_segment=0):
+    def __init__(self, A, params: AmgParams | None = None, coarse_f32=False, coarse_segment=0,
+                 relax=1.0, cycle_kind=0, cycle_iterations=1):
+        """relax, cycle_kind (0 REGULAR, 1 DECREASING) and cycle_iterations:
+        MultiGridSolver{relax, DefaultCycle{kind, iterations}}."""
         self.A = A  # keep the finest operator alive
         self.params = params or default_params(coarse_f32)
         self.h = lib().or_amg_setup(C.byref(A.mat), C.byref(self.params))
         if not self.h:
             raise ValueError("the coarsest factorisation failed")
         lib().or_amg_set_coarse_segment(self.h, int(coarse_segment))
+        lib().or_amg_set_cycle(self.h, float(relax), int(cycle_kind), int(cycle_iterations))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -64,6 +64,11 @@ struct or_hierarchy {
   double* lu_l;       /* unit lower factor, dense row major */
   double* lu_u;       /* upper factor, dense row major */
   int64_t coarse_seg; /* rows per smoother segment on levels >= 1 (0: one segment) */
+  /* MultiGridSolver{relax, DefaultCycle{kind, iterations}}: 0 in relax means
+   * 1 (the calloc'ed default), kind 0 REGULAR / 1 DECREASING, 0 in
+   * iterations means 1 */
+  double relax;
+  int cycle_kind, cycle_iters;
 };
 
 /* Thread segments of level l's smoother (parallel/execution-policy/adaptive.hpp
@@ -81,6 +86,11 @@ static int64_t* or_level_segments(const or_hierarchy* h, int l, int* nseg) {
 }
 
 void or_amg_set_coarse_segment(or_hierarchy* h, int64_t rows) { h->coarse_seg = rows; }
+void or_amg_set_cycle(or_hierarchy* h, double relax, int kind, int iterations) {
+  h->relax = relax;
+  h->cycle_kind = kind;
+  h->cycle_iters = iterations;
+}
 
 /* discretization/box-equidistant.hpp:62-79: h_d = L_d / n_d,
  * q_d = (prod_{e != d} h_e) / h_d with the product left-folded from 1 */

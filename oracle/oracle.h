@@ -157,6 +157,10 @@ void or_amg_stats(const or_hierarchy* h, int level, int64_t out[5]);
 
 /* rows per thread segment of the smoothers on levels >= 1 (0: one segment) */
 void or_amg_set_coarse_segment(or_hierarchy* h, int64_t rows);
+/* MultiGridSolver{relax, DefaultCycle{kind, iterations}} (multigrid/solver.hpp:183-186,
+ * multigrid/cycle.hpp:24-79): kind 0 REGULAR, 1 DECREASING; iterations >= 1.
+ * The default is relax 1 and DefaultCycle{REGULAR, 1} (a V-cycle). */
+void or_amg_set_cycle(or_hierarchy* h, double relax, int kind, int iterations);
 
 /* coarsest solve x = (L L^T)^{-1} b in f64 (direct/cholesky/solver.hpp:42-51) */
 void or_cholesky_solve(const or_csr* lower, const double* b, double* x);
@@ -164,7 +168,8 @@ void or_cholesky_solve(const or_csr* lower, const double* b, double* x);
  * sized by the caller (nnz <= n*(n+1)/2).  Returns nnz or -1 if not SPD. */
 int64_t or_cholesky_decompose(const or_csr* a, int64_t* rowptr, int64_t* cols, double* vals);
 
-/* one V-cycle z = M r (multigrid/solver.hpp:56-176), x0 = 0.  precision 64 or 32
+/* one cycle z = M r (multigrid/solver.hpp:56-176), x0 = 0, with the
+ * hierarchy's relax and cycle (a V-cycle by default).  precision 64 or 32
  * selects the vector/smoother precision (32: FP32 hierarchy, FP64 coarsest). */
 void or_vcycle_f64(const or_hierarchy* h, const double* r, double* z, int ultra);
 void or_vcycle_f32(const or_hierarchy* h, const float* r, float* z, int ultra);

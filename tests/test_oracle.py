@@ -988,3 +988,52 @@ def test_galerkin_nonsymmetric_dense():
         if Ac.shape[0] > 1:
             assert np.abs(Ac - P.T @ A.T @ P).max() > 1e-3 * np.abs(Ac).max()
         A = Ac
+
+
+# ---------------------------------------------------------------------------
+# MultiGridSolver{relax, DefaultCycle{kind, iterations}} (multigrid/solver.hpp:56-186)
+
+
+def dense_cycle(A, maps, sizes, r, relax, level=0):
+    """One cycle from zero on dense matrices: forward SOR, restriction of the
+    residual, recursion (a dense solve on the coarsest level), prolongation
+    scaled by 1/relax, backward SOR (the python reference's sor() semantics
+    for the sweeps, P^T A P for the coarse matrices)."""
+    if level == len(sizes) - 1:
+        return np.linalg.solve(A, r)
+    x = dense_sor(A, np.zeros(len(r)), r, 1.0, forward=True)
+    P = prolongator(maps[level], sizes[level + 1])
+    xc = dense_cycle(P.T @ A @ P, maps, sizes, P.T @ (r - A @ x), relax, level + 1)
+    x = x + (P @ xc) * (1.0 / relax)
+    return dense_sor(A, x, r, 1.0, forward=False)
+
+
+@pytest.mark.parametrize("relax", [1.0, 0.7, 1.6])
+def test_cycle_relax_against_dense_restatement(relax):
+    dims = (7, 6, 5)
+    S, c = lognormal_stencil(dims, seed=9)
+    H = O.Hierarchy(S, O.AmgParams(16, 2, 1.2, 1.0 / 3.0, 1.0 / 3.0, 6, 9, 0), relax=relax)
+    assert H.level_num >= 3
+    r = problems.uniform(S.N, 4)
+    sizes = [H.size(l) for l in range(H.level_num)]
+    maps = [H.agg_map(l) for l in range(H.level_num - 1)]
+    ref = dense_cycle(dense_operator(dims, c), maps, sizes, r, relax)
+    np.testing.assert_allclose(H.vcycle(r), ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    if relax != 1.0:
+        assert np.abs(H.vcycle(r) - O.Hierarchy(S, H.params).vcycle(r)).max() > 1e-6
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("iterations", [2, 3])
+def test_cycle_kinds_restart_revisited_levels(kind, iterations):
+    """A level's down step (pre-smoothing from zero, restriction) runs only at
+    the begin of its cycle info, and a level pushed again by its finer level
+    starts a new info: the repeated visits recompute the same coarse
+    correction from zero with an unchanged right-hand side, so every
+    DefaultCycle of either kind gives the V-cycle's result bit for bit."""
+    dims = (9, 8, 7)
+    S, _ = lognormal_stencil(dims, seed=2)
+    r = problems.uniform(S.N, 6)
+    v = O.Hierarchy(S, coarse_segment=64).vcycle(r)
+    w = O.Hierarchy(S, coarse_segment=64, cycle_kind=kind, cycle_iterations=iterations).vcycle(r)
+    np.testing.assert_array_equal(w, v)
