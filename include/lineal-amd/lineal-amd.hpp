@@ -551,11 +551,19 @@ struct HierarchyWrightParams {
   int64_t max_aggregate_size = 9;
 };
 
-// MultiGridSolver{relax = 1, DefaultCycle{REGULAR, 1}, wright} with SOR(1)
-// pre/post smoothers and a Cholesky coarsest solver (LU for a nonsymmetric
-// operator).  TLevelReal selects the
-// hierarchy's precision: float gives the mixed-precision preconditioner
-// (FP32 hierarchy, smoothers and level operators; FP64 coarsest solve).
+// DefaultCycle{kind, iterations} (multigrid/cycle.hpp:24-79)
+enum struct CycleKind : int { REGULAR = LGPU_CYCLE_REGULAR, DECREASING = LGPU_CYCLE_DECREASING };
+struct DefaultCycle {
+  CycleKind kind = CycleKind::REGULAR;
+  int iterations = 1;
+};
+
+// MultiGridSolver{relax, DefaultCycle{kind, iterations}, wright}
+// (multigrid/solver.hpp:183-186) with SOR(1) pre/post smoothers and a
+// Cholesky coarsest solver (LU for a nonsymmetric operator).  TLevelReal
+// selects the hierarchy's precision: float gives the mixed-precision
+// preconditioner (FP32 hierarchy, smoothers and level operators; FP64
+// coarsest solve).
 template <typename TLevelReal = double>
 class GpuMultiGridSolver {
  public:
@@ -563,6 +571,10 @@ class GpuMultiGridSolver {
                               SorVariant variant = SorVariant::REGULAR,
                               int64_t coarse_segment_rows = 64)
       : params_(params), variant_(variant), coarse_segment_rows_(coarse_segment_rows) {}
+  GpuMultiGridSolver(double relax, DefaultCycle cycle, HierarchyWrightParams params = {},
+                     SorVariant variant = SorVariant::REGULAR, int64_t coarse_segment_rows = 64)
+      : params_(params), variant_(variant), coarse_segment_rows_(coarse_segment_rows),
+        relax_(relax), cycle_(cycle) {}
 
   class Instance {
    public:
@@ -579,11 +591,14 @@ class GpuMultiGridSolver {
       p.precision = std::is_same_v<TLevelReal, float> ? LGPU_F32 : LGPU_F64;
       p.ultra = s.variant_ == SorVariant::ULTRA ? 1 : 0;
       p.coarse_segment_rows = s.coarse_segment_rows_;
+      p.relax = s.relax_;
+      p.cycle_kind = int(s.cycle_.kind);
+      p.cycle_iterations = s.cycle_.iterations;
       lgpu_amg h = nullptr;
       check(lgpu_amg_create(lhs.handle(), &p, &h));
       h_.reset(h);
     }
-    // sol = M rhs, one V-cycle from a constant-zero initial guess
+    // sol = M rhs, one cycle from a constant-zero initial guess
     // (multigrid/solver.hpp:47-53)
     void apply(GpuDenseVector<double>& sol, const GpuDenseVector<double>& rhs) const {
       check(lgpu_amg_vcycle(h_.get(), rhs.handle(), sol.handle()));
@@ -614,6 +629,8 @@ class GpuMultiGridSolver {
   HierarchyWrightParams params_;
   SorVariant variant_;
   int64_t coarse_segment_rows_;
+  double relax_ = 1.0;
+  DefaultCycle cycle_{};
 };
 
 // ---------------------------------------------------------------------------

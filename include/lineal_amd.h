@@ -196,7 +196,16 @@ typedef struct {
                                    (1..64, default 64); 0 sweeps each coarse level in one
                                    segment (lexicographic). The finest level is always
                                    swept lexicographically. */
+  /* MultiGridSolver{relax, DefaultCycle{cycle_kind, cycle_iterations}, wright}
+   * (multigrid/solver.hpp:183-186, multigrid/cycle.hpp:24-79): the coarse
+   * correction is added scaled by 1/relax (default 1); cycle_kind
+   * LGPU_CYCLE_REGULAR or LGPU_CYCLE_DECREASING, cycle_iterations >= 1
+   * (default REGULAR, 1: the V-cycle) */
+  double relax;
+  int cycle_kind;
+  int cycle_iterations;
 } lgpu_amg_params;
+enum { LGPU_CYCLE_REGULAR = 0, LGPU_CYCLE_DECREASING = 1 };
 
 void lgpu_amg_default_params(lgpu_amg_params* p);
 /* MultiGridSolver::instantiate -> HierarchyWrightInstance::create
@@ -245,8 +254,9 @@ int lgpu_amg_setup_seconds(lgpu_amg amg, double* seconds);
  * [3] coarsest factorisation, [4] the rest (row evaluation of the finest
  * operator, uploads). */
 int lgpu_amg_setup_split(lgpu_amg amg, double seconds[5]);
-/* z = M r, one V-cycle from zero (MultiGridSolver::Instance::apply with a
- * constant-zero in_sol, multigrid/solver.hpp:47-53); r and z are f64. */
+/* z = M r, one cycle from zero (MultiGridSolver::Instance::apply with a
+ * constant-zero in_sol, multigrid/solver.hpp:47-53); r and z are f64 (or
+ * both f32 with an f32 hierarchy). */
 int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z);
 
 /* ---- Krylov solvers ------------------------------------------------------- */

@@ -81,7 +81,13 @@ class _AmgParams(C.Structure):
         ("precision", C.c_int),
         ("ultra", C.c_int),
         ("coarse_segment_rows", C.c_int64),
+        ("relax", C.c_double),
+        ("cycle_kind", C.c_int),
+        ("cycle_iterations", C.c_int),
     ]
+
+
+CYCLE_REGULAR, CYCLE_DECREASING = 0, 1
 
 
 class _KrylovParams(C.Structure):
@@ -563,28 +569,34 @@ class HierarchyWrightParams:
     min_aggregate_size: int = 6
     max_aggregate_size: int = 9
 
-    def _struct(self, precision: int, ultra: bool, coarse_segment_rows: int):
+    def _struct(self, precision: int, ultra: bool, coarse_segment_rows: int, relax=1.0,
+                cycle=(CYCLE_REGULAR, 1)):
         return _AmgParams(self.max_level_num, self.min_coarsen_size, self.min_coarsen_factor,
                           self.strong_threshold, self.aggressive_strong_threshold,
                           self.min_aggregate_size, self.max_aggregate_size, precision, int(ultra),
-                          int(coarse_segment_rows))
+                          int(coarse_segment_rows), float(relax), int(cycle[0]), int(cycle[1]))
 
 
 class MultiGridSolver:
-    """MultiGridSolver{relax=1, DefaultCycle{REGULAR,1}, wright} (multigrid/solver.hpp:183-196).
+    """MultiGridSolver{relax, DefaultCycle{kind, iterations}, wright}
+    (multigrid/solver.hpp:183-196, multigrid/cycle.hpp:24-79).
 
-    precision: np.float64 for an FP64 V-cycle, np.float32 for the FP32 hierarchy,
+    precision: np.float64 for an FP64 cycle, np.float32 for the FP32 hierarchy,
     smoothers and level operators with an FP64 coarsest solve.
     coarse_segment_rows: rows per thread segment of the coarse-level smoothers
     (0: one lexicographic segment per level).
+    relax: the coarse correction is added scaled by 1/relax.
+    cycle: (CYCLE_REGULAR or CYCLE_DECREASING, iterations); (REGULAR, 1) is the V-cycle.
     """
 
     def __init__(self, wright: HierarchyWrightParams | None = None, precision=np.float64, ultra=False,
-                 coarse_segment_rows=64):
+                 coarse_segment_rows=64, relax=1.0, cycle=(CYCLE_REGULAR, 1)):
         self.wright = wright or HierarchyWrightParams()
         self.precision = precision
         self.ultra = ultra
         self.coarse_segment_rows = coarse_segment_rows
+        self.relax = relax
+        self.cycle = tuple(cycle)
 
     def instantiate(self, lhs: _Matrix) -> "MultiGridInstance":
         return MultiGridInstance(lhs, self)
@@ -593,7 +605,8 @@ class MultiGridSolver:
 class MultiGridInstance:
     def __init__(self, lhs: _Matrix, solver: MultiGridSolver):
         self.lhs = lhs
-        p = solver.wright._struct(_dtype_code(solver.precision), solver.ultra, solver.coarse_segment_rows)
+        p = solver.wright._struct(_dtype_code(solver.precision), solver.ultra, solver.coarse_segment_rows,
+                                  solver.relax, solver.cycle)
         h = _vp()
         _check(lib().lgpu_amg_create(lhs.h, C.byref(p), C.byref(h)))
         self.h = h

@@ -481,7 +481,14 @@ void coarse_sweep(lgpu_amg amg, CsrDev& a, const void* x_in, void* x_out, const 
   else csr_sweep(amg->ctx, a, x_in, x_out, b, omega, fwd, ultra);
 }
 
-// One V-cycle in precision T from zero: z = M r (multigrid/solver.hpp:56-176).
+// One multigrid cycle in the hierarchy's precision from zero: z = M r
+// (MultiGridSolver::Instance::apply_impl, multigrid/solver.hpp:56-176).  The
+// cycle is the reference's stack of DefaultCycleInfo {iter, iter_num}
+// (multigrid/cycle.hpp:24-79): a level runs its down step (forward SOR from
+// zero, residual, restriction) when iter == iter_num and its up step
+// (prolong-add scaled by 1/relax, backward SOR) when iter == 0; otherwise it
+// counts down and pushes its coarser level again (REGULAR: {iter_num},
+// DECREASING: {iter}).  DefaultCycle{REGULAR, 1} is the V-cycle.
 // `fine` smooths and forms the residual on the finest level: the hierarchy's
 // own finest operator, or a Krylov operator of the same size (the upwind
 // advection operator over the hierarchy of its diffusion part, so that the
@@ -491,36 +498,60 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine) {
   hipStream_t st = amg->ctx->stream;
   const int L = int(amg->lv.size());
   const bool ultra = amg->params.ultra != 0;
-  const double relax = 1.0, inv_relax = 1.0 / relax;
+  // the smoothers are SorSolver{1} (test/aux/default.hpp:156-186)
+  const double omega = 1.0;
+  const double inv_relax = 1.0 / amg->params.relax;
   auto xs = [&](int l) { return l == 0 ? z : amg->lv[size_t(l)].sol.ptr; };
   auto bs = [&](int l) { return l == 0 ? r : amg->lv[size_t(l)].rhs.ptr; };
-  for (int l = 0; l + 1 < L; ++l) {
+  auto down = [&](int l) {
     AmgLevel& cur = amg->lv[size_t(l)];
     if (l == 0) {
-      op_sor(fine, f32, nullptr, xs(0), bs(0), relax, true, ultra);
+      op_sor(fine, f32, nullptr, xs(0), bs(0), omega, true, ultra);
       op_apply(fine, SM_RESID, f32, xs(0), bs(0), cur.tmp.ptr, 0.0, nullptr);
     } else {
-      coarse_sweep(amg, cur.a, nullptr, xs(l), bs(l), relax, true, ultra);
+      coarse_sweep(amg, cur.a, nullptr, xs(l), bs(l), omega, true, ultra);
       KTimed kt(amg->ctx, LGPU_KT_CSR_PASS);
       csr_apply(cur.a.desc(), SM_RESID, xs(l), bs(l), cur.tmp.ptr, 0.0, nullptr, st);
     }
     const int64_t nc = amg->lv[size_t(l + 1)].n;
     restrict_sum(cur.tmp.ptr, f32, cur.c2f_ptr.as<int64_t>(), cur.c2f_idx.as<int32_t>(), nc,
                  cur.c2f_blk, amg->lv[size_t(l + 1)].rhs.ptr, st);
-  }
-  {
+  };
+  auto up = [&](int l) {
+    AmgLevel& cur = amg->lv[size_t(l)];
+    prolong_add(xs(l), amg->lv[size_t(l + 1)].sol.ptr, f32, cur.agg.as<uint32_t>(), cur.n,
+                inv_relax, cur.aux.ptr, st);
+    if (l == 0) op_sor(fine, f32, cur.aux.ptr, xs(0), bs(0), omega, false, ultra);
+    else coarse_sweep(amg, cur.a, cur.aux.ptr, xs(l), bs(l), omega, false, ultra);
+  };
+  auto coarsest = [&] {
     AmgLevel& c = amg->lv[size_t(L - 1)];
     if (amg->host.lu)
       dense_lu_solve(amg->chol.as<double>(), amg->upper.as<double>(), c.n, c.rhs.ptr, c.sol.ptr, f32, st);
     else
       dense_cholesky_solve(amg->chol.as<double>(), c.n, c.rhs.ptr, c.sol.ptr, f32, st);
-  }
-  for (int l = L - 2; l >= 0; --l) {
-    AmgLevel& cur = amg->lv[size_t(l)];
-    prolong_add(xs(l), amg->lv[size_t(l + 1)].sol.ptr, f32, cur.agg.as<uint32_t>(), cur.n,
-                inv_relax, cur.aux.ptr, st);
-    if (l == 0) op_sor(fine, f32, cur.aux.ptr, xs(0), bs(0), relax, false, ultra);
-    else coarse_sweep(amg, cur.a, cur.aux.ptr, xs(l), bs(l), relax, false, ultra);
+  };
+  const bool regular = amg->params.cycle_kind == LGPU_CYCLE_REGULAR;
+  std::vector<int> iter(size_t(L), 0), num(size_t(L), 0);
+  int top = 0;
+  iter[0] = num[0] = amg->params.cycle_iterations;
+  while (top >= 0) {
+    const int l = top;
+    if (l == L - 1) {
+      coarsest();
+      --top;
+      continue;
+    }
+    if (iter[size_t(l)] == num[size_t(l)]) down(l);
+    if (iter[size_t(l)] == 0) {
+      up(l);
+      --top;
+      continue;
+    }
+    --iter[size_t(l)];
+    ++top;
+    const int next = top == L - 1 ? 0 : (regular ? num[size_t(l)] : iter[size_t(l)]);
+    iter[size_t(top)] = num[size_t(top)] = next;
   }
 }
 
@@ -1167,6 +1198,9 @@ extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {
   p->precision = LGPU_F64;
   p->ultra = 0;
   p->coarse_segment_rows = 64;
+  p->relax = 1.0;
+  p->cycle_kind = LGPU_CYCLE_REGULAR;
+  p->cycle_iterations = 1;
 }
 
 extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out) {
@@ -1177,6 +1211,10 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
   LGPU_REQUIRE(params->coarse_segment_rows >= 0 && params->coarse_segment_rows <= kMaxSegmentRows,
                "coarse_segment_rows must lie in 0..64");
   LGPU_REQUIRE(!finest->is_csr || !finest->csr.f32, "the finest CSR operator must hold f64 values");
+  LGPU_REQUIRE(std::isfinite(params->relax) && params->relax != 0.0, "relax must be finite and nonzero");
+  LGPU_REQUIRE(params->cycle_kind == LGPU_CYCLE_REGULAR || params->cycle_kind == LGPU_CYCLE_DECREASING,
+               "unknown cycle kind");
+  LGPU_REQUIRE(params->cycle_iterations >= 1, "cycle_iterations must be at least 1");
   const auto t0 = std::chrono::steady_clock::now();
   auto amg = std::make_unique<lgpu_amg_s>();
   amg->ctx = finest->ctx;

@@ -302,3 +302,25 @@ def test_hierarchy_nonsymmetric_known_answers(case):
     M = la.MultiGridSolver(wright).instantiate(A)
     assert M.level_size(1) == len(aggregates)
     np.testing.assert_array_equal(M.aggregates(0), AC.expected_map(rp.size - 1, aggregates))
+
+
+@pytest.mark.parametrize("relax,cycle", [(0.7, (la.CYCLE_REGULAR, 1)), (1.6, (la.CYCLE_REGULAR, 2)),
+                                         (1.0, (la.CYCLE_DECREASING, 3))])
+@pytest.mark.parametrize("f32", [False, True])
+def test_cycle_relax_and_kinds_equal_oracle(relax, cycle, f32):
+    """MultiGridSolver{relax, DefaultCycle{kind, iterations}} against the
+    oracle's restatement of the reference's cycle stack, bit for bit."""
+    dims = (24, 20, 16)
+    A, S = pair(dims)
+    H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=64, relax=relax, cycle_kind=cycle[0],
+                    cycle_iterations=cycle[1])
+    M = la.MultiGridSolver(precision=np.float32 if f32 else np.float64, relax=relax,
+                           cycle=cycle).instantiate(A)
+    r = problems.uniform(S.N, 8)
+    if f32:
+        r32 = r.astype(np.float32)
+        z = M.apply(la.DenseVector(r32), la.DenseVector(S.N, np.float32)).numpy()
+        np.testing.assert_array_equal(z, H.vcycle(r32))
+    else:
+        z = M.apply(la.DenseVector(r), la.DenseVector(S.N)).numpy()
+        np.testing.assert_array_equal(z, H.vcycle(r))
