@@ -1,4 +1,4 @@
-// Header-only C++20 classes over the lineal_amd C ABI.  They mirror the
+// Header-only C++23 classes over the lineal_amd C ABI.  They mirror the
 // reference's matrix, vector and solver interfaces (paths relative to the
 // reference's include/lineal/) so that code written against those interfaces
 // can switch to the GPU classes:
@@ -25,6 +25,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <optional>
 #include <span>
 #include <stdexcept>
 #include <string>
@@ -433,39 +434,101 @@ class GpuEffDiffCalc {
 enum class SorDirection : bool { BACKWARD = false, FORWARD = true };
 enum class SorVariant : bool { REGULAR = false, ULTRA = true };
 
+namespace detail {
+// In-situ application of a sweep (SingleSweepImpl::apply(sol, rhs, aux),
+// iterative-stationary/broom.hpp:43-51, 92-123): each sweep reads the
+// current iterate from an auxiliary copy.
+template <typename TReal>
+class InSitu {
+ protected:
+  template <typename TSweep>
+  void in_situ(GpuDenseVector<TReal>& sol, int iterations, TSweep&& sweep) const {
+    if (!aux_ || aux_->size() != sol.size())
+      aux_ = std::make_unique<GpuDenseVector<TReal>>(sol.size(), sol.context());
+    for (int k = 0; k < iterations; ++k) {
+      aux_->assign(sol);
+      sweep(*aux_, sol);
+    }
+  }
+
+ private:
+  mutable std::unique_ptr<GpuDenseVector<TReal>> aux_;
+};
+}  // namespace detail
+
+// JacobiSolver{relax[, IterMan{iterations}]} (jacobi/solver.hpp:21-39)
 template <typename TReal>
 class GpuJacobiSolver {
  public:
-  explicit GpuJacobiSolver(TReal relax) : relax_(relax) {}
-  class Instance {
+  explicit GpuJacobiSolver(TReal relax, int iterations = 1) : relax_(relax), iterations_(iterations) {}
+  class Instance : detail::InSitu<TReal> {
    public:
-    Instance(const GpuMatrixBase& lhs, TReal relax) : lhs_(&lhs), relax_(relax) {}
+    Instance(const GpuMatrixBase& lhs, TReal relax, int iterations)
+        : lhs_(&lhs), relax_(relax), iterations_(iterations) {}
+    // ex-situ: iterations sweeps from sol_in (null: zero) into sol_out
     void apply(const GpuDenseVector<TReal>* sol_in, GpuDenseVector<TReal>& sol_out,
                const GpuDenseVector<TReal>& rhs) const {
-      check(lgpu_op_jacobi(lhs_->handle(), sol_in ? sol_in->handle() : nullptr, sol_out.handle(),
-                           rhs.handle(), double(relax_)));
+      sweep(sol_in, sol_out, rhs);
+      if (iterations_ > 1) in_situ_n(sol_out, rhs, iterations_ - 1);
+    }
+    // in-situ: iterations sweeps from sol
+    void apply(GpuDenseVector<TReal>& sol, const GpuDenseVector<TReal>& rhs) const {
+      in_situ_n(sol, rhs, iterations_);
     }
 
    private:
+    void sweep(const GpuDenseVector<TReal>* in, GpuDenseVector<TReal>& out,
+               const GpuDenseVector<TReal>& rhs) const {
+      check(lgpu_op_jacobi(lhs_->handle(), in ? in->handle() : nullptr, out.handle(), rhs.handle(),
+                           double(relax_)));
+    }
+    void in_situ_n(GpuDenseVector<TReal>& sol, const GpuDenseVector<TReal>& rhs, int n) const {
+      this->in_situ(sol, n, [&](const GpuDenseVector<TReal>& in, GpuDenseVector<TReal>& out) {
+        sweep(&in, out, rhs);
+      });
+    }
     const GpuMatrixBase* lhs_;
     TReal relax_;
+    int iterations_;
   };
-  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const { return Instance(lhs, relax_); }
+  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const {
+    return Instance(lhs, relax_, iterations_);
+  }
 
  private:
   TReal relax_;
+  int iterations_;
 };
 
+// SorSolver<Real, IterMan, dir, variant>{relax[, IterMan{iterations}]}
+// (gauss-seidel-variant/solver.hpp:22-44, broom.hpp:232-239)
 template <typename TReal, SorDirection tDir = SorDirection::FORWARD,
           SorVariant tVariant = SorVariant::REGULAR>
 class GpuSorSolver {
  public:
-  explicit GpuSorSolver(TReal relax) : relax_(relax) {}
-  class Instance {
+  explicit GpuSorSolver(TReal relax, int iterations = 1) : relax_(relax), iterations_(iterations) {}
+  class Instance : detail::InSitu<TReal> {
    public:
-    Instance(const GpuMatrixBase& lhs, TReal relax) : lhs_(&lhs), relax_(relax) {}
-    // one lexicographic sweep (one thread segment)
+    Instance(const GpuMatrixBase& lhs, TReal relax, int iterations = 1)
+        : lhs_(&lhs), relax_(relax), iterations_(iterations) {}
+    // ex-situ: iterations lexicographic sweeps (one thread segment) from
+    // sol_in (null: zero) into sol_out
     void apply(const GpuDenseVector<TReal>* sol_in, GpuDenseVector<TReal>& sol_out,
+               const GpuDenseVector<TReal>& rhs) const {
+      sweep(sol_in, sol_out, rhs);
+      if (iterations_ > 1)
+        this->in_situ(sol_out, iterations_ - 1,
+                      [&](const GpuDenseVector<TReal>& in, GpuDenseVector<TReal>& out) {
+                        sweep(&in, out, rhs);
+                      });
+    }
+    // in-situ: iterations sweeps from sol
+    void apply(GpuDenseVector<TReal>& sol, const GpuDenseVector<TReal>& rhs) const {
+      this->in_situ(sol, iterations_, [&](const GpuDenseVector<TReal>& in, GpuDenseVector<TReal>& out) {
+        sweep(&in, out, rhs);
+      });
+    }
+    void sweep(const GpuDenseVector<TReal>* sol_in, GpuDenseVector<TReal>& sol_out,
                const GpuDenseVector<TReal>& rhs) const {
       check(lgpu_op_sor(lhs_->handle(), sol_in ? sol_in->handle() : nullptr, sol_out.handle(),
                         rhs.handle(), double(relax_), tDir == SorDirection::FORWARD ? 1 : 0,
@@ -484,11 +547,15 @@ class GpuSorSolver {
    private:
     const GpuMatrixBase* lhs_;
     TReal relax_;
+    int iterations_;
   };
-  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const { return Instance(lhs, relax_); }
+  [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const {
+    return Instance(lhs, relax_, iterations_);
+  }
 
  private:
   TReal relax_;
+  int iterations_;
 };
 
 // SsorSolver<Real, FixedIterationManager, variant>{relax, IterMan{iterations}}
@@ -516,6 +583,10 @@ class GpuSsorSolver {
       for (int k = 1; k < iterations_; ++k)
         check(lgpu_op_ssor(lhs_->handle(), sol_out.handle(), sol_out.handle(), rhs.handle(),
                            double(relax_), ultra()));
+    }
+    // in-situ (DoubleSweepImpl, broom.hpp:125-186)
+    void apply(GpuDenseVector<TReal>& sol, const GpuDenseVector<TReal>& rhs) const {
+      apply(&sol, sol, rhs);
     }
     [[nodiscard]] lgpu_amg handle() const { return h_.get(); }
 
@@ -598,10 +669,22 @@ class GpuMultiGridSolver {
       check(lgpu_amg_create(lhs.handle(), &p, &h));
       h_.reset(h);
     }
-    // sol = M rhs, one cycle from a constant-zero initial guess
-    // (multigrid/solver.hpp:47-53)
-    void apply(GpuDenseVector<double>& sol, const GpuDenseVector<double>& rhs) const {
-      check(lgpu_amg_vcycle(h_.get(), rhs.handle(), sol.handle()));
+    // ex-situ: one cycle from sol_in (null: the constant-zero initial guess,
+    // the preconditioner's use) into sol_out (multigrid/solver.hpp:47-53)
+    void apply(const GpuDenseVector<TLevelReal>* sol_in, GpuDenseVector<TLevelReal>& sol_out,
+               const GpuDenseVector<TLevelReal>& rhs) const {
+      if (!sol_in && std::is_same_v<TLevelReal, double>)
+        check(lgpu_amg_vcycle(h_.get(), rhs.handle(), sol_out.handle()));
+      else
+        check(lgpu_amg_cycle(h_.get(), sol_in ? sol_in->handle() : nullptr, rhs.handle(),
+                             sol_out.handle()));
+    }
+    // in-situ: one cycle from sol (multigrid/solver.hpp:41-46)
+    void apply(GpuDenseVector<TLevelReal>& sol, const GpuDenseVector<TLevelReal>& rhs) const {
+      if (!aux_ || aux_->size() != sol.size())
+        aux_ = std::make_unique<GpuDenseVector<TLevelReal>>(sol.size(), sol.context());
+      aux_->assign(sol);
+      check(lgpu_amg_cycle(h_.get(), aux_->handle(), rhs.handle(), sol.handle()));
     }
     [[nodiscard]] int level_num() const {
       int l = 0;
@@ -622,6 +705,7 @@ class GpuMultiGridSolver {
     };
     const GpuMatrixBase* lhs_;
     std::unique_ptr<lgpu_amg_s, Del> h_;
+    mutable std::unique_ptr<GpuDenseVector<TLevelReal>> aux_;
   };
   [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const { return Instance(lhs, *this); }
 
@@ -635,8 +719,10 @@ class GpuMultiGridSolver {
 
 // ---------------------------------------------------------------------------
 // Krylov solvers: instantiate(lhs) -> Instance; instance.instantiate(sol, rhs)
-// -> SystemInstance whose solve() runs the iteration on the device until the
-// stop rule holds (run_iterative_solver with a relative-residual callback).
+// -> SystemInstance with the reference's iterate(), residual_norm(), rho()
+// and solution() (conjugate-gradients/solver.hpp:75-121), plus solve(), the
+// fused loop that iterates on the device until a relative-residual stop rule
+// holds.
 
 struct StopRule {
   double rel_tol = 1e-8;
@@ -656,8 +742,9 @@ struct SolveReport {
 namespace detail {
 using KrylovFn = int (*)(lgpu_op, lgpu_amg, lgpu_vec, lgpu_vec, const lgpu_krylov_params*, int*,
                          double*);
+using KrylovCreateFn = int (*)(lgpu_op, lgpu_amg, lgpu_vec, lgpu_vec, int, lgpu_krylov*);
 
-template <KrylovFn tFn, typename TPrecon>
+template <KrylovFn tFn, KrylovCreateFn tCreate, typename TPrecon>
 class KrylovSolver {
  public:
   // ConjugateGradientsSolver{type_tag<Defs>, rere_controller, precon}: the
@@ -673,9 +760,30 @@ class KrylovSolver {
     }
     class SystemInstance {
      public:
+      // the reference's constructor work (the initial residual, and for CG
+      // the preconditioned residual and rho) runs on the first call of
+      // iterate(), residual_norm() or rho(), which solve() never needs
       SystemInstance(const Instance& inst, GpuDenseVector<double>& sol,
                      const GpuDenseVector<double>& rhs)
           : inst_(&inst), sol_(&sol), rhs_(&rhs) {}
+      // SystemInstance::iterate (conjugate-gradients/solver.hpp:75-115)
+      void iterate() { check(lgpu_krylov_iterate(system())); }
+      [[nodiscard]] double residual_norm() const {
+        double v = 0;
+        check(lgpu_krylov_residual_norm(system(), &v));
+        return v;
+      }
+      [[nodiscard]] double rho() const {
+        double v = 0;
+        check(lgpu_krylov_rho(system(), &v));
+        return v;
+      }
+      [[nodiscard]] std::size_t iteration() const {
+        int v = 0;
+        check(lgpu_krylov_iteration(system(), &v));
+        return std::size_t(v);
+      }
+      // the fused loop from the current solution
       SolveReport solve(StopRule stop) {
         lgpu_krylov_params p{stop.rel_tol, stop.max_it, inst_->rere_every_};
         SolveReport r;
@@ -694,9 +802,23 @@ class KrylovSolver {
       [[nodiscard]] const GpuDenseVector<double>& solution() const { return *sol_; }
 
      private:
+      lgpu_krylov system() const {
+        if (!sys_) {
+          lgpu_krylov h = nullptr;
+          check(tCreate(inst_->lhs_->handle(),
+                        inst_->precon_.empty() ? nullptr : inst_->precon_.front().handle(),
+                        rhs_->handle(), sol_->handle(), inst_->rere_every_, &h));
+          sys_.reset(h);
+        }
+        return sys_.get();
+      }
+      struct Del {
+        void operator()(lgpu_krylov k) const { lgpu_krylov_destroy(k); }
+      };
       const Instance* inst_;
       GpuDenseVector<double>* sol_;
       const GpuDenseVector<double>* rhs_;
+      mutable std::unique_ptr<lgpu_krylov_s, Del> sys_;
     };
     [[nodiscard]] SystemInstance instantiate(GpuDenseVector<double>& sol,
                                              const GpuDenseVector<double>& rhs) const {
@@ -717,13 +839,12 @@ class KrylovSolver {
 }  // namespace detail
 
 template <typename TPrecon = GpuMultiGridSolver<double>>
-using GpuConjugateGradients = detail::KrylovSolver<&lgpu_cg_solve, TPrecon>;
+using GpuConjugateGradients = detail::KrylovSolver<&lgpu_cg_solve, &lgpu_cg_create, TPrecon>;
 template <typename TPrecon = GpuMultiGridSolver<double>>
-using GpuBiCgStab = detail::KrylovSolver<&lgpu_bicgstab_solve, TPrecon>;
+using GpuBiCgStab = detail::KrylovSolver<&lgpu_bicgstab_solve, &lgpu_bicgstab_create, TPrecon>;
 
-// run_iterative_solver (linear-solver/runner.hpp:17-84) with the relative
-// residual stop rule of the reference's test callbacks
-// (test/aux/iterative.hpp:25-65).
+// run_iterative_solver with the relative residual stop rule, run as the
+// fused device loop.
 template <typename TSolver>
 inline SolveReport run_iterative_solver(const TSolver& solver, const GpuMatrixBase& lhs,
                                         GpuDenseVector<double>& sol,
@@ -731,6 +852,32 @@ inline SolveReport run_iterative_solver(const TSolver& solver, const GpuMatrixBa
   auto inst = solver.instantiate(lhs);
   auto sys = inst.instantiate(sol, rhs);
   return sys.solve(stop);
+}
+
+// run_iterative_solver(solver, lhs, sol, rhs, aux, env, iter_op)
+// (linear-solver/runner.hpp:17-84): iter_op(iteration, instance) returns an
+// std::optional; the first value it returns is the result.  A Krylov
+// solver's system instance is built from (sol, rhs) and iter_op sees it at
+// iteration 0 and after every iterate(); a stationary solver's instance is
+// applied in-situ to sol and iter_op sees it after every apply, from 1.
+template <typename TSolver, typename TVector, typename TIterOp>
+requires(!std::is_same_v<std::decay_t<TIterOp>, StopRule>)
+inline auto run_iterative_solver(const TSolver& solver, const GpuMatrixBase& lhs, TVector& sol,
+                                 const TVector& rhs, TIterOp&& iter_op) {
+  auto inst = solver.instantiate(lhs);
+  if constexpr (requires { inst.instantiate(sol, rhs); }) {
+    auto sys = inst.instantiate(sol, rhs);
+    if (auto res = iter_op(std::size_t{0}, sys); res.has_value()) return *res;
+    for (std::size_t i = 1;; ++i) {
+      sys.iterate();
+      if (auto res = iter_op(i, sys); res.has_value()) return *res;
+    }
+  } else {
+    for (std::size_t i = 1;; ++i) {
+      inst.apply(sol, rhs);
+      if (auto res = iter_op(i, inst); res.has_value()) return *res;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -258,6 +258,11 @@ int lgpu_amg_setup_split(lgpu_amg amg, double seconds[5]);
  * constant-zero in_sol, multigrid/solver.hpp:47-53); r and z are f64 (or
  * both f32 with an f32 hierarchy). */
 int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z);
+/* One cycle from the initial guess x_in: MultiGridSolver::Instance::apply
+ * (multigrid/solver.hpp:41-53) with the finest pre-smoothing starting from
+ * x_in instead of zero; x_in may be NULL (= lgpu_amg_vcycle) but must not
+ * alias z.  All three vectors have the hierarchy's precision. */
+int lgpu_amg_cycle(lgpu_amg amg, lgpu_vec x_in, lgpu_vec r, lgpu_vec z);
 
 /* ---- Krylov solvers ------------------------------------------------------- */
 typedef struct {
@@ -278,6 +283,37 @@ int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
 /* BiCgStabSolver (bicgstab/solver.hpp:51-151), right-preconditioned. */
 int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
                         const lgpu_krylov_params* params, int* iterations, double* res_hist);
+
+/* ---- Krylov system instances, one iteration per call ----------------------
+ * ConjugateGradientsSolver / BiCgStabSolver ::Instance::instantiate(sol, rhs)
+ * -> SystemInstance with iterate(env), residual_norm(), rho(), solution()
+ * (conjugate-gradients/solver.hpp:38-198, bicgstab/solver.hpp:46-170), the
+ * object run_iterative_solver hands to its iteration callback
+ * (linear-solver/runner.hpp:52-68).  Creating one performs the
+ * SystemInstance constructor's work (r = b - A x and its norm; CG: the
+ * preconditioned residual and rho; BiCGStab: r0 = p = r, rho = ||r||^2) and
+ * synchronises; so does every lgpu_krylov_iterate, which runs one iteration
+ * with the true residual at iterations rere_every, 2 rere_every, ... (0:
+ * never; test/aux/iterative.hpp:97-99).  x receives the iterates (the
+ * solution() of the reference); b and x must outlive the handle.  Each
+ * instance owns its work vectors and scalars, so several may be iterated in
+ * turn; lgpu_cg_solve / lgpu_bicgstab_solve remain the fused loops. */
+typedef struct lgpu_krylov_s* lgpu_krylov;
+int lgpu_cg_create(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x, int rere_every,
+                   lgpu_krylov* out);
+int lgpu_bicgstab_create(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x, int rere_every,
+                         lgpu_krylov* out);
+/* SystemInstance::iterate (conjugate-gradients/solver.hpp:75-115,
+ * bicgstab/solver.hpp:71-151): one iteration; returns LGPU_SYNC_TIMEOUT if a
+ * wavefront sweep's wait expired */
+int lgpu_krylov_iterate(lgpu_krylov k);
+/* SystemInstance::residual_norm() = sqrt(||r||^2) and rho() of the last
+ * iteration (or of the construction) */
+int lgpu_krylov_residual_norm(lgpu_krylov k, double* out);
+int lgpu_krylov_rho(lgpu_krylov k, double* out);
+/* iterations performed since creation */
+int lgpu_krylov_iteration(lgpu_krylov k, int* out);
+int lgpu_krylov_destroy(lgpu_krylov k);
 
 /* ---- matrix and vector files (io/tensor.hpp:25-263) -----------------------
  * SMF: u64 rows, u64 columns, u64 nonzeros, u64 row offsets[rows + 1], then

@@ -147,6 +147,14 @@ SIGNATURES = {
     "lgpu_amg_setup_seconds": (C.c_int, [_vp, _f64p]),
     "lgpu_amg_setup_split": (C.c_int, [_vp, _f64p]),
     "lgpu_amg_vcycle": (C.c_int, [_vp, _vp, _vp]),
+    "lgpu_amg_cycle": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "lgpu_cg_create": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
+    "lgpu_bicgstab_create": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
+    "lgpu_krylov_iterate": (C.c_int, [_vp]),
+    "lgpu_krylov_residual_norm": (C.c_int, [_vp, _f64p]),
+    "lgpu_krylov_rho": (C.c_int, [_vp, _f64p]),
+    "lgpu_krylov_iteration": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "lgpu_krylov_destroy": (C.c_int, [_vp]),
     "lgpu_cg_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
     "lgpu_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(_KrylovParams), C.POINTER(C.c_int), _f64p]),
     "lgpu_ctx_kernel_timing": (C.c_int, [_vp, C.c_int]),
@@ -483,36 +491,55 @@ class CsrMatrix(_Matrix):
 # stationary solvers (linear-solver/iterative-stationary)
 
 
-class JacobiSolver:
-    """JacobiSolver (iterative-stationary/jacobi/solver.hpp:21-39), one sweep per apply."""
+class _SweepInstance:
+    """A stationary solver instance whose apply is `iterations` sweeps:
+    apply(sol_in, sol_out, rhs) ex-situ (sol_in None: the constant-zero
+    guess) and apply_in_situ(sol, rhs) in-situ through an auxiliary vector
+    (SingleSweepImpl, iterative-stationary/broom.hpp:27-123)."""
 
-    def __init__(self, relax: float):
-        self.relax = relax
+    def __init__(self, lhs, sweep, iterations):
+        self.lhs, self._sweep, self.iterations = lhs, sweep, iterations
+        self._aux = None
+
+    def apply(self, sol_in, sol_out, rhs):
+        self._sweep(sol_in, sol_out, rhs)
+        for _ in range(self.iterations - 1):
+            self.apply_in_situ_once(sol_out, rhs)
+        return sol_out
+
+    def apply_in_situ_once(self, sol, rhs):
+        if self._aux is None or self._aux.dtype != sol.dtype:
+            self._aux = DenseVector(sol.n, sol.dtype, sol.ctx)
+        self._aux.assign(sol)
+        self._sweep(self._aux, sol, rhs)
+
+    def apply_in_situ(self, sol, rhs):
+        for _ in range(self.iterations):
+            self.apply_in_situ_once(sol, rhs)
+        return sol
+
+
+class JacobiSolver:
+    """JacobiSolver (iterative-stationary/jacobi/solver.hpp:21-39); `iterations`
+    sweeps per apply (FixedIterationManager)."""
+
+    def __init__(self, relax: float, iterations: int = 1):
+        self.relax, self.iterations = relax, iterations
 
     def instantiate(self, lhs: _Matrix):
-        solver = self
-
-        class Instance:
-            def apply(self, sol_in, sol_out, rhs):
-                lhs.jacobi(sol_in, sol_out, rhs, solver.relax)
-
-        return Instance()
+        return _SweepInstance(lhs, lambda i, o, b: lhs.jacobi(i, o, b, self.relax), self.iterations)
 
 
 class SorSolver:
-    """SorSolver (gauss-seidel-variant/solver.hpp:22-41) with one thread segment."""
+    """SorSolver<Real, IterMan, dir, variant>{relax[, IterMan{iterations}]}
+    (gauss-seidel-variant/solver.hpp:22-41) with one thread segment."""
 
-    def __init__(self, relax: float, forward: bool = True, ultra: bool = False):
-        self.relax, self.forward, self.ultra = relax, forward, ultra
+    def __init__(self, relax: float, forward: bool = True, ultra: bool = False, iterations: int = 1):
+        self.relax, self.forward, self.ultra, self.iterations = relax, forward, ultra, iterations
 
     def instantiate(self, lhs: _Matrix):
-        solver = self
-
-        class Instance:
-            def apply(self, sol_in, sol_out, rhs):
-                lhs.sor(sol_in, sol_out, rhs, solver.relax, solver.forward, solver.ultra)
-
-        return Instance()
+        return _SweepInstance(lhs, lambda i, o, b: lhs.sor(i, o, b, self.relax, self.forward, self.ultra),
+                              self.iterations)
 
 
 class SsorSolver:
@@ -543,6 +570,9 @@ class SsorInstance:
         for _ in range(s.iterations - 1):
             _check(lib().lgpu_op_ssor(self.lhs.h, sol_out.h, sol_out.h, rhs.h, float(s.relax), int(s.ultra)))
         return sol_out
+
+    def apply_in_situ(self, sol, rhs):
+        return self.apply(sol, sol, rhs)
 
     def __del__(self):
         try:
@@ -652,9 +682,24 @@ class MultiGridInstance:
         return dict(zip(("criterion", "aggregation", "galerkin", "coarsest", "rest"), out.tolist()))
 
     def apply(self, r: DenseVector, z: DenseVector):
-        """z = M r: one V-cycle from a zero initial guess."""
+        """z = M r: one cycle from a zero initial guess (the preconditioner)."""
         _check(lib().lgpu_amg_vcycle(self.h, r.h, z.h))
         return z
+
+    def apply_ex_situ(self, sol_in: DenseVector | None, sol_out: DenseVector, rhs: DenseVector):
+        """MultiGridSolver::Instance::apply(in_sol, out_sol, rhs): one cycle from
+        sol_in (None: zero) into sol_out (multigrid/solver.hpp:47-53)."""
+        _check(lib().lgpu_amg_cycle(self.h, _vh(sol_in), rhs.h, sol_out.h))
+        return sol_out
+
+    def apply_in_situ(self, sol: DenseVector, rhs: DenseVector):
+        """MultiGridSolver::Instance::apply(sol, rhs): one cycle from sol
+        (multigrid/solver.hpp:41-46)."""
+        aux = getattr(self, "_aux", None)
+        if aux is None or aux.dtype != sol.dtype:
+            aux = self._aux = DenseVector(sol.n, sol.dtype, sol.ctx)
+        aux.assign(sol)
+        return self.apply_ex_situ(aux, sol, rhs)
 
     def __del__(self):
         try:
@@ -680,14 +725,59 @@ class SolveResult:
         return self.status == OK
 
 
+class KrylovSystemInstance:
+    """SystemInstance of ConjugateGradientsSolver / BiCgStabSolver
+    (conjugate-gradients/solver.hpp:38-198, bicgstab/solver.hpp:46-170):
+    constructed from (sol, rhs), then iterate(), residual_norm(), rho(),
+    solution().  Each instance owns its device work vectors and scalars."""
+
+    def __init__(self, create_fn, lhs, precon, sol: DenseVector, rhs: DenseVector, rere_every: int):
+        self.sol, self.rhs, self.lhs, self.precon = sol, rhs, lhs, precon
+        h = _vp()
+        _check(getattr(lib(), create_fn)(lhs.h, _vh(precon), rhs.h, sol.h, int(rere_every), C.byref(h)))
+        self.h = h
+
+    def iterate(self):
+        _check(lib().lgpu_krylov_iterate(self.h))
+
+    def residual_norm(self) -> float:
+        out = C.c_double()
+        _check(lib().lgpu_krylov_residual_norm(self.h, C.byref(out)))
+        return out.value
+
+    def rho(self) -> float:
+        out = C.c_double()
+        _check(lib().lgpu_krylov_rho(self.h, C.byref(out)))
+        return out.value
+
+    @property
+    def iteration(self) -> int:
+        out = C.c_int()
+        _check(lib().lgpu_krylov_iteration(self.h, C.byref(out)))
+        return out.value
+
+    def solution(self) -> DenseVector:
+        return self.sol
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and not _finalizing:
+                lib().lgpu_krylov_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
 class _Krylov:
     _fn = ""
+    _create = ""
 
     def __init__(self, rere_every: int = 8, precon: MultiGridInstance | None = None):
         self.rere_every = rere_every
         self.precon = precon
 
     def solve(self, lhs: _Matrix, sol: DenseVector, rhs: DenseVector, rel_tol=1e-8, max_it=1000) -> SolveResult:
+        """The fused loop: iterations until ||r_k|| <= rel_tol ||r_0|| or max_it."""
         p = _KrylovParams(float(rel_tol), int(max_it), int(self.rere_every))
         it = C.c_int()
         hist = np.zeros(max_it + 1)
@@ -696,22 +786,64 @@ class _Krylov:
         _check(st, allow=(NOT_CONVERGED, BREAKDOWN, NONFINITE))
         return SolveResult(st, it.value, hist[: it.value + 1])
 
+    def instantiate(self, lhs: _Matrix) -> "_KrylovInstance":
+        return _KrylovInstance(self, lhs)
+
+
+class _KrylovInstance:
+    def __init__(self, solver: _Krylov, lhs: _Matrix):
+        self.solver, self.lhs = solver, lhs
+
+    def instantiate(self, sol: DenseVector, rhs: DenseVector) -> KrylovSystemInstance:
+        s = self.solver
+        return KrylovSystemInstance(s._create, self.lhs, s.precon, sol, rhs, s.rere_every)
+
 
 class ConjugateGradientsSolver(_Krylov):
     """ConjugateGradientsSolver (conjugate-gradients/solver.hpp:25-221)."""
 
     _fn = "lgpu_cg_solve"
+    _create = "lgpu_cg_create"
 
 
 class BiCgStabSolver(_Krylov):
     """BiCgStabSolver (bicgstab/solver.hpp:26-259)."""
 
     _fn = "lgpu_bicgstab_solve"
+    _create = "lgpu_bicgstab_create"
 
 
-def run_iterative_solver(solver: _Krylov, lhs, sol, rhs, rel_tol=1e-8, max_it=1000) -> SolveResult:
-    """run_iterative_solver (linear-solver/runner.hpp:17-84) with a relative-residual stop."""
-    return solver.solve(lhs, sol, rhs, rel_tol, max_it)
+def run_iterative_solver(solver, lhs, sol, rhs, iter_op=None, rel_tol=1e-8, max_it=1000):
+    """run_iterative_solver(solver, lhs, sol, rhs, aux, env, iter_op)
+    (linear-solver/runner.hpp:17-84).
+
+    With iter_op (called as iter_op(iteration, instance), returning None to
+    continue or the result to stop): a Krylov solver's system instance is
+    built from (sol, rhs) and iter_op sees it at iteration 0 and after every
+    iterate(); a stationary solver (JacobiSolver, SorSolver, SsorSolver,
+    MultiGridSolver) is applied in-situ to sol and iter_op sees its instance
+    after every apply, from iteration 1.  Without iter_op a Krylov solver runs
+    the fused loop with the relative-residual stop rule and returns a
+    SolveResult."""
+    if iter_op is None:
+        return solver.solve(lhs, sol, rhs, rel_tol, max_it)
+    inst = solver.instantiate(lhs)
+    if isinstance(solver, _Krylov):
+        system = inst.instantiate(sol, rhs)
+        res = iter_op(0, system)
+        i = 1
+        while res is None:
+            system.iterate()
+            res = iter_op(i, system)
+            i += 1
+        return res
+    i = 1
+    while True:
+        inst.apply_in_situ(sol, rhs)
+        res = iter_op(i, inst)
+        if res is not None:
+            return res
+        i += 1
 
 
 # ---------------------------------------------------------------------------

@@ -455,20 +455,20 @@ void post_scalars(lgpu_ctx ctx) {
 }
 void wait_scalars(lgpu_ctx ctx) { LGPU_HIP_CHECK(hipEventSynchronize(ctx->scal_ev)); }
 
-// r = b - A x (f64), ||r||^2 into scal[slot]
-void residual_norm(lgpu_op op, const double* b, const double* x, double* r, int slot) {
+// r = b - A x (f64), ||r||^2 into the device scalar *dst
+void residual_norm(lgpu_op op, const double* b, const double* x, double* r, double* dst) {
   lgpu_ctx ctx = op->ctx;
   const int64_t np = op_partials(op);
   double* part = ctx->partials_for(np);
   op_apply(op, SM_RESID_NORM, false, x, b, r, 0.0, part);
-  reduce_partials(part, np, ctx->scal.as<double>() + slot, ctx->stream);
+  reduce_partials(part, np, dst, ctx->stream);
 }
 
-void dot_into(lgpu_ctx ctx, const void* a, const void* b, bool f32, int64_t n, int slot) {
+void dot_into(lgpu_ctx ctx, const void* a, const void* b, bool f32, int64_t n, double* dst) {
   const int64_t np = dot_partials(n);
   double* part = ctx->partials_for(np);
   vec_dot(a, b, f32, n, part, ctx->stream);
-  reduce_partials(part, np, ctx->scal.as<double>() + slot, ctx->stream);
+  reduce_partials(part, np, dst, ctx->stream);
 }
 
 // smoother of a coarse level: thread segments of coarse_segment_rows rows, or
@@ -493,7 +493,7 @@ void coarse_sweep(lgpu_amg amg, CsrDev& a, const void* x_in, void* x_out, const 
 // own finest operator, or a Krylov operator of the same size (the upwind
 // advection operator over the hierarchy of its diffusion part, so that the
 // forward sweep follows the flow).
-void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine) {
+void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine, const void* x0 = nullptr) {
   const bool f32 = amg->f32;
   hipStream_t st = amg->ctx->stream;
   const int L = int(amg->lv.size());
@@ -506,7 +506,7 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine) {
   auto down = [&](int l) {
     AmgLevel& cur = amg->lv[size_t(l)];
     if (l == 0) {
-      op_sor(fine, f32, nullptr, xs(0), bs(0), omega, true, ultra);
+      op_sor(fine, f32, x0, xs(0), bs(0), omega, true, ultra);
       op_apply(fine, SM_RESID, f32, xs(0), bs(0), cur.tmp.ptr, 0.0, nullptr);
     } else {
       coarse_sweep(amg, cur.a, nullptr, xs(l), bs(l), omega, true, ultra);
@@ -806,7 +806,7 @@ extern "C" int lgpu_vec_copy(lgpu_vec src, lgpu_vec dst) {
 extern "C" int lgpu_vec_dot(lgpu_vec a, lgpu_vec b, double* out) {
   API_BEGIN
   LGPU_REQUIRE(a && b && out && a->n == b->n && a->f32 == b->f32, "invalid dot");
-  dot_into(a->ctx, a->ptr, b->ptr, a->f32, a->n, SC_RR);
+  dot_into(a->ctx, a->ptr, b->ptr, a->f32, a->n, a->ctx->scal.as<double>() + SC_RR);
   *out = read_scalar(a->ctx, SC_RR);
   API_END
 }
@@ -1249,8 +1249,14 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
     a.n = h.sizes[size_t(l)];
     if (l > 0) {
       a.a.upload(h.coarse[size_t(l - 1)], amg->f32);
+      // the level vectors persist between cycles, as the reference's
+      // hierarchy vectors do (hierarchy/wright.hpp:227-228): a DECREASING
+      // cycle prolongs a coarse solution no visit of the same cycle wrote.
+      // They start at zero (the reference leaves them uninitialised).
       a.rhs.alloc(size_t(a.n) * es);
       a.sol.alloc(size_t(a.n) * es);
+      LGPU_HIP_CHECK(hipMemset(a.rhs.ptr, 0, size_t(a.n) * es));
+      LGPU_HIP_CHECK(hipMemset(a.sol.ptr, 0, size_t(a.n) * es));
     }
     if (l + 1 < L) {
       const setup::Level& t = h.transfer[size_t(l)];
@@ -1348,6 +1354,24 @@ extern "C" int lgpu_amg_setup_split(lgpu_amg amg, double seconds[5]) {
   API_END
 }
 
+extern "C" int lgpu_amg_cycle(lgpu_amg amg, lgpu_vec x_in, lgpu_vec r, lgpu_vec z) {
+  API_BEGIN
+  LGPU_REQUIRE(amg && amg->ssor_iters == 0, "needs a multigrid hierarchy");
+  const int64_t n = amg->lv[0].n;
+  check_vec(r, n, "r");
+  check_vec(z, n, "z");
+  LGPU_REQUIRE(r->ptr != z->ptr, "r and z must not alias");
+  LGPU_REQUIRE(r->f32 == amg->f32 && z->f32 == amg->f32, "the vectors need the hierarchy's precision");
+  if (x_in) {
+    check_vec(x_in, n, "x_in");
+    LGPU_REQUIRE(x_in->f32 == amg->f32, "x_in needs the hierarchy's precision");
+    LGPU_REQUIRE(x_in->ptr != z->ptr, "x_in and z must not alias");
+  }
+  vcycle(amg, r->ptr, z->ptr, amg->finest, x_in ? x_in->ptr : nullptr);
+  if (take_wait_error(amg->ctx)) return wait_timeout();
+  API_END
+}
+
 extern "C" int lgpu_amg_vcycle(lgpu_amg amg, lgpu_vec r, lgpu_vec z) {
   API_BEGIN
   LGPU_REQUIRE(amg, "NULL AMG");
@@ -1382,168 +1406,120 @@ namespace {
 
 bool rere(int iteration, int every) { return every > 0 && iteration > 0 && iteration % every == 0; }
 
-// End of a Krylov solve.  Every convergence check reads the wait-error word
-// with the scalars, so an expiry in the work an iteration used has already
-// stopped the loop (LGPU_SYNC_TIMEOUT).  Work queued after the last check
-// is the next iteration's speculative head, whose results are discarded: an
-// expiry there only clears the word.  A solve that never reached a check
-// (a zero or non-finite initial residual) takes the word as it stands.
-int finish(lgpu_ctx ctx, int status, bool checked) {
-  const bool late = take_wait_error(ctx);
-  if (status == LGPU_SYNC_TIMEOUT) return wait_timeout();
-  if (late && !checked) return wait_timeout();
-  return status;
-}
+// The device state of a Krylov SystemInstance: ConjugateGradientsSolver's
+// (conjugate-gradients/solver.hpp:50-121) or BiCgStabSolver's
+// (bicgstab/solver.hpp:51-151).  Scalars (||r||^2, rho, <p, A p>, alpha,
+// omega ...) stay in a device block; start() and each iteration are queued on
+// the context's stream, and post()/wait() bring the block to the host
+// together with the context's wait-error word.
+struct KrylovSystem {
+  lgpu_ctx ctx = nullptr;
+  lgpu_op op = nullptr;
+  lgpu_amg precon = nullptr;
+  bool bicg = false;
+  int64_t n = 0;
+  int rere_every = 0;
+  double* x = nullptr;
+  const double* b = nullptr;
+  double* w[6] = {};         // CG: r, p, q; BiCGStab: r, r0, p, v, t, px
+  double* scal = nullptr;    // SC_COUNT device scalars
+  double* host = nullptr;    // their pinned mirror
+  hipEvent_t ev = nullptr;
+  int it = 0, cur = SC_RHO0, nxt = SC_RHO1;
+  bool head_queued = false;
 
-}  // namespace
+  hipStream_t st() const { return ctx->stream; }
+  bool mixed() const { return precon && precon->f32 && !bicg; }
 
-extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
-                             const lgpu_krylov_params* params, int* iterations, double* res_hist) {
-  API_BEGIN
-  LGPU_REQUIRE(op && params, "NULL argument");
-  const int64_t n = op->rows();
-  check_vec(b, n, "b");
-  check_vec(x, n, "x");
-  LGPU_REQUIRE(!b->f32 && !x->f32, "the Krylov vectors are f64");
-  LGPU_REQUIRE(!precon || (precon->ctx == op->ctx && precon->finest->rows() == n),
-               "the preconditioner must share the operator's context and size");
-  lgpu_ctx ctx = op->ctx;
-  hipStream_t st = ctx->stream;
-  double* scal = ctx->scal.as<double>();
-  double *r = ctx->work_vec(0, n), *p = ctx->work_vec(1, n), *q = ctx->work_vec(2, n);
-  double* xv = static_cast<double*>(x->ptr);
-  const double* bv = static_cast<const double*>(b->ptr);
-  // conjugate-gradients/solver.hpp:58-71
-  residual_norm(op, bv, xv, r, SC_RR);
-  int cur = SC_RHO0, nxt = SC_RHO1;
-  if (precon) {
-    precondition(precon, r, p, op);
-    dot_into(ctx, p, r, false, n, cur);
-  } else {
-    LGPU_HIP_CHECK(hipMemcpyAsync(p, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st));
-    LGPU_HIP_CHECK(hipMemcpyAsync(scal + cur, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st));
+  // SystemInstance constructor: r = b - A x with ||r||^2, then
+  // CG: p = M r, rho = <p, r> (p = r, rho = ||r||^2 without a preconditioner);
+  // BiCGStab: r0 = p = r, rho = ||r||^2
+  void start() {
+    double* r = w[0];
+    residual_norm(op, b, x, r, scal + SC_RR);
+    if (bicg) {
+      LGPU_HIP_CHECK(hipMemcpyAsync(w[1], r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st()));
+      LGPU_HIP_CHECK(hipMemcpyAsync(w[2], r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st()));
+      LGPU_HIP_CHECK(hipMemcpyAsync(scal + cur, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st()));
+    } else if (precon) {
+      precondition(precon, r, w[1], op);
+      dot_into(ctx, w[1], r, false, n, scal + cur);
+    } else {
+      LGPU_HIP_CHECK(hipMemcpyAsync(w[1], r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st()));
+      LGPU_HIP_CHECK(hipMemcpyAsync(scal + cur, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st()));
+    }
   }
-  const double r0 = std::sqrt(read_scalar(ctx, SC_RR));
-  if (res_hist) res_hist[0] = r0;
-  int it = 0, status = LGPU_NOT_CONVERGED;
-  if (!std::isfinite(r0)) status = LGPU_NONFINITE;
-  else if (r0 == 0.0) status = LGPU_OK;
-  // q = A p and <p, q> of an iteration; it reads p only, so it is queued
-  // before the host has read the previous iteration's residual norm
-  auto spmv_pq = [&] {
-    const int64_t np = op_partials(op);
-    double* part = ctx->partials_for(np);
-    op_apply(op, SM_SPMV_DOT, false, p, nullptr, q, 0.0, part);
-    reduce_partials(part, np, scal + SC_PQ, st);
-  };
-  bool pq_queued = false, checked = false;
-  while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
-    // conjugate-gradients/solver.hpp:75-115
-    if (!pq_queued) spmv_pq();
-    pq_queued = false;
-    // FP32 V-cycle: the update also rounds r into the cycle's input, and the
+
+  // the part of an iteration that reads p only (CG: q = A p and <p, q>;
+  // BiCGStab: y = M p and v = A y), so that it can be queued before the
+  // host has read the previous iteration's scalars
+  void head() {
+    if (bicg) {
+      double *p = w[2], *v = w[3], *px = w[5];
+      double* y = precon ? px : p;
+      if (precon) precondition(precon, p, px, op);
+      op_apply(op, SM_SPMV, false, y, nullptr, v, 0.0, nullptr);
+    } else {
+      const int64_t np = op_partials(op);
+      double* part = ctx->partials_for(np);
+      op_apply(op, SM_SPMV_DOT, false, w[1], nullptr, w[2], 0.0, part);
+      reduce_partials(part, np, scal + SC_PQ, st());
+    }
+  }
+
+  // the rest of iteration `it` (SystemInstance::iterate), then it + 1
+  void rest() {
+    if (bicg) rest_bicg();
+    else rest_cg();
+    std::swap(cur, nxt);
+    ++it;
+  }
+
+  void rest_cg() {
+    double *r = w[0], *p = w[1], *q = w[2];
+    // FP32 cycle: the update also rounds r into the cycle's input, and the
     // dot and the direction update read the cycle's f32 output directly (it
     // widens exactly), so no conversion pass runs in between
-    const bool mixed = precon && precon->f32;
-    float* r32 = mixed ? static_cast<float*>(precon->r32.ptr) : nullptr;
+    float* r32 = mixed() ? static_cast<float*>(precon->r32.ptr) : nullptr;
     bool r32_ready = false;
-    if (rere(it, params->rere_every)) {
-      axpy_scal(xv, p, scal, cur, SC_PQ, 1.0, n, st);
-      residual_norm(op, bv, xv, r, SC_RR);
+    if (rere(it, rere_every)) {
+      axpy_scal(x, p, scal, cur, SC_PQ, 1.0, n, st());
+      residual_norm(op, b, x, r, scal + SC_RR);
     } else {
       const int64_t nd = dot_partials(n);
       double* pd = ctx->partials_for(nd);
-      cg_update(xv, r, r32, p, q, scal, cur, SC_PQ, n, pd, st);
-      reduce_partials(pd, nd, scal + SC_RR, st);
-      r32_ready = mixed;
+      cg_update(x, r, r32, p, q, scal, cur, SC_PQ, n, pd, st());
+      reduce_partials(pd, nd, scal + SC_RR, st());
+      r32_ready = mixed();
     }
-    if (mixed) {
-      if (!r32_ready) vec_convert(r, false, r32, true, n, st);
+    if (mixed()) {
+      if (!r32_ready) vec_convert(r, false, r32, true, n, st());
       float* z32 = static_cast<float*>(precon->z32.ptr);
       vcycle(precon, r32, z32, op);
       const int64_t nd = dot_partials(n);
       double* pd = ctx->partials_for(nd);
-      vec_dot_mixed(r, z32, n, pd, st);
-      reduce_partials(pd, nd, scal + nxt, st);
-      xpby_scal(p, z32, scal, nxt, cur, n, st);
+      vec_dot_mixed(r, z32, n, pd, st());
+      reduce_partials(pd, nd, scal + nxt, st());
+      xpby_scal(p, z32, scal, nxt, cur, n, st());
     } else if (precon) {
       precondition(precon, r, q, op);
-      dot_into(ctx, r, q, false, n, nxt);
-      xpby_scal(p, q, scal, nxt, cur, n, st);
+      dot_into(ctx, r, q, false, n, scal + nxt);
+      xpby_scal(p, q, scal, nxt, cur, n, st());
     } else {
-      LGPU_HIP_CHECK(hipMemcpyAsync(scal + nxt, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st));
-      xpby_scal(p, r, scal, nxt, cur, n, st);
+      LGPU_HIP_CHECK(hipMemcpyAsync(scal + nxt, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st()));
+      xpby_scal(p, r, scal, nxt, cur, n, st());
     }
-    std::swap(cur, nxt);
-    ++it;
-    // the next iteration's product runs while the host checks this one (a
-    // final extra product only overwrites q and the <p, q> slot)
-    post_scalars(ctx);
-    if (it < params->max_it) {
-      spmv_pq();
-      pq_queued = true;
-    }
-    wait_scalars(ctx);
-    checked = true;
-    const double rn = std::sqrt(ctx->host_scal[SC_RR]);
-    if (res_hist) res_hist[it] = rn;
-    if (ctx->host_err()) status = LGPU_SYNC_TIMEOUT;
-    else if (!std::isfinite(rn)) status = LGPU_NONFINITE;
-    else if (rn <= params->rel_tol * r0) status = LGPU_OK;
-    else if (ctx->host_scal[cur] == 0.0) status = LGPU_BREAKDOWN;
   }
-  if (iterations) *iterations = it;
-  return finish(ctx, status, checked);
-  API_END
-}
 
-extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
-                                   const lgpu_krylov_params* params, int* iterations,
-                                   double* res_hist) {
-  API_BEGIN
-  LGPU_REQUIRE(op && params, "NULL argument");
-  const int64_t n = op->rows();
-  check_vec(b, n, "b");
-  check_vec(x, n, "x");
-  LGPU_REQUIRE(!b->f32 && !x->f32, "the Krylov vectors are f64");
-  LGPU_REQUIRE(!precon || (precon->ctx == op->ctx && precon->finest->rows() == n),
-               "the preconditioner must share the operator's context and size");
-  lgpu_ctx ctx = op->ctx;
-  hipStream_t st = ctx->stream;
-  double* scal = ctx->scal.as<double>();
-  double *r = ctx->work_vec(0, n), *r0 = ctx->work_vec(1, n), *p = ctx->work_vec(2, n),
-         *v = ctx->work_vec(3, n), *t = ctx->work_vec(4, n), *px = ctx->work_vec(5, n);
-  double* xv = static_cast<double*>(x->ptr);
-  const double* bv = static_cast<const double*>(b->ptr);
-  // bicgstab/solver.hpp:64-68
-  residual_norm(op, bv, xv, r, SC_RR);
-  LGPU_HIP_CHECK(hipMemcpyAsync(r0, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st));
-  LGPU_HIP_CHECK(hipMemcpyAsync(p, r, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, st));
-  int cur = SC_RHO0, nxt = SC_RHO1;
-  LGPU_HIP_CHECK(hipMemcpyAsync(scal + cur, scal + SC_RR, sizeof(double), hipMemcpyDeviceToDevice, st));
-  const double rn0 = std::sqrt(read_scalar(ctx, SC_RR));
-  if (res_hist) res_hist[0] = rn0;
-  int it = 0, status = LGPU_NOT_CONVERGED;
-  if (!std::isfinite(rn0)) status = LGPU_NONFINITE;
-  else if (rn0 == 0.0) status = LGPU_OK;
-  const int64_t nd = dot_partials(n);
-  // y = M p and v = A y of an iteration; they read p only, so they are
-  // queued before the host has read the previous iteration's residual norm
-  double* const y = precon ? px : p;
-  auto head = [&] {
-    if (precon) precondition(precon, p, px, op);
-    op_apply(op, SM_SPMV, false, y, nullptr, v, 0.0, nullptr);
-  };
-  bool head_queued = false, checked = false;
-  while (status == LGPU_NOT_CONVERGED && it < params->max_it) {
-    // bicgstab/solver.hpp:71-151
-    if (!head_queued) head();
-    head_queued = false;
-    dot_into(ctx, r0, v, false, n, SC_R0V);
-    axpy_scal(xv, y, scal, cur, SC_R0V, 1.0, n, st);
+  void rest_bicg() {
+    double *r = w[0], *r0 = w[1], *p = w[2], *v = w[3], *t = w[4], *px = w[5];
+    double* y = precon ? px : p;
+    const int64_t nd = dot_partials(n);
+    dot_into(ctx, r0, v, false, n, scal + SC_R0V);
+    axpy_scal(x, y, scal, cur, SC_R0V, 1.0, n, st());
     double* pd = ctx->partials_for(nd);
-    bicg_r_update(r, v, scal, cur, SC_R0V, SC_ALPHA, n, pd, st);
-    reduce_partials(pd, nd, scal + SC_RR, st);
+    bicg_r_update(r, v, scal, cur, SC_R0V, SC_ALPHA, n, pd, st());
+    reduce_partials(pd, nd, scal + SC_RR, st());
     double* z = r;
     if (precon) {
       precondition(precon, r, px, op);
@@ -1551,43 +1527,239 @@ extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu
     }
     op_apply(op, SM_SPMV, false, z, nullptr, t, 0.0, nullptr);
     pd = ctx->partials_for(nd);
-    vec_dot2(t, r, n, pd, st);
-    reduce_partials2(pd, nd, scal + SC_TR, scal + SC_TT, st);
-    axpy_scal(xv, z, scal, SC_TR, SC_TT, 1.0, n, st);
-    if (rere(it, params->rere_every)) {
+    vec_dot2(t, r, n, pd, st());
+    reduce_partials2(pd, nd, scal + SC_TR, scal + SC_TT, st());
+    axpy_scal(x, z, scal, SC_TR, SC_TT, 1.0, n, st());
+    if (rere(it, rere_every)) {
       // omega still has to be stored for the direction update
       pd = ctx->partials_for(nd);
-      bicg_r_update(t, t, scal, SC_TR, SC_TT, SC_OMEGA, 0, pd, st);
-      residual_norm(op, bv, xv, r, SC_RR);
+      bicg_r_update(t, t, scal, SC_TR, SC_TT, SC_OMEGA, 0, pd, st());
+      residual_norm(op, b, x, r, scal + SC_RR);
     } else {
       pd = ctx->partials_for(nd);
-      bicg_r_update(r, t, scal, SC_TR, SC_TT, SC_OMEGA, n, pd, st);
+      bicg_r_update(r, t, scal, SC_TR, SC_TT, SC_OMEGA, n, pd, st());
     }
     pd = ctx->partials_for(nd);
-    vec_dot2(r, r0, n, pd, st);
-    reduce_partials2(pd, nd, scal + nxt, scal + SC_RR, st);
-    // dot2 gives (r.r0, r.r); the second is ||r||^2
-    bicg_p_update(p, r, v, scal, nxt, cur, n, st);
-    std::swap(cur, nxt);
-    ++it;
-    // the next iteration's preconditioner and product run while the host
-    // checks this one (a final extra pair only overwrites px and v)
-    post_scalars(ctx);
-    if (it < params->max_it) {
-      head();
-      head_queued = true;
-    }
-    wait_scalars(ctx);
-    checked = true;
-    const double rn = std::sqrt(ctx->host_scal[SC_RR]);
-    if (res_hist) res_hist[it] = rn;
-    if (ctx->host_err()) status = LGPU_SYNC_TIMEOUT;
-    else if (!std::isfinite(rn)) status = LGPU_NONFINITE;
-    else if (rn <= params->rel_tol * rn0) status = LGPU_OK;
-    else if (ctx->host_scal[cur] == 0.0) status = LGPU_BREAKDOWN;
+    vec_dot2(r, r0, n, pd, st());
+    // (<r, r0>, ||r||^2)
+    reduce_partials2(pd, nd, scal + nxt, scal + SC_RR, st());
+    bicg_p_update(p, r, v, scal, nxt, cur, n, st());
   }
-  if (iterations) *iterations = it;
+
+  // the scalar block (and the wait-error word) into the pinned mirror,
+  // without waiting: work queued after this call overlaps the copy
+  void post() {
+    LGPU_HIP_CHECK(hipMemcpyAsync(host, scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, st()));
+    if (scal != ctx->scal.as<double>())
+      LGPU_HIP_CHECK(hipMemcpyAsync(host + SC_ERR, ctx->err(), sizeof(int), hipMemcpyDeviceToHost, st()));
+    LGPU_HIP_CHECK(hipEventRecord(ev, st()));
+  }
+  void wait() const { LGPU_HIP_CHECK(hipEventSynchronize(ev)); }
+  double rr() const { return host[SC_RR]; }
+  double rho() const { return host[cur]; }
+  bool err() const { return reinterpret_cast<const int*>(host + SC_ERR)[0] != 0; }
+};
+
+void check_krylov_args(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x) {
+  LGPU_REQUIRE(op, "NULL operator");
+  const int64_t n = op->rows();
+  check_vec(b, n, "b");
+  check_vec(x, n, "x");
+  LGPU_REQUIRE(!b->f32 && !x->f32, "the Krylov vectors are f64");
+  LGPU_REQUIRE(x->ptr != b->ptr, "x and b must not alias");
+  LGPU_REQUIRE(!precon || (precon->ctx == op->ctx && precon->finest->rows() == n),
+               "the preconditioner must share the operator's context and size");
+}
+
+// End of a fused Krylov solve.  Every convergence check reads the wait-error
+// word with the scalars, so an expiry in the work an iteration used has
+// already stopped the loop (LGPU_SYNC_TIMEOUT).  Work queued after the last
+// check is the next iteration's speculative head, whose results are
+// discarded: an expiry there only clears the word.  A solve that never
+// reached a check (a zero or non-finite initial residual) takes the word as
+// it stands.
+int finish(lgpu_ctx ctx, int status, bool checked) {
+  const bool late = take_wait_error(ctx);
+  if (status == LGPU_SYNC_TIMEOUT) return wait_timeout();
+  if (late && !checked) return wait_timeout();
+  return status;
+}
+
+// run_iterative_solver (linear-solver/runner.hpp:17-84) with the relative
+// residual stop rule, on the context's shared work vectors and scalars.  The
+// next iteration's head is queued before the host waits for this one's
+// scalars, so the host's turnaround overlaps it.
+int krylov_solve(bool bicg, lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
+                 const lgpu_krylov_params* params, int* iterations, double* res_hist) {
+  LGPU_REQUIRE(params, "NULL params");
+  check_krylov_args(op, precon, b, x);
+  lgpu_ctx ctx = op->ctx;
+  KrylovSystem s;
+  s.ctx = ctx;
+  s.op = op;
+  s.precon = precon;
+  s.bicg = bicg;
+  s.n = op->rows();
+  s.rere_every = params->rere_every;
+  s.x = static_cast<double*>(x->ptr);
+  s.b = static_cast<const double*>(b->ptr);
+  for (int k = 0; k < (bicg ? 6 : 3); ++k) s.w[k] = ctx->work_vec(k, s.n);
+  s.scal = ctx->scal.as<double>();
+  s.host = ctx->host_scal;
+  s.ev = ctx->scal_ev;
+  s.start();
+  const double r0 = std::sqrt(read_scalar(ctx, SC_RR));
+  if (res_hist) res_hist[0] = r0;
+  int status = LGPU_NOT_CONVERGED;
+  bool checked = false;
+  if (!std::isfinite(r0)) status = LGPU_NONFINITE;
+  else if (r0 == 0.0) status = LGPU_OK;
+  while (status == LGPU_NOT_CONVERGED && s.it < params->max_it) {
+    if (!s.head_queued) s.head();
+    s.head_queued = false;
+    s.rest();
+    s.post();
+    if (s.it < params->max_it) {
+      s.head();
+      s.head_queued = true;
+    }
+    s.wait();
+    checked = true;
+    const double rn = std::sqrt(s.rr());
+    if (res_hist) res_hist[s.it] = rn;
+    if (s.err()) status = LGPU_SYNC_TIMEOUT;
+    else if (!std::isfinite(rn)) status = LGPU_NONFINITE;
+    else if (rn <= params->rel_tol * r0) status = LGPU_OK;
+    else if (s.rho() == 0.0) status = LGPU_BREAKDOWN;
+  }
+  if (iterations) *iterations = s.it;
   return finish(ctx, status, checked);
+}
+
+}  // namespace
+
+// A SystemInstance on its own work vectors and scalar block, iterated one
+// step per call (lgpu_krylov_iterate).
+struct lgpu_krylov_s {
+  KrylovSystem s;
+  DevBuf vecs, scal;
+  double* host = nullptr;
+  hipEvent_t ev = nullptr;
+  ~lgpu_krylov_s() {
+    if (s.ctx) (void)hipStreamSynchronize(s.ctx->stream);
+    if (host) (void)hipHostFree(host);
+    if (ev) (void)hipEventDestroy(ev);
+  }
+};
+
+extern "C" int lgpu_cg_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
+                             const lgpu_krylov_params* params, int* iterations, double* res_hist) {
+  API_BEGIN
+  return krylov_solve(false, op, precon, b, x, params, iterations, res_hist);
+  API_END
+}
+
+extern "C" int lgpu_bicgstab_solve(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
+                                   const lgpu_krylov_params* params, int* iterations,
+                                   double* res_hist) {
+  API_BEGIN
+  return krylov_solve(true, op, precon, b, x, params, iterations, res_hist);
+  API_END
+}
+
+namespace {
+
+int krylov_create(bool bicg, lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x, int rere_every,
+                  lgpu_krylov* out) {
+  LGPU_REQUIRE(out, "NULL output");
+  check_krylov_args(op, precon, b, x);
+  LGPU_REQUIRE(rere_every >= 0, "rere_every must not be negative");
+  auto k = std::make_unique<lgpu_krylov_s>();
+  KrylovSystem& s = k->s;
+  s.ctx = op->ctx;
+  s.op = op;
+  s.precon = precon;
+  s.bicg = bicg;
+  s.n = op->rows();
+  s.rere_every = rere_every;
+  s.x = static_cast<double*>(x->ptr);
+  s.b = static_cast<const double*>(b->ptr);
+  const int nv = bicg ? 6 : 3;
+  k->vecs.alloc(size_t(nv) * size_t(s.n) * sizeof(double));
+  for (int i = 0; i < nv; ++i) s.w[i] = k->vecs.as<double>() + size_t(i) * size_t(s.n);
+  k->scal.alloc(SC_COUNT * sizeof(double));
+  LGPU_HIP_CHECK(hipMemsetAsync(k->scal.ptr, 0, SC_COUNT * sizeof(double), s.ctx->stream));
+  s.scal = k->scal.as<double>();
+  LGPU_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&k->host), SC_COUNT * sizeof(double), hipHostMallocDefault));
+  LGPU_HIP_CHECK(hipEventCreateWithFlags(&k->ev, hipEventDisableTiming));
+  s.host = k->host;
+  s.ev = k->ev;
+  s.start();
+  s.post();
+  s.wait();
+  if (s.err()) {
+    take_wait_error(s.ctx);
+    return wait_timeout();
+  }
+  *out = k.release();
+  return LGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int lgpu_cg_create(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x, int rere_every,
+                              lgpu_krylov* out) {
+  API_BEGIN
+  return krylov_create(false, op, precon, b, x, rere_every, out);
+  API_END
+}
+
+extern "C" int lgpu_bicgstab_create(lgpu_op op, lgpu_amg precon, lgpu_vec b, lgpu_vec x,
+                                    int rere_every, lgpu_krylov* out) {
+  API_BEGIN
+  return krylov_create(true, op, precon, b, x, rere_every, out);
+  API_END
+}
+
+extern "C" int lgpu_krylov_iterate(lgpu_krylov k) {
+  API_BEGIN
+  LGPU_REQUIRE(k, "NULL Krylov system");
+  KrylovSystem& s = k->s;
+  s.head();
+  s.rest();
+  s.post();
+  s.wait();
+  if (s.err()) {
+    take_wait_error(s.ctx);
+    return wait_timeout();
+  }
+  API_END
+}
+
+extern "C" int lgpu_krylov_residual_norm(lgpu_krylov k, double* out) {
+  API_BEGIN
+  LGPU_REQUIRE(k && out, "NULL argument");
+  *out = std::sqrt(k->s.rr());
+  API_END
+}
+
+extern "C" int lgpu_krylov_rho(lgpu_krylov k, double* out) {
+  API_BEGIN
+  LGPU_REQUIRE(k && out, "NULL argument");
+  *out = k->s.rho();
+  API_END
+}
+
+extern "C" int lgpu_krylov_iteration(lgpu_krylov k, int* out) {
+  API_BEGIN
+  LGPU_REQUIRE(k && out, "NULL argument");
+  *out = k->s.it;
+  API_END
+}
+
+extern "C" int lgpu_krylov_destroy(lgpu_krylov k) {
+  API_BEGIN
+  delete k;
   API_END
 }
 

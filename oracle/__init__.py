@@ -127,6 +127,8 @@ def lib():
             "or_cholesky_decompose": (C.c_int64, [cp, i64p, i64p, f64p]),
             "or_vcycle_f64": (None, [hp, f64p, f64p, C.c_int]),
             "or_vcycle_f32": (None, [hp, f32p, f32p, C.c_int]),
+            "or_cycle_f64": (None, [hp, f64p, f64p, f64p, C.c_int]),
+            "or_cycle_f32": (None, [hp, f32p, f32p, f32p, C.c_int]),
             "or_pcg": (C.c_int, [mp, hp, C.c_int, C.c_int, f64p, f64p, C.c_double, C.c_int, f64p]),
             "or_pbicgstab": (C.c_int, [mp, hp, C.c_int, C.c_int, f64p, f64p, C.c_double, C.c_int, f64p]),
             "or_amg_is_lu": (C.c_int, [hp]),
@@ -383,9 +385,14 @@ class Hierarchy:
         """True if the coarsest level is solved by LU (nonsymmetric hierarchy)."""
         return bool(lib().or_amg_is_lu(self.h))
 
-    def vcycle(self, r, ultra=False):
+    def vcycle(self, r, ultra=False, x_in=None):
+        """One cycle z = M r from zero, or from the initial guess x_in."""
         z = np.zeros_like(r)
-        _fn("or_vcycle", r.dtype)(self.h, _pf(r), _pf(z), int(ultra))
+        if x_in is None:
+            _fn("or_vcycle", r.dtype)(self.h, _pf(r), _pf(z), int(ultra))
+        else:
+            x_in = np.ascontiguousarray(x_in, dtype=r.dtype)
+            _fn("or_cycle", r.dtype)(self.h, _pf(x_in), _pf(r), _pf(z), int(ultra))
         return z
 
 

@@ -69,6 +69,17 @@ struct or_hierarchy {
    * iterations means 1 */
   double relax;
   int cycle_kind, cycle_iters;
+  /* the coarse levels' right-hand sides and solutions, owned by the
+   * hierarchy and kept between cycle applications as the reference's
+   * hierarchy vectors are (hierarchy/wright.hpp:227-228); zero at setup
+   * (the reference leaves them uninitialised).  A DECREASING cycle reads a
+   * coarse solution its coarser level computed in an earlier application.
+   * One set per precision, allocated at setup and shared by preconditioner
+   * views of the hierarchy. */
+  double* rhs64[32];
+  double* sol64[32];
+  float* rhs32[32];
+  float* sol32[32];
 };
 
 /* Thread segments of level l's smoother (parallel/execution-policy/adaptive.hpp
@@ -893,6 +904,12 @@ or_hierarchy* or_amg_setup(const or_mat* finest, const or_amg_params* p) {
     fine.csr = &h->mats[level + 1];
     ++level;
   }
+  for (int l = 1; l < h->nlev; ++l) {
+    h->rhs64[l] = (double*)calloc((size_t)h->size[l], sizeof(double));
+    h->sol64[l] = (double*)calloc((size_t)h->size[l], sizeof(double));
+    h->rhs32[l] = (float*)calloc((size_t)h->size[l], sizeof(float));
+    h->sol32[l] = (float*)calloc((size_t)h->size[l], sizeof(float));
+  }
   const or_csr* c = &h->mats[h->nlev - 1];
   const int64_t n = c->n;
   if (!c->symmetric) {
@@ -934,6 +951,12 @@ void or_amg_free(or_hierarchy* h) {
   free((void*)h->chol.vals);
   free(h->lu_l);
   free(h->lu_u);
+  for (int l = 0; l < 32; ++l) {
+    free(h->rhs64[l]);
+    free(h->sol64[l]);
+    free(h->rhs32[l]);
+    free(h->sol32[l]);
+  }
   free(h);
 }
 
@@ -951,15 +974,23 @@ void or_amg_stats(const or_hierarchy* h, int level, int64_t out[5]) {
 
 #define REAL double
 #define SUF(name) name##_f64
+#define STATE_RHS(h) ((h)->rhs64)
+#define STATE_SOL(h) ((h)->sol64)
 #include "oracle_vcycle.inc"
 #undef REAL
 #undef SUF
+#undef STATE_RHS
+#undef STATE_SOL
 
 #define REAL float
 #define SUF(name) name##_f32
+#define STATE_RHS(h) ((h)->rhs32)
+#define STATE_SOL(h) ((h)->sol32)
 #include "oracle_vcycle.inc"
 #undef REAL
 #undef SUF
+#undef STATE_RHS
+#undef STATE_SOL
 
 /* ---------------------------------------------------------------------------
  * Krylov drivers */

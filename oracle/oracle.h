@@ -173,6 +173,11 @@ int64_t or_cholesky_decompose(const or_csr* a, int64_t* rowptr, int64_t* cols, d
  * selects the vector/smoother precision (32: FP32 hierarchy, FP64 coarsest). */
 void or_vcycle_f64(const or_hierarchy* h, const double* r, double* z, int ultra);
 void or_vcycle_f32(const or_hierarchy* h, const float* r, float* z, int ultra);
+/* one cycle from the initial guess x_in (MultiGridSolver::Instance::apply,
+ * multigrid/solver.hpp:41-53): the finest pre-smoothing starts from x_in
+ * (NULL: zero, the same as or_vcycle) */
+void or_cycle_f64(const or_hierarchy* h, const double* x_in, const double* r, double* z, int ultra);
+void or_cycle_f32(const or_hierarchy* h, const float* x_in, const float* r, float* z, int ultra);
 
 /* Krylov drivers (conjugate-gradients/solver.hpp, bicgstab/solver.hpp) with the
  * true residual recomputed at iterations 8, 16, ... and stopping on

@@ -4,6 +4,12 @@
 // test/aux/materials.hpp:217-233).
 //   known solution 2: PCG-AMG and BiCGStab-AMG reach residual <= 1e-13 and
 //                     |x - 2|_inf <= 1e-13 within 32 iterations
+//   the reference's solver suite (test/aux/iterative.hpp:67-181) through
+//                     run_iterative_solver with its cg and gs iteration
+//                     callbacks (test/aux/iterative.hpp:25-65): cg, bicgstab,
+//                     sor, ssor1, sor5, ssor5, pcg_ssor1, pbicgstab_ssor1,
+//                     pcg_amg, pbicgstab_amg for both SOR variants, against
+//                     the bounds of test/shared-stencil.hpp:157-168
 //   Dirichlet rhs:    PCG-AMG residual |A x - b|_inf <= 2e-14
 //   CSR copy:         CSR SpMV equals the matrix-free SpMV
 //   adjoint:          outlet flux of a constant-coefficient values stencil
@@ -21,6 +27,8 @@
 #include <filesystem>
 #include <fstream>
 #include <iterator>
+#include <optional>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -44,6 +52,79 @@ static double inf_norm_diff(const std::vector<double>& a, double v) {
   double m = 0;
   for (double x : a) m = std::fmax(m, std::fabs(x - v));
   return m;
+}
+
+// test/aux/iterative.hpp:25-45: iterate until iter > iter_num or rho == 0 and
+// return the residual norm of the system instance
+template <typename TSolver>
+static double cg(const TSolver& solver, const lg::GpuMatrixBase& lhs, lg::GpuDenseVector<double>& sol,
+                 const lg::GpuDenseVector<double>& rhs, std::size_t iter_num) {
+  return lg::run_iterative_solver(solver, lhs, sol, rhs, [&](std::size_t iter, auto& inst) {
+    const double res = inst.residual_norm();
+    const double rho = inst.rho();
+    if (!std::isfinite(res)) throw std::runtime_error("non-finite residual norm");
+    return (iter > iter_num || rho == 0) ? std::optional<double>{res} : std::nullopt;
+  });
+}
+
+// test/aux/iterative.hpp:47-65: the Euclidean norm of lhs * sol - rhs after
+// every application of a stationary solver
+template <typename TSolver>
+static double gs(const TSolver& solver, const lg::GpuMatrixBase& lhs, lg::GpuDenseVector<double>& sol,
+                 const lg::GpuDenseVector<double>& rhs, std::size_t iter_num) {
+  lg::GpuDenseVector<double> r(sol.size(), sol.context());
+  return lg::run_iterative_solver(solver, lhs, sol, rhs, [&](std::size_t iter, auto&) {
+    lhs.residual(rhs, sol, r);
+    double sq = 0;
+    for (double v : r.to_host()) sq += v * v;
+    return iter > iter_num ? std::optional<double>{std::sqrt(sq)} : std::nullopt;
+  });
+}
+
+// test::suite (test/aux/iterative.hpp:67-181) with the bounds of
+// test/shared-stencil.hpp:157-168; returns the number of failed solvers
+template <lg::SorVariant tVariant>
+static int suite(const lg::GpuStencilLookupMatrix& lhs, lg::GpuDenseVector<double>& sol,
+                 const lg::GpuDenseVector<double>& rhs) {
+  const std::vector<double> bounds{1e-12, 1e-12, 1e-8, 1e-13, 1e-13, 1e-13, 1e-13, 1e-12, 1e-13, 1e-13};
+  const char* names[] = {"cg",    "bicgstab",  "sor",        "ssor1",   "sor5",
+                         "ssor5", "pcg_ssor1", "pbicgstab_ssor1", "pcg_amg", "pbicgstab_amg"};
+  int failed = 0;
+  std::size_t k = 0;
+  auto op = [&](double res) {
+    const double err = inf_norm_diff(sol.to_host(), 2.0);
+    const bool ok = res <= bounds[k] && err <= bounds[k];
+    if (!ok) {
+      std::printf("FAIL   %s: residual %.3e, error %.3e, bound %.1e\n", names[k], res, err, bounds[k]);
+      ++failed;
+    }
+    ++k;
+  };
+  auto zero = [&] { sol.fill(0.0); };
+  using Ssor = lg::GpuSsorSolver<double, tVariant>;
+  const Ssor ssor1(1.0, 1);
+  const lg::GpuMultiGridSolver<double> amg({}, tVariant);
+  zero();
+  op(cg(lg::GpuConjugateGradients<Ssor>(8), lhs, sol, rhs, 128));
+  zero();
+  op(cg(lg::GpuBiCgStab<Ssor>(8), lhs, sol, rhs, 128));
+  zero();
+  op(gs(lg::GpuSorSolver<double, lg::SorDirection::FORWARD, tVariant>(1.0), lhs, sol, rhs, 128));
+  zero();
+  op(gs(ssor1, lhs, sol, rhs, 128));
+  zero();
+  op(gs(lg::GpuSorSolver<double, lg::SorDirection::FORWARD, tVariant>(1.0, 5), lhs, sol, rhs, 128));
+  zero();
+  op(gs(Ssor(1.0, 5), lhs, sol, rhs, 128));
+  zero();
+  op(cg(lg::GpuConjugateGradients<Ssor>(8, &ssor1), lhs, sol, rhs, 128));
+  zero();
+  op(cg(lg::GpuBiCgStab<Ssor>(8, &ssor1), lhs, sol, rhs, 128));
+  zero();
+  op(cg(lg::GpuConjugateGradients<lg::GpuMultiGridSolver<double>>(8, &amg), lhs, sol, rhs, 32));
+  zero();
+  op(cg(lg::GpuBiCgStab<lg::GpuMultiGridSolver<double>>(8, &amg), lhs, sol, rhs, 32));
+  return failed;
 }
 
 int main(int argc, char** argv) {
@@ -77,6 +158,15 @@ int main(int argc, char** argv) {
       const double err = inf_norm_diff(x.to_host(), 2.0);
       expect(r.residual_norm() <= 1e-13 && err <= 1e-13,
              which == 0 ? "pcg_amg known solution 2" : "bicgstab_amg known solution 2", err);
+    }
+
+    // the reference's solver suite through iteration callbacks
+    {
+      lg::GpuDenseVector<double> sol(n, 0.0, ctx);
+      const int fr = suite<lg::SorVariant::REGULAR>(A, sol, b);
+      expect(fr == 0, "reference suite through callbacks, regular SOR", double(fr));
+      const int fu = suite<lg::SorVariant::ULTRA>(A, sol, b);
+      expect(fu == 0, "reference suite through callbacks, ultra SOR", double(fu));
     }
 
     // Dirichlet rhs (test/shared-stencil.hpp:192-239)

@@ -53,6 +53,6 @@ def test_cpp_header_compiles_against_the_c_abi(tmp_path):
     src = tmp_path / "use.cpp"
     src.write_text('#include "lineal-amd/lineal-amd.hpp"\n'
                    "int main() { lineal::gpu::HierarchyWrightParams p; return int(p.max_level_num) - 16; }\n")
-    res = subprocess.run([gxx, "-std=c++20", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+    res = subprocess.run([gxx, "-std=c++23", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
                           f"-I{os.path.join(ROOT, 'include')}", str(src)], capture_output=True, text=True)
     assert res.returncode == 0, res.stderr

@@ -1,6 +1,7 @@
 """The header-only C++ classes on the GPU: tests/cpp/stencil_api.cpp runs the
-reference's stencil test scenarios (known solution, Dirichlet residual, CSR
-copy, smoothers, adjoint closed form, effective diffusivity of the compressed
+reference's stencil test scenarios (known solution, the solver suite through
+run_iterative_solver's iteration callbacks, Dirichlet residual, CSR copy,
+smoothers, adjoint closed form, effective diffusivity of the compressed
 system) through them, and reads, writes and solves the reference's UFZ_CT_02
 files from tests/golden."""
 
@@ -22,4 +23,4 @@ def test_cpp_api_program():
     res = subprocess.run([EXE, golden], capture_output=True, text=True, timeout=120)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
-    assert res.stdout.count("ok  ") == 11
+    assert res.stdout.count("ok  ") == 13

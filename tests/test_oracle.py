@@ -1023,17 +1023,77 @@ def test_cycle_relax_against_dense_restatement(relax):
         assert np.abs(H.vcycle(r) - O.Hierarchy(S, H.params).vcycle(r)).max() > 1e-6
 
 
-@pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("iterations", [2, 3])
-def test_cycle_kinds_restart_revisited_levels(kind, iterations):
-    """A level's down step (pre-smoothing from zero, restriction) runs only at
-    the begin of its cycle info, and a level pushed again by its finer level
-    starts a new info: the repeated visits recompute the same coarse
-    correction from zero with an unchanged right-hand side, so every
-    DefaultCycle of either kind gives the V-cycle's result bit for bit."""
-    dims = (9, 8, 7)
-    S, _ = lognormal_stencil(dims, seed=2)
+def test_regular_cycles_restart_revisited_levels(iterations):
+    """DefaultCycle{REGULAR, N}: a level's down step (pre-smoothing from zero,
+    restriction) runs only at the begin of its cycle info, and a level pushed
+    again by its finer level starts a new info {N, N}: the repeated visits
+    recompute the same coarse correction from zero with an unchanged
+    right-hand side, so the result is the V-cycle's bit for bit."""
+    dims = (24, 20, 16)
+    S, _ = lognormal_stencil(dims, seed=1)
     r = problems.uniform(S.N, 6)
     v = O.Hierarchy(S, coarse_segment=64).vcycle(r)
-    w = O.Hierarchy(S, coarse_segment=64, cycle_kind=kind, cycle_iterations=iterations).vcycle(r)
-    np.testing.assert_array_equal(w, v)
+    H = O.Hierarchy(S, coarse_segment=64, cycle_kind=0, cycle_iterations=iterations)
+    assert H.level_num >= 5
+    np.testing.assert_array_equal(H.vcycle(r), v)
+
+
+def dense_truncated_cycle(A, maps, sizes, r, depth, level=0):
+    """The V-cycle with level `depth` (above the coarsest) smoothed only:
+    forward SOR from zero, a zero coarse correction, backward SOR."""
+    x = dense_sor(A, np.zeros(len(r)), r, 1.0, forward=True)
+    if level == len(sizes) - 1:
+        return np.linalg.solve(A, r)
+    if level < depth:
+        P = prolongator(maps[level], sizes[level + 1])
+        Ac = P.T @ A @ P
+        xc = dense_truncated_cycle(Ac, maps, sizes, P.T @ (r - A @ x), depth, level + 1)
+        x = x + P @ xc
+    return dense_sor(A, x, r, 1.0, forward=False)
+
+
+@pytest.mark.parametrize("iterations", [2, 3])
+def test_decreasing_cycles_stop_at_their_depth(iterations):
+    """DefaultCycle{DECREASING, N}: level l is first pushed with {N - l, N - l}
+    (get_coarser takes the remaining count), so level N is pushed with
+    {0, 0} and runs its down and up steps in one visit without visiting a
+    coarser level: its up step prolongs the hierarchy's stored coarse
+    solution, which no visit of this cycle writes (zero here; the
+    reference's hierarchy vectors start uninitialised).  With those zero,
+    the result is the V-cycle truncated at level N, restated densely; it is
+    the same on every application."""
+    dims = (12, 10, 9)
+    S, c = lognormal_stencil(dims, seed=1)
+    prm = O.AmgParams(16, 2, 1.2, 1.0 / 3.0, 1.0 / 3.0, 6, 9, 0)
+    H = O.Hierarchy(S, prm, cycle_kind=1, cycle_iterations=iterations)
+    assert H.level_num > iterations + 2
+    r = problems.uniform(S.N, 6)
+    sizes = [H.size(l) for l in range(H.level_num)]
+    maps = [H.agg_map(l) for l in range(H.level_num - 1)]
+    ref = dense_truncated_cycle(dense_operator(dims, c), maps, sizes, r, iterations)
+    got = H.vcycle(r)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())
+    np.testing.assert_array_equal(H.vcycle(r), got)
+    assert np.abs(got - O.Hierarchy(S, prm).vcycle(r)).max() > 1e-6
+
+
+def test_cycle_from_an_initial_guess_against_dense_restatement():
+    """MultiGridSolver::Instance::apply with an initial guess
+    (multigrid/solver.hpp:41-53): the finest pre-smoothing starts from it."""
+    dims = (7, 6, 5)
+    S, c = lognormal_stencil(dims, seed=9)
+    H = O.Hierarchy(S, O.AmgParams(16, 2, 1.2, 1.0 / 3.0, 1.0 / 3.0, 6, 9, 0))
+    A = dense_operator(dims, c)
+    r, x0 = problems.uniform(S.N, 4), problems.uniform(S.N, 5)
+    sizes = [H.size(l) for l in range(H.level_num)]
+    maps = [H.agg_map(l) for l in range(H.level_num - 1)]
+    # the two-grid cycle from x0 is x0 plus the cycle from zero of the
+    # residual equation only in exact arithmetic; restate it directly
+    x = dense_sor(A, x0, r, 1.0, forward=True)
+    P = prolongator(maps[0], sizes[1])
+    xc = dense_cycle(P.T @ A @ P, maps, sizes, P.T @ (r - A @ x), 1.0, 1)
+    ref = dense_sor(A, x + P @ xc, r, 1.0, forward=False)
+    got = H.vcycle(r, x_in=x0)
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_array_equal(H.vcycle(r, x_in=np.zeros(S.N)), H.vcycle(r))
