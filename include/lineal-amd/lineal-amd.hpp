@@ -650,23 +650,19 @@ class GpuMultiGridSolver {
   class Instance {
    public:
     Instance(const GpuMatrixBase& lhs, const GpuMultiGridSolver& s) : lhs_(&lhs) {
-      lgpu_amg_params p;
-      lgpu_amg_default_params(&p);
-      p.max_level_num = s.params_.max_level_num;
-      p.min_coarsen_size = s.params_.min_coarsen_size;
-      p.min_coarsen_factor = s.params_.min_coarsen_factor;
-      p.strong_threshold = s.params_.strong_threshold;
-      p.aggressive_strong_threshold = s.params_.aggressive_strong_threshold;
-      p.min_aggregate_size = s.params_.min_aggregate_size;
-      p.max_aggregate_size = s.params_.max_aggregate_size;
-      p.precision = std::is_same_v<TLevelReal, float> ? LGPU_F32 : LGPU_F64;
-      p.ultra = s.variant_ == SorVariant::ULTRA ? 1 : 0;
-      p.coarse_segment_rows = s.coarse_segment_rows_;
-      p.relax = s.relax_;
-      p.cycle_kind = int(s.cycle_.kind);
-      p.cycle_iterations = s.cycle_.iterations;
+      const lgpu_amg_params p = s.c_params();
       lgpu_amg h = nullptr;
       check(lgpu_amg_create(lhs.handle(), &p, &h));
+      h_.reset(h);
+    }
+    // over a hierarchy built elsewhere (e.g. the reference's own host setup):
+    // levels[l] maps level l to level l + 1 (lgpu_amg_create_from_levels)
+    Instance(const GpuMatrixBase& lhs, const GpuMultiGridSolver& s,
+             std::span<const lgpu_amg_level> levels)
+        : lhs_(&lhs) {
+      const lgpu_amg_params p = s.c_params();
+      lgpu_amg h = nullptr;
+      check(lgpu_amg_create_from_levels(lhs.handle(), &p, int(levels.size()), levels.data(), &h));
       h_.reset(h);
     }
     // ex-situ: one cycle from sol_in (null: the constant-zero initial guess,
@@ -708,6 +704,24 @@ class GpuMultiGridSolver {
     mutable std::unique_ptr<GpuDenseVector<TLevelReal>> aux_;
   };
   [[nodiscard]] Instance instantiate(const GpuMatrixBase& lhs) const { return Instance(lhs, *this); }
+  [[nodiscard]] lgpu_amg_params c_params() const {
+    lgpu_amg_params p;
+    lgpu_amg_default_params(&p);
+    p.max_level_num = params_.max_level_num;
+    p.min_coarsen_size = params_.min_coarsen_size;
+    p.min_coarsen_factor = params_.min_coarsen_factor;
+    p.strong_threshold = params_.strong_threshold;
+    p.aggressive_strong_threshold = params_.aggressive_strong_threshold;
+    p.min_aggregate_size = params_.min_aggregate_size;
+    p.max_aggregate_size = params_.max_aggregate_size;
+    p.precision = std::is_same_v<TLevelReal, float> ? LGPU_F32 : LGPU_F64;
+    p.ultra = variant_ == SorVariant::ULTRA ? 1 : 0;
+    p.coarse_segment_rows = coarse_segment_rows_;
+    p.relax = relax_;
+    p.cycle_kind = int(cycle_.kind);
+    p.cycle_iterations = cycle_.iterations;
+    return p;
+  }
 
  private:
   HierarchyWrightParams params_;

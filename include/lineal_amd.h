@@ -231,6 +231,35 @@ int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out
  * contexts or host threads) race.  Synchronises the stream like lgpu_op_ssor. */
 int lgpu_op_ssor_noalloc(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
                          int ultra);
+/* One transfer of a hierarchy built elsewhere: the aggregate map of a level
+ * (n fine rows; each row's aggregate, or 0xFFFFFFFE for an isolated row), the
+ * coarse -> fine lists that fix the order of the restriction sums
+ * (transport/vector-shared.hpp:49-56; c2f_ptr with coarse_n + 1 entries and
+ * c2f_idx, or both NULL for ascending fine rows), and the coarse level's
+ * matrix (CSR, sorted columns, diagonal stored, f64 values). */
+typedef struct {
+  const uint32_t* agg;
+  const int64_t* c2f_ptr;
+  const int32_t* c2f_idx;
+  int64_t coarse_n;
+  const int64_t* rowptr;
+  const int32_t* cols;
+  const double* vals;
+} lgpu_amg_level;
+/* MultiGridSolver over a hierarchy built elsewhere, e.g. by the reference's
+ * own host setup (HierarchyWrightInstance::create, hierarchy/wright.hpp:145-309):
+ * transfer_num transfers, levels[l] mapping level l (level 0: `finest`) to
+ * level l + 1.  The library factors the coarsest matrix (Cholesky, or LU for a
+ * nonsymmetric finest operator) and uploads everything; params supply the
+ * precision (coarse values rounded to f32 for an f32 hierarchy), the
+ * smoother variant, the coarse segments, relax and the cycle.  The maps and
+ * matrices are validated (ranges, consistency of the lists with the map,
+ * sorted columns with a diagonal). */
+int lgpu_amg_create_from_levels(lgpu_op finest, const lgpu_amg_params* params, int transfer_num,
+                                const lgpu_amg_level* levels, lgpu_amg* out);
+/* host copy of transfer `level`'s coarse -> fine lists (level + 1 < level
+ * count; c2f_ptr: coarse rows + 1 entries, c2f_idx: its last entry) */
+int lgpu_amg_transfer(lgpu_amg amg, int level, int64_t* c2f_ptr, int32_t* c2f_idx);
 /* SsorSolver<Real, FixedIterationManager, variant>{omega, IterMan{iterations}}
  * as a Krylov preconditioner (gauss-seidel-variant/solver.hpp:94-96,
  * broom.hpp:159-171; test/aux/iterative.hpp:143-165): a preconditioner handle

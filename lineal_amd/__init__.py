@@ -90,6 +90,18 @@ class _AmgParams(C.Structure):
 CYCLE_REGULAR, CYCLE_DECREASING = 0, 1
 
 
+class _AmgLevel(C.Structure):
+    _fields_ = [
+        ("agg", _u32p),
+        ("c2f_ptr", _i64p),
+        ("c2f_idx", _i32p),
+        ("coarse_n", C.c_int64),
+        ("rowptr", _i64p),
+        ("cols", _i32p),
+        ("vals", _f64p),
+    ]
+
+
 class _KrylovParams(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_it", C.c_int), ("rere_every", C.c_int)]
 
@@ -139,6 +151,8 @@ SIGNATURES = {
     "lgpu_ssor_precon_create": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int, C.POINTER(_vp)]),
     "lgpu_amg_default_params": (None, [C.POINTER(_AmgParams)]),
     "lgpu_amg_create": (C.c_int, [_vp, C.POINTER(_AmgParams), C.POINTER(_vp)]),
+    "lgpu_amg_create_from_levels": (C.c_int, [_vp, C.POINTER(_AmgParams), C.c_int, _vp, C.POINTER(_vp)]),
+    "lgpu_amg_transfer": (C.c_int, [_vp, C.c_int, _i64p, _i32p]),
     "lgpu_amg_destroy": (C.c_int, [_vp]),
     "lgpu_amg_level_num": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "lgpu_amg_level_size": (C.c_int, [_vp, C.c_int, _i64p]),
@@ -632,14 +646,69 @@ class MultiGridSolver:
         return MultiGridInstance(lhs, self)
 
 
+@dataclass
+class AmgLevel:
+    """One transfer of a hierarchy built elsewhere: the aggregate map of the
+    finer level (0xFFFFFFFE: isolated), optional coarse -> fine lists in
+    summation order, and the coarse matrix as CSR arrays."""
+
+    agg: np.ndarray
+    rowptr: np.ndarray
+    cols: np.ndarray
+    vals: np.ndarray
+    c2f_ptr: np.ndarray | None = None
+    c2f_idx: np.ndarray | None = None
+
+
 class MultiGridInstance:
-    def __init__(self, lhs: _Matrix, solver: MultiGridSolver):
+    def __init__(self, lhs: _Matrix, solver: MultiGridSolver, levels: list[AmgLevel] | None = None):
         self.lhs = lhs
         p = solver.wright._struct(_dtype_code(solver.precision), solver.ultra, solver.coarse_segment_rows,
                                   solver.relax, solver.cycle)
         h = _vp()
-        _check(lib().lgpu_amg_create(lhs.h, C.byref(p), C.byref(h)))
+        if levels is None:
+            _check(lib().lgpu_amg_create(lhs.h, C.byref(p), C.byref(h)))
+        else:
+            keep, arr = [], (_AmgLevel * len(levels))()
+            for k, lv in enumerate(levels):
+                agg = np.ascontiguousarray(lv.agg, np.uint32)
+                rp = np.ascontiguousarray(lv.rowptr, np.int64)
+                cols = np.ascontiguousarray(lv.cols, np.int32)
+                vals = np.ascontiguousarray(lv.vals, np.float64)
+                keep += [agg, rp, cols, vals]
+                arr[k].agg, arr[k].rowptr = agg.ctypes.data_as(_u32p), rp.ctypes.data_as(_i64p)
+                arr[k].cols, arr[k].vals = cols.ctypes.data_as(_i32p), vals.ctypes.data_as(_f64p)
+                arr[k].coarse_n = rp.size - 1
+                if lv.c2f_ptr is not None:
+                    cp = np.ascontiguousarray(lv.c2f_ptr, np.int64)
+                    ci = np.ascontiguousarray(lv.c2f_idx, np.int32)
+                    keep += [cp, ci]
+                    arr[k].c2f_ptr, arr[k].c2f_idx = cp.ctypes.data_as(_i64p), ci.ctypes.data_as(_i32p)
+            _check(lib().lgpu_amg_create_from_levels(lhs.h, C.byref(p), len(levels), arr, C.byref(h)))
         self.h = h
+
+    @classmethod
+    def from_levels(cls, lhs: _Matrix, levels: list[AmgLevel], solver: MultiGridSolver | None = None):
+        """A hierarchy built elsewhere (lgpu_amg_create_from_levels)."""
+        return cls(lhs, solver or MultiGridSolver(), levels)
+
+    def transfer(self, level: int):
+        """(c2f_ptr, c2f_idx) of transfer `level`."""
+        nc = self.level_size(level + 1)
+        ptr = np.empty(nc + 1, np.int64)
+        _check(lib().lgpu_amg_transfer(self.h, level, ptr.ctypes.data_as(_i64p), None))
+        idx = np.empty(max(int(ptr[-1]), 1), np.int32)
+        _check(lib().lgpu_amg_transfer(self.h, level, ptr.ctypes.data_as(_i64p), idx.ctypes.data_as(_i32p)))
+        return ptr, idx[: int(ptr[-1])]
+
+    def export_levels(self) -> list[AmgLevel]:
+        """This hierarchy's transfers, as lgpu_amg_create_from_levels takes them."""
+        out = []
+        for l in range(self.level_num - 1):
+            rp, cols, vals = self.matrix(l + 1)
+            ptr, idx = self.transfer(l)
+            out.append(AmgLevel(self.aggregates(l), rp, cols, vals, ptr, idx))
+        return out
 
     @property
     def level_num(self) -> int:

@@ -1203,9 +1203,9 @@ extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {
   p->cycle_iterations = 1;
 }
 
-extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out) {
-  API_BEGIN
-  LGPU_REQUIRE(finest && params && out, "NULL argument");
+namespace {
+
+void check_amg_params(lgpu_op finest, const lgpu_amg_params* params) {
   LGPU_REQUIRE(params->precision == LGPU_F64 || params->precision == LGPU_F32, "unknown precision");
   LGPU_REQUIRE(params->max_aggregate_size >= 1 && params->min_aggregate_size >= 1, "invalid aggregate sizes");
   LGPU_REQUIRE(params->coarse_segment_rows >= 0 && params->coarse_segment_rows <= kMaxSegmentRows,
@@ -1215,28 +1215,11 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
   LGPU_REQUIRE(params->cycle_kind == LGPU_CYCLE_REGULAR || params->cycle_kind == LGPU_CYCLE_DECREASING,
                "unknown cycle kind");
   LGPU_REQUIRE(params->cycle_iterations >= 1, "cycle_iterations must be at least 1");
-  const auto t0 = std::chrono::steady_clock::now();
-  auto amg = std::make_unique<lgpu_amg_s>();
-  amg->ctx = finest->ctx;
-  amg->finest = finest;
-  amg->params = *params;
-  amg->f32 = params->precision == LGPU_F32;
-  setup::Params sp;
-  sp.max_level_num = params->max_level_num;
-  sp.min_coarsen_size = params->min_coarsen_size;
-  sp.min_coarsen_factor = params->min_coarsen_factor;
-  sp.strong_threshold = params->strong_threshold;
-  sp.aggressive_strong_threshold = params->aggressive_strong_threshold;
-  sp.min_aggregate_size = params->min_aggregate_size;
-  sp.max_aggregate_size = params->max_aggregate_size;
-  sp.coarse_f32 = amg->f32;
-  if (finest->is_csr) {
-    amg->host = setup::build(finest->host_csr, sp);
-  } else {
-    setup::RowMatrix rows;
-    stencil_host_rows(finest, rows);
-    amg->host = setup::build(rows, sp);
-  }
+}
+
+// Uploads the host hierarchy amg->host (levels, transfers, coarse matrices
+// and the coarsest factor) and allocates the level vectors.
+void upload_hierarchy(lgpu_amg amg) {
   LGPU_REQUIRE(amg->host.spd, amg->host.lu ? "the coarsest LU factorisation met a zero pivot"
                                           : "coarsest matrix is not positive definite");
   const setup::Hierarchy& h = amg->host;
@@ -1283,13 +1266,139 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
     LGPU_HIP_CHECK(hipMemcpy(amg->upper.ptr, h.upper_dense.data(), size_t(nc * nc) * sizeof(double), hipMemcpyHostToDevice));
   }
   if (amg->f32) {
-    finest->ensure_f32();
+    amg->finest->ensure_f32();
     amg->r32.alloc(size_t(h.sizes[0]) * sizeof(float));
     amg->z32.alloc(size_t(h.sizes[0]) * sizeof(float));
   }
   LGPU_HIP_CHECK(hipDeviceSynchronize());
+}
+
+}  // namespace
+
+extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out) {
+  API_BEGIN
+  LGPU_REQUIRE(finest && params && out, "NULL argument");
+  check_amg_params(finest, params);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto amg = std::make_unique<lgpu_amg_s>();
+  amg->ctx = finest->ctx;
+  amg->finest = finest;
+  amg->params = *params;
+  amg->f32 = params->precision == LGPU_F32;
+  setup::Params sp;
+  sp.max_level_num = params->max_level_num;
+  sp.min_coarsen_size = params->min_coarsen_size;
+  sp.min_coarsen_factor = params->min_coarsen_factor;
+  sp.strong_threshold = params->strong_threshold;
+  sp.aggressive_strong_threshold = params->aggressive_strong_threshold;
+  sp.min_aggregate_size = params->min_aggregate_size;
+  sp.max_aggregate_size = params->max_aggregate_size;
+  sp.coarse_f32 = amg->f32;
+  if (finest->is_csr) {
+    amg->host = setup::build(finest->host_csr, sp);
+  } else {
+    setup::RowMatrix rows;
+    stencil_host_rows(finest, rows);
+    amg->host = setup::build(rows, sp);
+  }
+  upload_hierarchy(amg.get());
   amg->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *out = amg.release();
+  API_END
+}
+
+extern "C" int lgpu_amg_create_from_levels(lgpu_op finest, const lgpu_amg_params* params,
+                                           int transfer_num, const lgpu_amg_level* levels,
+                                           lgpu_amg* out) {
+  API_BEGIN
+  LGPU_REQUIRE(finest && params && out && levels, "NULL argument");
+  LGPU_REQUIRE(transfer_num >= 1 && transfer_num < 32, "transfer_num must lie in 1..31");
+  check_amg_params(finest, params);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto amg = std::make_unique<lgpu_amg_s>();
+  amg->ctx = finest->ctx;
+  amg->finest = finest;
+  amg->params = *params;
+  amg->f32 = params->precision == LGPU_F32;
+  setup::Hierarchy& h = amg->host;
+  const bool symmetric = finest->is_csr ? finest->host_csr.symmetric : finest->desc.adv == 0.0;
+  int64_t n = finest->rows();
+  h.sizes.push_back(n);
+  for (int l = 0; l < transfer_num; ++l) {
+    const lgpu_amg_level& in = levels[l];
+    LGPU_REQUIRE(in.agg && in.rowptr && in.cols && in.vals, "NULL level array");
+    const int64_t nc = in.coarse_n;
+    LGPU_REQUIRE(nc >= 1 && nc < n + 1, "a coarse level must have between 1 and the fine level's rows");
+    setup::Level t;
+    t.agg.assign(in.agg, in.agg + n);
+    std::vector<int64_t> count(size_t(nc), 0);
+    for (int64_t v = 0; v < n; ++v) {
+      const uint32_t a = t.agg[size_t(v)];
+      LGPU_REQUIRE(a == setup::kIsolated || int64_t(a) < nc, "aggregate index out of range");
+      if (a != setup::kIsolated) ++count[a];
+    }
+    t.c2f_ptr.assign(size_t(nc + 1), 0);
+    for (int64_t c = 0; c < nc; ++c) {
+      LGPU_REQUIRE(count[size_t(c)] > 0, "every aggregate needs a member");
+      t.c2f_ptr[size_t(c + 1)] = t.c2f_ptr[size_t(c)] + count[size_t(c)];
+    }
+    t.c2f_idx.resize(size_t(t.c2f_ptr[size_t(nc)]));
+    if (in.c2f_ptr && in.c2f_idx) {
+      // the caller's coarse -> fine lists fix the order of the restriction sums
+      for (int64_t c = 0; c <= nc; ++c)
+        LGPU_REQUIRE(in.c2f_ptr[c] == t.c2f_ptr[size_t(c)], "c2f_ptr does not match the aggregate map");
+      for (int64_t c = 0; c < nc; ++c)
+        for (int64_t k = in.c2f_ptr[c]; k < in.c2f_ptr[c + 1]; ++k) {
+          const int32_t v = in.c2f_idx[k];
+          LGPU_REQUIRE(v >= 0 && v < n && int64_t(t.agg[size_t(v)]) == c,
+                       "c2f_idx does not match the aggregate map");
+          t.c2f_idx[size_t(k)] = v;
+        }
+    } else {
+      std::vector<int64_t> pos(t.c2f_ptr.begin(), t.c2f_ptr.end() - 1);
+      for (int64_t v = 0; v < n; ++v)
+        if (t.agg[size_t(v)] != setup::kIsolated) t.c2f_idx[size_t(pos[t.agg[size_t(v)]]++)] = int32_t(v);
+    }
+    setup::RowMatrix m;
+    m.n = nc;
+    m.f32 = amg->f32;
+    m.symmetric = symmetric;
+    m.rowptr.assign(in.rowptr, in.rowptr + nc + 1);
+    LGPU_REQUIRE(m.rowptr[0] == 0 && m.rowptr[size_t(nc)] >= nc, "malformed coarse row pointer");
+    const int64_t nnz = m.rowptr[size_t(nc)];
+    m.cols.assign(in.cols, in.cols + nnz);
+    m.vals.resize(size_t(nnz));
+    for (int64_t i = 0; i < nc; ++i) {
+      bool diag = false;
+      LGPU_REQUIRE(m.rowptr[size_t(i + 1)] >= m.rowptr[size_t(i)], "malformed coarse row pointer");
+      for (int64_t k = m.rowptr[size_t(i)]; k < m.rowptr[size_t(i + 1)]; ++k) {
+        LGPU_REQUIRE(in.cols[k] >= 0 && in.cols[k] < nc, "coarse column out of range");
+        if (k > m.rowptr[size_t(i)]) LGPU_REQUIRE(in.cols[k] > in.cols[k - 1], "coarse columns must be sorted and unique");
+        diag = diag || in.cols[k] == i;
+        m.vals[size_t(k)] = amg->f32 ? double(float(in.vals[k])) : in.vals[k];
+      }
+      LGPU_REQUIRE(diag, "every coarse row must store its diagonal");
+    }
+    h.transfer.push_back(std::move(t));
+    h.coarse.push_back(std::move(m));
+    h.sizes.push_back(nc);
+    n = nc;
+  }
+  const auto tc = std::chrono::steady_clock::now();
+  setup::factor_coarsest(h);
+  h.seconds_coarsest = std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
+  upload_hierarchy(amg.get());
+  amg->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out = amg.release();
+  API_END
+}
+
+extern "C" int lgpu_amg_transfer(lgpu_amg amg, int level, int64_t* c2f_ptr, int32_t* c2f_idx) {
+  API_BEGIN
+  LGPU_REQUIRE(amg && level >= 0 && level + 1 < int(amg->lv.size()), "invalid level");
+  const setup::Level& t = amg->host.transfer[size_t(level)];
+  if (c2f_ptr) std::memcpy(c2f_ptr, t.c2f_ptr.data(), t.c2f_ptr.size() * sizeof(int64_t));
+  if (c2f_idx) std::memcpy(c2f_idx, t.c2f_idx.data(), t.c2f_idx.size() * sizeof(int32_t));
   API_END
 }
 

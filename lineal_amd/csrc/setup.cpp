@@ -522,14 +522,21 @@ Hierarchy build(const RowMatrix& finest, const Params& p) {
     fine = &h.coarse.back();
   }
   const auto tc = Clock::now();
+  factor_coarsest(h);
+  h.seconds_coarsest = since(tc);
+  return h;
+}
+
+// The coarsest solver's factorisation: Cholesky for a symmetric coarsest
+// matrix, row-wise LU for a nonsymmetric one (BiCgStab's hierarchies,
+// test/aux/iterative.hpp:122-181)
+void factor_coarsest(Hierarchy& h) {
   if (!h.coarse.back().symmetric) {
-    // nonsymmetric hierarchies (BiCgStab) take an LU coarsest solver
-    // (test/aux/iterative.hpp:122-181)
     h.lu = true;
     h.spd = lu_dense(h.coarse.back(), h.chol_dense, h.upper_dense);
-    h.seconds_coarsest = since(tc);
-    return h;
+    return;
   }
+  h.lu = false;
   RowMatrix lower;
   h.spd = cholesky(h.coarse.back(), lower);
   const int64_t n = h.coarse.back().n;
@@ -538,8 +545,6 @@ Hierarchy build(const RowMatrix& finest, const Params& p) {
     for (int64_t i = 0; i < n; ++i)
       for (int64_t k = lower.rowptr[i]; k < lower.rowptr[i + 1]; ++k)
         h.chol_dense[size_t(i * n + lower.cols[k])] = lower.vals[k];
-  h.seconds_coarsest = since(tc);
-  return h;
 }
 
 }  // namespace lgpu::setup

@@ -62,6 +62,10 @@ constexpr uint32_t kIsolated = 0xFFFFFFFEu;
 
 Hierarchy build(const RowMatrix& finest, const Params& p);
 
+// The coarsest factor of a hierarchy whose levels are set (Cholesky or, for a
+// nonsymmetric coarsest matrix, LU); sets lu, spd, chol_dense, upper_dense.
+void factor_coarsest(Hierarchy& h);
+
 // Sparse Cholesky-Banachiewicz of a CSR matrix (column order rows); returns
 // false if a pivot is not positive.  Output rows in column order, diagonal last.
 bool cholesky(const RowMatrix& a, RowMatrix& lower);

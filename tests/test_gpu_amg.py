@@ -326,3 +326,49 @@ def test_cycle_relax_and_kinds_equal_oracle(relax, cycle, f32):
             z = M.apply(la.DenseVector(r), la.DenseVector(S.N)).numpy()
             np.testing.assert_array_equal(z, H.vcycle(r))
         r = problems.uniform(S.N, 9)
+
+
+@pytest.mark.parametrize("f32", [False, True])
+@pytest.mark.parametrize("kind", ["values", "spheres"])
+def test_hierarchy_import_round_trip_and_oracle_levels(f32, kind):
+    """lgpu_amg_create_from_levels: the library's own hierarchy exported and
+    imported again gives a bit-identical cycle; the oracle's hierarchy (the
+    host setup restated from the reference) imported with its coarse -> fine
+    lists gives the oracle's cycle bit for bit."""
+    dims = (24, 20, 16) if kind == "values" else (24, 24, 24)
+    A, S = pair(dims, kind)
+    prec = np.float32 if f32 else np.float64
+    solver = la.MultiGridSolver(precision=prec)
+    M = solver.instantiate(A)
+    M2 = la.MultiGridInstance.from_levels(A, M.export_levels(), solver)
+    assert M2.level_num == M.level_num
+    H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=64)
+    levels = []
+    for l in range(H.level_num - 1):
+        ptr, idx = H.c2f(l)
+        m = H.matrix(l + 1)
+        levels.append(la.AmgLevel(H.agg_map(l), m.rowptr, m.cols, m.vals, ptr, idx))
+    M3 = la.MultiGridInstance.from_levels(A, levels, solver)
+    r = problems.uniform(S.N, 12)
+    rv = r.astype(prec)
+    zs = [X.apply(la.DenseVector(rv), la.DenseVector(S.N, prec)).numpy() for X in (M, M2, M3)]
+    np.testing.assert_array_equal(zs[1], zs[0])
+    np.testing.assert_array_equal(zs[2], H.vcycle(rv))
+
+
+def test_hierarchy_import_rejects_inconsistent_levels():
+    dims = (12, 10, 9)
+    A, S = pair(dims)
+    M = la.MultiGridSolver().instantiate(A)
+    levels = M.export_levels()
+    bad = levels[0]
+    agg = bad.agg.copy()
+    agg[0] = bad.rowptr.size + 5  # out of range
+    with pytest.raises(la.LinealError):
+        la.MultiGridInstance.from_levels(A, [la.AmgLevel(agg, bad.rowptr, bad.cols, bad.vals)] + levels[1:])
+    idx = bad.c2f_idx.copy()
+    idx[[0, -1]] = idx[[-1, 0]]  # lists no longer match the map
+    if bad.agg[idx[0]] != bad.agg[idx[-1]]:
+        with pytest.raises(la.LinealError):
+            la.MultiGridInstance.from_levels(A, [la.AmgLevel(bad.agg, bad.rowptr, bad.cols, bad.vals,
+                                                             bad.c2f_ptr, idx)] + levels[1:])
