@@ -620,6 +620,7 @@ struct HierarchyWrightParams {
   double aggressive_strong_threshold = 1.0 / 3.0;
   int64_t min_aggregate_size = 6;
   int64_t max_aggregate_size = 9;
+  bool setup_on_device = true;  // criterion and Galerkin products on the GPU
 };
 
 // DefaultCycle{kind, iterations} (multigrid/cycle.hpp:24-79)
@@ -720,6 +721,7 @@ class GpuMultiGridSolver {
     p.relax = relax_;
     p.cycle_kind = int(cycle_.kind);
     p.cycle_iterations = cycle_.iterations;
+    p.setup_on_device = params_.setup_on_device ? 1 : 0;
     return p;
   }
 

@@ -204,6 +204,11 @@ typedef struct {
   double relax;
   int cycle_kind;
   int cycle_iterations;
+  /* where the order-independent setup phases run: 1 (default) the strength
+   * criterion and the Galerkin products on the GPU, 0 on host threads; the
+   * greedy aggregation runs on the host either way, and the hierarchy is the
+   * same bit for bit */
+  int setup_on_device;
 } lgpu_amg_params;
 enum { LGPU_CYCLE_REGULAR = 0, LGPU_CYCLE_DECREASING = 1 };
 

@@ -84,6 +84,7 @@ class _AmgParams(C.Structure):
         ("relax", C.c_double),
         ("cycle_kind", C.c_int),
         ("cycle_iterations", C.c_int),
+        ("setup_on_device", C.c_int),
     ]
 
 
@@ -612,13 +613,17 @@ class HierarchyWrightParams:
     aggressive_strong_threshold: float = 1.0 / 3.0
     min_aggregate_size: int = 6
     max_aggregate_size: int = 9
+    # the strength criterion and Galerkin products on the GPU (True) or on
+    # host threads; the hierarchy is the same either way
+    setup_on_device: bool = True
 
     def _struct(self, precision: int, ultra: bool, coarse_segment_rows: int, relax=1.0,
                 cycle=(CYCLE_REGULAR, 1)):
         return _AmgParams(self.max_level_num, self.min_coarsen_size, self.min_coarsen_factor,
                           self.strong_threshold, self.aggressive_strong_threshold,
                           self.min_aggregate_size, self.max_aggregate_size, precision, int(ultra),
-                          int(coarse_segment_rows), float(relax), int(cycle[0]), int(cycle[1]))
+                          int(coarse_segment_rows), float(relax), int(cycle[0]), int(cycle[1]),
+                          int(self.setup_on_device))
 
 
 class MultiGridSolver:

@@ -1201,6 +1201,7 @@ extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {
   p->relax = 1.0;
   p->cycle_kind = LGPU_CYCLE_REGULAR;
   p->cycle_iterations = 1;
+  p->setup_on_device = 1;
 }
 
 namespace {
@@ -1294,12 +1295,14 @@ extern "C" int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lg
   sp.min_aggregate_size = params->min_aggregate_size;
   sp.max_aggregate_size = params->max_aggregate_size;
   sp.coarse_f32 = amg->f32;
+  std::unique_ptr<setup::Accel> accel;
+  if (params->setup_on_device) accel = make_device_setup(finest->ctx->stream);
   if (finest->is_csr) {
-    amg->host = setup::build(finest->host_csr, sp);
+    amg->host = setup::build(finest->host_csr, sp, accel.get());
   } else {
     setup::RowMatrix rows;
     stencil_host_rows(finest, rows);
-    amg->host = setup::build(rows, sp);
+    amg->host = setup::build(rows, sp, accel.get());
   }
   upload_hierarchy(amg.get());
   amg->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

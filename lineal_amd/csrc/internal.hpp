@@ -4,12 +4,19 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "lineal_amd.h"
 
 namespace lgpu {
+
+namespace setup {
+struct Accel;
+}
+// the AMG setup's criterion and Galerkin phases on the device (k_setup.hip)
+std::unique_ptr<setup::Accel> make_device_setup(hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // error reporting

@@ -43,6 +43,24 @@ struct Graph {
   bool symmetric = true, f32 = false;
 };
 
+// The edges (off-diagonal entries in row order) without values or reverse
+// edges: what the aggregation reads when the criterion ran elsewhere.
+void build_edges(const RowMatrix& m, Graph& g, int threads) {
+  g.n = m.n;
+  g.symmetric = m.symmetric;
+  g.f32 = m.f32;
+  g.eptr.assign(size_t(m.n + 1), 0);
+  for (int64_t i = 0; i < m.n; ++i) g.eptr[i + 1] = g.eptr[i] + (m.rowptr[i + 1] - m.rowptr[i] - 1);
+  g.head.resize(size_t(g.eptr[m.n]));
+  parallel_for(m.n, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      int64_t k = g.eptr[i];
+      for (int64_t p = m.rowptr[i]; p < m.rowptr[i + 1]; ++p)
+        if (m.cols[p] != i) g.head[k++] = m.cols[p];
+    }
+  });
+}
+
 void build_graph(const RowMatrix& m, Graph& g, int threads) {
   g.n = m.n;
   g.symmetric = m.symmetric;
@@ -484,7 +502,7 @@ bool lu_dense(const RowMatrix& a, std::vector<double>& lower, std::vector<double
 }
 
 // HierarchyWrightInstance::create_impl (multigrid/hierarchy/wright.hpp:120-143, 145-309)
-Hierarchy build(const RowMatrix& finest, const Params& p) {
+Hierarchy build(const RowMatrix& finest, const Params& p, Accel* accel) {
   using Clock = std::chrono::steady_clock;
   auto since = [](Clock::time_point t0) {
     return std::chrono::duration<double>(Clock::now() - t0).count();
@@ -497,15 +515,26 @@ Hierarchy build(const RowMatrix& finest, const Params& p) {
   for (int level = 0;; ++level) {
     auto t0 = Clock::now();
     Graph g;
-    build_graph(*fine, g, p.threads);
-    criterion(g, aggressive ? p.aggressive_strong_threshold : p.strong_threshold, p.threads);
+    const double theta = aggressive ? p.aggressive_strong_threshold : p.strong_threshold;
+    if (accel) {
+      build_edges(*fine, g, p.threads);
+      StrengthFlags f;
+      accel->criterion(*fine, theta, f);
+      g.depends = std::move(f.depends);
+      g.influences = std::move(f.influences);
+      g.isolated = std::move(f.isolated);
+    } else {
+      build_graph(*fine, g, p.threads);
+      criterion(g, theta, p.threads);
+    }
     h.seconds_criterion += since(t0);
     t0 = Clock::now();
     Aggregator aggr(g, p.min_aggregate_size, p.max_aggregate_size, aggressive);
     h.transfer.push_back(aggr.run());
     h.seconds_aggregation += since(t0);
     t0 = Clock::now();
-    h.coarse.push_back(galerkin(*fine, h.transfer.back(), p.coarse_f32, p.threads));
+    h.coarse.push_back(accel ? accel->galerkin(*fine, h.transfer.back(), p.coarse_f32)
+                             : galerkin(*fine, h.transfer.back(), p.coarse_f32, p.threads));
     h.seconds_galerkin += since(t0);
     const int64_t nc = h.coarse.back().n;
     h.sizes.push_back(nc);

@@ -60,7 +60,23 @@ struct Hierarchy {
 constexpr uint32_t kUnaggregated = 0xFFFFFFFFu;
 constexpr uint32_t kIsolated = 0xFFFFFFFEu;
 
-Hierarchy build(const RowMatrix& finest, const Params& p);
+// Strength flags of a level: per off-diagonal entry in row order (depends,
+// influences) and per vertex (isolated).
+struct StrengthFlags {
+  std::vector<uint8_t> depends, influences, isolated;
+};
+
+// Another executor of the setup phases that do not depend on a visiting
+// order: the strength criterion and the Galerkin product.  Their results
+// must equal the host's bit for bit (the same operations in the same order
+// per vertex / per coarse row); the greedy aggregation stays on the host.
+struct Accel {
+  virtual ~Accel() = default;
+  virtual void criterion(const RowMatrix& m, double theta, StrengthFlags& out) = 0;
+  virtual RowMatrix galerkin(const RowMatrix& fine, const Level& lv, bool coarse_f32) = 0;
+};
+
+Hierarchy build(const RowMatrix& finest, const Params& p, Accel* accel = nullptr);
 
 // The coarsest factor of a hierarchy whose levels are set (Cholesky or, for a
 // nonsymmetric coarsest matrix, LU); sets lu, spd, chol_dense, upper_dense.

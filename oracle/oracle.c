@@ -1006,7 +1006,26 @@ static or_hierarchy* precon_view(const or_hierarchy* h, const or_mat* a) {
   or_hierarchy* v = (or_hierarchy*)malloc(sizeof(or_hierarchy));
   *v = *h;
   v->finest = *a;
+  /* the solve's own coarse vectors (zero), so that solves with one
+   * hierarchy may run concurrently */
+  for (int l = 1; l < v->nlev; ++l) {
+    v->rhs64[l] = (double*)calloc((size_t)v->size[l], sizeof(double));
+    v->sol64[l] = (double*)calloc((size_t)v->size[l], sizeof(double));
+    v->rhs32[l] = (float*)calloc((size_t)v->size[l], sizeof(float));
+    v->sol32[l] = (float*)calloc((size_t)v->size[l], sizeof(float));
+  }
   return v;
+}
+
+static void precon_view_free(or_hierarchy* v) {
+  if (!v) return;
+  for (int l = 1; l < v->nlev; ++l) {
+    free(v->rhs64[l]);
+    free(v->sol64[l]);
+    free(v->rhs32[l]);
+    free(v->sol32[l]);
+  }
+  free(v);
 }
 
 /* SsorBroom::sweep (gauss-seidel-variant/solver.hpp:45-88) repeated `iters`
@@ -1099,7 +1118,7 @@ int or_pcg(const or_mat* a, const or_hierarchy* h, int precon, int ultra, const 
   free(r);
   free(p);
   free(q);
-  free(hv);
+  precon_view_free(hv);
   return it;
 }
 
@@ -1153,7 +1172,7 @@ int or_pbicgstab(const or_mat* a, const or_hierarchy* h, int precon, int ultra,
   free(v);
   free(t);
   free(y);
-  free(hv);
+  precon_view_free(hv);
   return it;
 }
 

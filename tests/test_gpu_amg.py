@@ -372,3 +372,30 @@ def test_hierarchy_import_rejects_inconsistent_levels():
         with pytest.raises(la.LinealError):
             la.MultiGridInstance.from_levels(A, [la.AmgLevel(bad.agg, bad.rowptr, bad.cols, bad.vals,
                                                              bad.c2f_ptr, idx)] + levels[1:])
+
+
+@pytest.mark.parametrize("case", ["values", "spheres", "advection", "f32", "csr"])
+def test_device_setup_equals_host_setup(case):
+    """The strength criterion and the Galerkin products on the GPU give the
+    host setup's hierarchy bit for bit: the same aggregate maps (the
+    aggregation reads the device's strength flags) and coarse matrices."""
+    dims = (40, 36, 32) if case in ("values", "f32") else (32, 32, 32)
+    if case == "spheres":
+        A, _ = pair(dims, "spheres")
+    elif case == "advection":
+        m = problems.spheres(dims, 3, rmin=2, rmax=4)
+        A = la.StencilMatrix.lookup(la.BoxInfo(dims, velocity_x=10.0 / dims[0]), m, (1.0, 1e-6, 0.0, 0.0))
+    else:
+        A, _ = pair(dims)
+        if case == "csr":
+            A = A.to_csr()
+    prec = np.float32 if case == "f32" else np.float64
+    host = la.MultiGridSolver(la.HierarchyWrightParams(setup_on_device=False), precision=prec).instantiate(A)
+    dev = la.MultiGridSolver(la.HierarchyWrightParams(setup_on_device=True), precision=prec).instantiate(A)
+    assert dev.level_num == host.level_num >= 3
+    for l in range(host.level_num - 1):
+        np.testing.assert_array_equal(dev.aggregates(l), host.aggregates(l))
+        np.testing.assert_array_equal(dev.transfer(l)[1], host.transfer(l)[1])
+    for l in range(1, host.level_num):
+        for a, b in zip(dev.matrix(l), host.matrix(l)):
+            np.testing.assert_array_equal(a, b)
