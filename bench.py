@@ -45,32 +45,76 @@ def kernel_ms(ctx, fn, launches):
     return ctx.timer_stop() / launches
 
 
-def cpu_baseline(n_sample, sigma, n_target, iterations_target):
-    """The oracle (single-threaded C port) on a bounded sample: a complete
-    FP64 PCG-AMG solve at n_sample^3, scaled to the target by cell count and
-    iteration count."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """The host threads this process may use (the box's CPU share)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(n, sigma, runs, x_gpu, x_gpu_fp64):
+    """The oracle's FP64 PCG-AMG at the benchmark's size on the host's
+    threads (oracle/baseline.c: OpenMP, thread-segment SOR with one axis-0
+    slab per thread on the finest level and one row block per thread on the
+    coarse levels, built -O3 -march=native on this host): the median of
+    `runs` complete solves from x = 0 to 1e-8; the hierarchy is built once,
+    untimed.  Also the relative differences of the GPU mixed-precision and
+    FP64 solutions to this FP64 CPU solution (its thread segments take
+    another iteration path, so the difference measures the solve tolerance,
+    not the precision)."""
     import oracle as O
     from lineal_amd import problems
 
-    dims = (n_sample,) * 3
+    native = True
+    try:
+        O.baseline_lib(native=True)
+    except Exception as e:  # no compiler for this host: the portable build
+        log(f"  native baseline build unavailable ({e}); using the x86-64-v3 build")
+        native = False
+    T = cpu_threads()
+    dims = (n,) * 3
     c = problems.lognormal(dims, sigma, 1)
     S = O.Stencil(dims, coeff=c)
     b = S.rhs()
     t0 = time.perf_counter()
-    H = O.Hierarchy(S)
-    t1 = time.perf_counter()
-    _, it, _ = O.pcg(S, b, H, precon=1, rel_tol=1e-8, max_it=500)
-    t2 = time.perf_counter()
-    per_it_cell = (t2 - t1) / it / S.N
-    value = per_it_cell * n_target ** 3 * iterations_target
+    H = O.BaselineHierarchy(S, native)
+    t_setup = time.perf_counter() - t0
+    log(f"  cpu hierarchy {t_setup:.1f} s (one thread, untimed)")
+    times, its, x = [], [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        x, it = O.baseline_pcg(S, b, H, rel_tol=1e-8, max_it=1000, threads=T, native=native)
+        times.append(time.perf_counter() - t0)
+        its.append(it)
+        log(f"  cpu solve {times[-1]:.2f} s, {it} iterations, {T} threads")
+    err = float(np.abs(x_gpu - x).max() / np.abs(x).max())
+    err64 = float(np.abs(x_gpu_fp64 - x).max() / np.abs(x).max())
     return {
-        "value": value,
+        "value": float(np.median(times)),
         "unit": "s",
-        "cores": 1,
+        "cores": T,
         "kind": "port",
-        "sample": (f"oracle FP64 PCG-AMG solve at {n_sample}^3 lognormal sigma={sigma}: {it} iterations "
-                   f"in {t2 - t1:.2f} s (+{t1 - t0:.2f} s setup), scaled to {n_target}^3 x "
-                   f"{iterations_target} iterations by cells and iterations"),
+        "sample": (f"the oracle's FP64 PCG-AMG to rel. residual 1e-8 at {n}^3 lognormal sigma={sigma}, full "
+                   f"solve from x = 0, median of {runs} runs ({', '.join(f'{t:.2f}' for t in times)} s, "
+                   f"{its[-1]} iterations) on {T} threads of {cpu_model()} (OpenMP, thread-segment SOR: "
+                   f"one axis-0 slab per thread on the finest level, one row block per thread on coarse "
+                   f"levels; -O3 -march={'native' if native else 'x86-64-v3'}); hierarchy built once "
+                   f"({t_setup:.1f} s, untimed)"),
+        "mixed_gpu_vs_cpu_fp64_rel_error": err,
+        "fp64_gpu_vs_cpu_fp64_rel_error": err64,
+        "cpu_iterations": its[-1],
     }
 
 
@@ -82,7 +126,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline solves (median)")
     ap.add_argument("--workload", default="lognormal", choices=["lognormal", "two-phase", "mixed512"],
                     help="lognormal: the headline AMG-CG metric (default); two-phase and mixed512: "
                          "the other solve configurations, one JSON line each")
@@ -276,8 +320,12 @@ def main():
         },
     }
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle)")
-        result["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.sigma, n, its64[-1])
+        log("cpu baseline (the oracle on the host's threads)")
+        cb = cpu_baseline(n, args.sigma, args.cpu_runs, x_mixed, x_fp64)
+        result["config"]["mixed_gpu_vs_cpu_fp64_rel_error"] = cb.pop("mixed_gpu_vs_cpu_fp64_rel_error")
+        result["config"]["fp64_gpu_vs_cpu_fp64_rel_error"] = cb.pop("fp64_gpu_vs_cpu_fp64_rel_error")
+        result["config"]["cpu_fp64_iterations"] = cb.pop("cpu_iterations")
+        result["cpu_baseline"] = cb
     print(json.dumps(result), flush=True)
 
 

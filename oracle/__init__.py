@@ -432,6 +432,55 @@ def cholesky_decompose(A: Csr):
     return Csr(rp, cols[:nnz], vals[:nnz], symmetric=False)
 
 
+_baseline_libs = {}
+
+
+def baseline_lib(native=True):
+    """The threaded CPU baseline (oracle/baseline.c): the build for this
+    host's CPU when `native` (made on first use), else the portable one."""
+    name = "liblineal_baseline_native.so" if native else "liblineal_baseline.so"
+    if name not in _baseline_libs:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", _HERE, "baseline-native" if native else "baseline"],
+                           check=True)
+        L = C.CDLL(path)
+        L.bl_pcg.restype = C.c_int
+        L.bl_pcg.argtypes = [C.POINTER(MatStruct), C.c_void_p, f64p, f64p, C.c_double, C.c_int, C.c_int]
+        _baseline_libs[name] = L
+    return _baseline_libs[name]
+
+
+class BaselineHierarchy:
+    """The oracle's hierarchy built by the baseline library (the same code at
+    -O3), for baseline_pcg."""
+
+    def __init__(self, A, native=True):
+        self.A, self.L = A, baseline_lib(native)
+        self.L.or_amg_setup.restype = C.c_void_p
+        self.L.or_amg_setup.argtypes = [C.POINTER(MatStruct), C.POINTER(AmgParams)]
+        self.L.or_amg_free.argtypes = [C.c_void_p]
+        self.params = default_params()
+        self.h = self.L.or_amg_setup(C.byref(A.mat), C.byref(self.params))
+        if not self.h:
+            raise ValueError("the coarsest factorisation failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.or_amg_free(self.h)
+            self.h = None
+
+
+def baseline_pcg(A, b, hierarchy, rel_tol=1e-8, max_it=500, threads=0, native=True, x0=None):
+    """PCG-AMG on `threads` host threads (0: all) with thread-segment SOR
+    smoothing (oracle/baseline.c); `hierarchy` is a Hierarchy or a
+    BaselineHierarchy; returns (x, iterations)."""
+    x = np.zeros(A.N) if x0 is None else np.array(x0, dtype=np.float64)
+    it = baseline_lib(native).bl_pcg(C.byref(A.mat), hierarchy.h, _pf(np.ascontiguousarray(b, np.float64)),
+                                     _pf(x), float(rel_tol), int(max_it), int(threads))
+    return x, it
+
+
 def pcg(A, b, hierarchy=None, precon=0, ultra=False, rel_tol=1e-8, max_it=500, x0=None):
     x = np.zeros_like(b) if x0 is None else x0.astype(np.float64).copy()
     hist = np.zeros(max_it + 1)

@@ -1097,3 +1097,25 @@ def test_cycle_from_an_initial_guess_against_dense_restatement():
     got = H.vcycle(r, x_in=x0)
     np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
     np.testing.assert_array_equal(H.vcycle(r, x_in=np.zeros(S.N)), H.vcycle(r))
+
+
+# ---------------------------------------------------------------------------
+# the threaded CPU baseline (oracle/baseline.c)
+
+
+def test_threaded_baseline_one_thread_is_the_oracle_pcg():
+    """With one thread the baseline runs the oracle's PCG-AMG operation for
+    operation (one segment per level); with more threads its thread-segment
+    sweeps converge to the same solution."""
+    dims = (20, 18, 16)
+    S, _ = lognormal_stencil(dims)
+    b = S.rhs()
+    H = O.Hierarchy(S)
+    x_ref, it_ref, _ = O.pcg(S, b, H, precon=1, rel_tol=1e-8, max_it=200)
+    for native in (False, True):
+        x1, it1 = O.baseline_pcg(S, b, H, threads=1, native=native)
+        assert it1 == it_ref
+        np.testing.assert_array_equal(x1, x_ref)
+    x4, it4 = O.baseline_pcg(S, b, O.BaselineHierarchy(S, native=False), threads=4, native=False)
+    assert abs(it4 - it_ref) <= 4
+    assert np.abs(x4 - x_ref).max() <= 1e-6 * np.abs(x_ref).max()
