@@ -426,6 +426,44 @@ class GpuEffDiffCalc {
   double area_part_ = 0.0;
 };
 
+// CsrMatrix over fix::DenseMatrix<Real, tB, tB> block values
+// (tensor/dynamic/csr/shared.hpp:37-1103, tensor/fixed/matrix.hpp:151):
+// blocks row-major, TReal the stored value type; products accumulate in f64.
+template <typename TReal, int tB>
+class GpuBlockCsrMatrix {
+  static_assert(tB >= 1 && tB <= 4);
+
+ public:
+  GpuBlockCsrMatrix(std::span<const int64_t> rowptr, std::span<const int32_t> cols,
+                    std::span<const double> vals, const GpuContext& ctx)
+      : ctx_(&ctx) {
+    lgpu_bcsr h = nullptr;
+    check(lgpu_bcsr_create(ctx.handle(), int64_t(rowptr.size()) - 1, tB, rowptr.data(), cols.data(),
+                           vals.data(), GpuDenseVector<TReal>::dtype, &h));
+    h_.reset(h);
+  }
+  [[nodiscard]] std::size_t row_num() const {
+    int64_t nb = 0;
+    check(lgpu_bcsr_rows(h_.get(), &nb, nullptr));
+    return std::size_t(nb) * tB;
+  }
+  void multiply(const GpuDenseVector<TReal>& x, GpuDenseVector<TReal>& y) const {
+    check(lgpu_bcsr_spmv(h_.get(), x.handle(), y.handle()));
+  }
+  [[nodiscard]] GpuDenseVector<TReal> operator*(const GpuDenseVector<TReal>& x) const {
+    GpuDenseVector<TReal> y(x.size(), *ctx_);
+    multiply(x, y);
+    return y;
+  }
+
+ private:
+  struct Del {
+    void operator()(lgpu_bcsr m) const { lgpu_bcsr_destroy(m); }
+  };
+  const GpuContext* ctx_;
+  std::unique_ptr<lgpu_bcsr_s, Del> h_;
+};
+
 // ---------------------------------------------------------------------------
 // stationary iterative solvers: instantiate(lhs) -> Instance with ex-situ
 // apply(sol_in, sol_out, rhs) (broom.hpp:188-244); a null sol_in is the

@@ -38,6 +38,7 @@ typedef struct lgpu_ctx_s* lgpu_ctx;
 typedef struct lgpu_vec_s* lgpu_vec;
 typedef struct lgpu_op_s* lgpu_op;
 typedef struct lgpu_amg_s* lgpu_amg;
+typedef struct lgpu_bcsr_s* lgpu_bcsr;
 
 /* ---- errors and context ------------------------------------------------ */
 const char* lgpu_status_string(int status);
@@ -177,6 +178,25 @@ int lgpu_op_ssor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double o
  * only from its own segment. */
 int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
                           int forward, int ultra, int64_t segment_rows);
+
+/* ---- CSR with block values ------------------------------------------------
+ * CsrMatrix over B x B block values, fix::DenseMatrix<Real, B, B>
+ * (tensor/dynamic/csr/shared.hpp:37-1103, tensor/fixed/matrix.hpp:151; the
+ * reference's test/shared-blocked.cpp uses B = 3 and 2).  block_rows block
+ * rows, rowptr (block_rows + 1) and cols (block columns, any order) per block
+ * entry, vals with B * B row-major entries per block given in f64 and stored
+ * in `dtype`; B in 1..4.  Vectors have block_rows * B entries of that dtype. */
+int lgpu_bcsr_create(lgpu_ctx ctx, int64_t block_rows, int block_size, const int64_t* rowptr,
+                     const int32_t* cols, const double* vals, int dtype, lgpu_bcsr* out);
+int lgpu_bcsr_destroy(lgpu_bcsr m);
+int lgpu_bcsr_rows(lgpu_bcsr m, int64_t* block_rows, int* block_size);
+/* y = A x, MatrixVectorProductExpr's row path
+ * (component-wise/matrix-vector-product.hpp:71-82): per block row,
+ * sum = compat::fmadd(A_ij, x_j, sum) = A_ij x_j + sum over the row's blocks
+ * in stored order (tensor/fixed/compat.hpp:24-33), component r of A_ij x_j
+ * being 0 + a_r0 x_0 + a_r1 x_1 + ... (tensor/fixed/expressions.hpp:22-44);
+ * all in f64, the sums stored in the vectors' type. */
+int lgpu_bcsr_spmv(lgpu_bcsr m, lgpu_vec x, lgpu_vec y);
 
 /* ---- aggregation AMG ----------------------------------------------------- */
 /* HierarchyWrightParams + AggregationParams + DuneDependencyCriterion

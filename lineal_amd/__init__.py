@@ -140,6 +140,10 @@ SIGNATURES = {
     "lgpu_eff_diff_area_part": (C.c_int, [_vp, _u8p, _f64p]),
     "lgpu_eff_diff": (C.c_int, [_vp, _vp, _vp, C.c_double, _f64p]),
     "lgpu_op_destroy": (C.c_int, [_vp]),
+    "lgpu_bcsr_create": (C.c_int, [_vp, _i64, C.c_int, _i64p, _i32p, _f64p, C.c_int, C.POINTER(_vp)]),
+    "lgpu_bcsr_destroy": (C.c_int, [_vp]),
+    "lgpu_bcsr_rows": (C.c_int, [_vp, _i64p, C.POINTER(C.c_int)]),
+    "lgpu_bcsr_spmv": (C.c_int, [_vp, _vp, _vp]),
     "lgpu_op_rows": (C.c_int, [_vp, _i64p]),
     "lgpu_op_rhs": (C.c_int, [_vp, _vp]),
     "lgpu_op_spmv": (C.c_int, [_vp, _vp, _vp]),
@@ -500,6 +504,41 @@ class CsrMatrix(_Matrix):
         _check(lib().lgpu_csr_create(ctx.h, rp.size - 1, rp.ctypes.data_as(_i64p), ci.ctypes.data_as(_i32p),
                                      va.ctypes.data_as(_f64p), _dtype_code(dtype), int(symmetric), C.byref(h)))
         return cls(h, ctx)
+
+
+class BlockCsrMatrix:
+    """CsrMatrix over B x B block values (tensor/dynamic/csr/shared.hpp:37-1103
+    with fix::DenseMatrix<Real, B, B>, tensor/fixed/matrix.hpp:151); vectors
+    hold block_rows * B entries."""
+
+    def __init__(self, rowptr, cols, vals, dtype=np.float64, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        rp = np.ascontiguousarray(rowptr, np.int64)
+        ci = np.ascontiguousarray(cols, np.int32)
+        va = np.ascontiguousarray(vals, np.float64)
+        self.block = 1 if va.ndim == 1 else va.shape[1]
+        self.dtype = np.dtype(dtype)
+        h = _vp()
+        _check(lib().lgpu_bcsr_create(self.ctx.h, rp.size - 1, self.block, rp.ctypes.data_as(_i64p),
+                                      ci.ctypes.data_as(_i32p), va.ctypes.data_as(_f64p),
+                                      _dtype_code(dtype), C.byref(h)))
+        self.h = h
+        self.row_num = (rp.size - 1) * self.block
+
+    def spmv(self, x: DenseVector, y: DenseVector):
+        _check(lib().lgpu_bcsr_spmv(self.h, x.h, y.h))
+        return y
+
+    def __mul__(self, x: DenseVector) -> DenseVector:
+        return self.spmv(x, DenseVector(x.n, x.dtype, self.ctx))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and not _finalizing:
+                lib().lgpu_bcsr_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "liblineal_amd.so")
 ARCH = os.environ.get("LINEAL_AMD_ARCH", "gfx950")
 
 SOURCES = ["abi.cpp", "setup.cpp", "k_stencil.hip", "k_sor.hip", "k_csr.hip", "k_blas.hip",
-           "k_adjoint.hip", "k_effdiff.hip", "io.cpp", "scratch.hip", "k_setup.hip"]
+           "k_adjoint.hip", "k_effdiff.hip", "io.cpp", "scratch.hip", "k_setup.hip", "k_bcsr.hip"]
 
 # -ffp-contract=off: only the fma() calls written in the sources are fused, so
 # every kernel performs exactly the oracle's operations.  Packed FP32 ops

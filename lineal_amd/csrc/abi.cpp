@@ -1184,6 +1184,84 @@ extern "C" int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, 
 }
 
 // ---------------------------------------------------------------------------
+// CSR with block values
+
+struct lgpu_bcsr_s {
+  lgpu_ctx ctx = nullptr;
+  int64_t nb = 0;
+  int block = 1;
+  bool f32 = false;
+  DevBuf rowptr, cols, vals;
+  BcsrDesc desc() const {
+    return BcsrDesc{nb, block, rowptr.as<int64_t>(), cols.as<int32_t>(), vals.ptr, f32 ? 1 : 0};
+  }
+};
+
+extern "C" int lgpu_bcsr_create(lgpu_ctx ctx, int64_t block_rows, int block_size,
+                                const int64_t* rowptr, const int32_t* cols, const double* vals,
+                                int dtype, lgpu_bcsr* out) {
+  API_BEGIN
+  LGPU_REQUIRE(ctx && out && rowptr && block_rows >= 0, "invalid block CSR arguments");
+  LGPU_REQUIRE(block_size >= 1 && block_size <= 4, "block_size must lie in 1..4");
+  LGPU_REQUIRE(dtype == LGPU_F64 || dtype == LGPU_F32, "unknown dtype");
+  LGPU_REQUIRE(rowptr[0] == 0, "malformed row pointer");
+  const int64_t nnz = rowptr[block_rows];
+  LGPU_REQUIRE(nnz == 0 || (cols && vals), "NULL column or value array");
+  for (int64_t i = 0; i < block_rows; ++i) {
+    LGPU_REQUIRE(rowptr[i + 1] >= rowptr[i], "malformed row pointer");
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      LGPU_REQUIRE(cols[k] >= 0 && cols[k] < block_rows, "column index out of range");
+  }
+  auto m = std::make_unique<lgpu_bcsr_s>();
+  m->ctx = ctx;
+  m->nb = block_rows;
+  m->block = block_size;
+  m->f32 = dtype == LGPU_F32;
+  const size_t ne = size_t(nnz) * size_t(block_size * block_size);
+  m->rowptr.alloc(size_t(block_rows + 1) * sizeof(int64_t));
+  LGPU_HIP_CHECK(hipMemcpy(m->rowptr.ptr, rowptr, size_t(block_rows + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  m->cols.alloc(size_t(nnz) * sizeof(int32_t));
+  if (nnz) LGPU_HIP_CHECK(hipMemcpy(m->cols.ptr, cols, size_t(nnz) * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (m->f32) {
+    std::vector<float> v(vals, vals + ne);
+    m->vals.alloc(ne * sizeof(float));
+    if (ne) LGPU_HIP_CHECK(hipMemcpy(m->vals.ptr, v.data(), ne * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    m->vals.alloc(ne * sizeof(double));
+    if (ne) LGPU_HIP_CHECK(hipMemcpy(m->vals.ptr, vals, ne * sizeof(double), hipMemcpyHostToDevice));
+  }
+  *out = m.release();
+  API_END
+}
+
+extern "C" int lgpu_bcsr_destroy(lgpu_bcsr m) {
+  API_BEGIN
+  delete m;
+  API_END
+}
+
+extern "C" int lgpu_bcsr_rows(lgpu_bcsr m, int64_t* block_rows, int* block_size) {
+  API_BEGIN
+  LGPU_REQUIRE(m, "NULL matrix");
+  if (block_rows) *block_rows = m->nb;
+  if (block_size) *block_size = m->block;
+  API_END
+}
+
+extern "C" int lgpu_bcsr_spmv(lgpu_bcsr m, lgpu_vec x, lgpu_vec y) {
+  API_BEGIN
+  LGPU_REQUIRE(m, "NULL matrix");
+  const int64_t n = m->nb * m->block;
+  check_vec(x, n, "x");
+  check_vec(y, n, "y");
+  LGPU_REQUIRE(x->f32 == m->f32 && y->f32 == m->f32, "the vectors need the matrix's value type");
+  LGPU_REQUIRE(x->ptr != y->ptr, "x and y must not alias");
+  KTimed kt(m->ctx, LGPU_KT_CSR_PASS);
+  bcsr_spmv(m->desc(), x->ptr, y->ptr, m->ctx->stream);
+  API_END
+}
+
+// ---------------------------------------------------------------------------
 // AMG
 
 extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {

@@ -126,6 +126,17 @@ struct CsrDesc {
 // rows per block of the CSR passes
 constexpr int kCsrBlockRows = 256;
 
+// CSR with B x B block values (B = 1..4), row-major blocks
+struct BcsrDesc {
+  int64_t nb;            // block rows
+  int block;             // B
+  const int64_t* rowptr;
+  const int32_t* cols;
+  const void* vals;      // f64 or f32, nnz_blocks * B * B
+  int f32;
+};
+void bcsr_spmv(const BcsrDesc& a, const void* x, void* y, hipStream_t st);
+
 // ---------------------------------------------------------------------------
 // kernel launchers (stream ordered)
 

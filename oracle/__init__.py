@@ -140,6 +140,8 @@ def lib():
             "or_ssor_f64": (None, [mp, f64p, f64p, f64p, C.c_double, C.c_int, C.c_int]),
             "or_eff_diff_area_part": (C.c_double, [sp, C.POINTER(C.c_uint8)]),
             "or_eff_diff": (C.c_double, [sp, i64p, f64p, C.c_double]),
+            "or_bcsr_spmv_f64": (None, [C.c_int64, C.c_int, i64p, i64p, f64p, f64p, f64p]),
+            "or_bcsr_spmv_f32": (None, [C.c_int64, C.c_int, i64p, i64p, f64p, f32p, f32p]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -624,3 +626,16 @@ def dvf_write(path, v):
     v = np.asarray(v, np.float64)
     with open(path, "wb") as f:
         f.write(np.array([v.size], "<u8").tobytes() + v.astype("<f8").tobytes())
+
+
+def bcsr_spmv(rowptr, cols, vals, x):
+    """y = A x for a CSR matrix with B x B block values (vals: blocks x B x B);
+    x f64 or f32 selects the variant."""
+    rp = np.ascontiguousarray(rowptr, np.int64)
+    ci = np.ascontiguousarray(cols, np.int64)
+    va = np.ascontiguousarray(vals, np.float64)
+    bsz = va.shape[1] if va.ndim == 3 else 1
+    y = np.zeros_like(x)
+    _fn("or_bcsr_spmv", x.dtype)(rp.size - 1, bsz, _p(rp, C.c_int64), _p(ci, C.c_int64),
+                                 _p(va.reshape(-1), C.c_double), _pf(x), _pf(y))
+    return y

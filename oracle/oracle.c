@@ -1301,3 +1301,44 @@ double or_eff_diff(const or_stencil* s, const int64_t* cmap, const double* sol,
   out *= s->len[0];
   return out / (s->p_end - s->p_start);
 }
+
+/* ---------------------------------------------------------------------------
+ * CSR with B x B block values: y = A x with MatrixVectorProductExpr's row path
+ * (component-wise/matrix-vector-product.hpp:71-82): sum = A_ij x_j + sum over
+ * the row's blocks in stored order (compat::fmadd, tensor/fixed/compat.hpp:24-33),
+ * component r of A_ij x_j the left fold 0 + a_r0 x_0 + a_r1 x_1 + ...
+ * (tensor/fixed/expressions.hpp:22-44), in f64; y rounded to the vectors'
+ * type.  vals: B*B row-major entries per block. */
+void or_bcsr_spmv_f64(int64_t nb, int bsz, const int64_t* rowptr, const int64_t* cols,
+                      const double* vals, const double* x, double* y) {
+  for (int64_t i = 0; i < nb; ++i) {
+    double sum[4] = {0, 0, 0, 0};
+    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const double* a = vals + p * bsz * bsz;
+      const double* xj = x + cols[p] * bsz;
+      for (int r = 0; r < bsz; ++r) {
+        double t = 0.0;
+        for (int c = 0; c < bsz; ++c) t = t + a[r * bsz + c] * xj[c];
+        sum[r] = t + sum[r];
+      }
+    }
+    for (int r = 0; r < bsz; ++r) y[i * bsz + r] = sum[r];
+  }
+}
+
+void or_bcsr_spmv_f32(int64_t nb, int bsz, const int64_t* rowptr, const int64_t* cols,
+                      const double* vals, const float* x, float* y) {
+  for (int64_t i = 0; i < nb; ++i) {
+    double sum[4] = {0, 0, 0, 0};
+    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const double* a = vals + p * bsz * bsz;
+      const float* xj = x + cols[p] * bsz;
+      for (int r = 0; r < bsz; ++r) {
+        double t = 0.0;
+        for (int c = 0; c < bsz; ++c) t = t + (double)(float)a[r * bsz + c] * (double)xj[c];
+        sum[r] = t + sum[r];
+      }
+    }
+    for (int r = 0; r < bsz; ++r) y[i * bsz + r] = (float)sum[r];
+  }
+}

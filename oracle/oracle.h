@@ -207,6 +207,15 @@ double or_eff_diff_area_part(const or_stencil* s, const uint8_t* none_mats);
 double or_eff_diff(const or_stencil* s, const int64_t* cmap, const double* sol,
                    double area_part);
 
+/* CSR with B x B block values (B <= 4, B*B row-major entries per block):
+ * y = A x accumulated in f64 (tensor/fixed/compat.hpp:24-33,
+ * component-wise/matrix-vector-product.hpp:71-82); the f32 variant rounds the
+ * block values to f32 and stores y in f32 */
+void or_bcsr_spmv_f64(int64_t nb, int bsz, const int64_t* rowptr, const int64_t* cols,
+                      const double* vals, const double* x, double* y);
+void or_bcsr_spmv_f32(int64_t nb, int bsz, const int64_t* rowptr, const int64_t* cols,
+                      const double* vals, const float* x, float* y);
+
 #ifdef __cplusplus
 }
 #endif

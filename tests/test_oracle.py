@@ -1119,3 +1119,76 @@ def test_threaded_baseline_one_thread_is_the_oracle_pcg():
     x4, it4 = O.baseline_pcg(S, b, O.BaselineHierarchy(S, native=False), threads=4, native=False)
     assert abs(it4 - it_ref) <= 4
     assert np.abs(x4 - x_ref).max() <= 1e-6 * np.abs(x_ref).max()
+
+
+# ---------------------------------------------------------------------------
+# CSR with block values (test/shared-blocked.cpp)
+
+
+def blocked_reference_system(dim=73, bsz=3):
+    """test/shared-blocked.cpp:55-67: block row i holds -(i)·I at column i-1,
+    (2(i+1) + 0.1)·I at i and -(i+2)·I at i+1, in f32 (LoReal)."""
+    rp, cols, vals = [0], [], []
+    eye = np.eye(bsz)
+    for i in range(dim):
+        if i > 0:
+            cols.append(i - 1)
+            vals.append(float(np.float32(-i)) * eye)
+        cols.append(i)
+        vals.append(float(np.float32(2 * (i + 1)) + np.float32(0.1)) * eye)
+        if i + 1 < dim:
+            cols.append(i + 1)
+            vals.append(float(np.float32(-(i + 2))) * eye)
+        rp.append(len(cols))
+    return np.array(rp, np.int64), np.array(cols, np.int64), np.array(vals)
+
+
+def blocked_reference_product(dim=73, bsz=3):
+    """The reference's own expected product (test/shared-blocked.cpp:149-158):
+    per block row, sum += cast<Real>(A_ij) * cast<Real>(x_j) with x = 1 in
+    f64, then cast to f32."""
+    out = np.zeros(dim * bsz, np.float32)
+    for i in range(dim):
+        s = 0.0
+        if i > 0:
+            s = float(np.float32(-i)) + s
+        s = float(np.float32(2 * (i + 1)) + np.float32(0.1)) + s
+        if i + 1 < dim:
+            s = float(np.float32(-(i + 2))) + s
+        out[i * bsz:(i + 1) * bsz] = np.float32(s)
+    return out
+
+
+def test_bcsr_spmv_reference_known_answer():
+    rp, cols, vals = blocked_reference_system()
+    y = O.bcsr_spmv(rp, cols, vals, np.ones(73 * 3, np.float32))
+    np.testing.assert_array_equal(y, blocked_reference_product())
+
+
+@pytest.mark.parametrize("bsz", [1, 2, 3, 4])
+def test_bcsr_spmv_against_dense(bsz):
+    """Random blocks (row-major) with unsorted columns and an empty row
+    against the dense product: a transposed block or a wrong offset fails."""
+    rng = np.random.default_rng(bsz)
+    nb = 23
+    rp, cols = [0], []
+    for i in range(nb):
+        k = 0 if i == 5 else rng.integers(1, 5)
+        cols += list(rng.choice(nb, k, replace=False))
+        rp.append(len(cols))
+    rp, cols = np.array(rp, np.int64), np.array(cols, np.int64)
+    vals = rng.standard_normal((cols.size, bsz, bsz))
+    D = np.zeros((nb * bsz, nb * bsz))
+    for i in range(nb):
+        for p in range(rp[i], rp[i + 1]):
+            D[i * bsz:(i + 1) * bsz, cols[p] * bsz:(cols[p] + 1) * bsz] += vals[p]
+    x = rng.standard_normal(nb * bsz)
+    np.testing.assert_allclose(O.bcsr_spmv(rp, cols, vals, x), D @ x, rtol=1e-12, atol=1e-12)
+    v32 = vals.astype(np.float32).astype(np.float64)
+    x32 = x.astype(np.float32)
+    D32 = np.zeros_like(D)
+    for i in range(nb):
+        for p in range(rp[i], rp[i + 1]):
+            D32[i * bsz:(i + 1) * bsz, cols[p] * bsz:(cols[p] + 1) * bsz] += v32[p]
+    np.testing.assert_allclose(O.bcsr_spmv(rp, cols, vals, x32), (D32 @ x32.astype(np.float64)).astype(np.float32),
+                               rtol=1e-6, atol=1e-6)
