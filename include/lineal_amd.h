@@ -166,6 +166,16 @@ int lgpu_op_jacobi(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double
  * lgpu_op_ssor and lgpu_amg_vcycle. */
 int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
                 int forward, int ultra);
+/* Coefficient cache of the stencil SOR sweeps.  By default a stencil
+ * operator's sweeps read its face weights and diagonal from a per-cell cache
+ * (four values per cell in the sweep's precision, built by the first sweep of
+ * that precision, kept with the operator; an AMG hierarchy builds its finest
+ * level's at creation).  enable = 0 frees the caches and makes the sweeps
+ * evaluate the weights on the fly; both ways give the same bits.  No
+ * counterpart in the reference, whose sweeps evaluate the stencil on the fly
+ * (tensor/dynamic/stencil/matrix/algorithm.hpp:91-227); for callers that
+ * trade the memory against sweep speed.  No effect on CSR operators. */
+int lgpu_op_set_sor_cache(lgpu_op op, int enable);
 /* One SSOR step, SsorBroom::sweep
  * (linear-solver/iterative-stationary/gauss-seidel-variant/solver.hpp:45-88):
  * a forward sweep x_in -> aux, then a backward sweep aux -> x_out, both
@@ -249,7 +259,9 @@ int lgpu_amg_create(lgpu_op finest, const lgpu_amg_params* params, lgpu_amg* out
  * library allocates on the operator's first SSOR call, lives in a static
  * per-GPU device scratch of 2^22 f64 (32 MiB), so operators with more than
  * 4194304 rows are rejected (LGPU_INVALID_ARG).  Nothing is allocated and no
- * kernel-class timing is recorded.  f32 sweeps on a per-cell coefficient
+ * kernel-class timing is recorded.  The sweeps read the operator's SOR
+ * coefficient cache when it exists and evaluate the stencil otherwise (both
+ * give the same bits; building the cache would allocate).  f32 sweeps on a per-cell coefficient
  * stencil need the operator's f32 coefficient copy, which the first f32 pass
  * of another entry point makes (otherwise LGPU_INVALID_ARG).  Every call on
  * one GPU shares that GPU's scratch: calls that overlap on one GPU (other

@@ -150,6 +150,7 @@ SIGNATURES = {
     "lgpu_op_residual": (C.c_int, [_vp, _vp, _vp, _vp]),
     "lgpu_op_jacobi": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
     "lgpu_op_sor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int]),
+    "lgpu_op_set_sor_cache": (C.c_int, [_vp, C.c_int]),
     "lgpu_op_sor_segmented": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int, C.c_int, _i64]),
     "lgpu_op_ssor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int]),
     "lgpu_op_ssor_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_int]),
@@ -460,6 +461,13 @@ class StencilMatrix(_Matrix):
         b = DenseVector(self.row_num, dtype, self.ctx)
         _check(lib().lgpu_op_rhs(self.h, b.h))
         return b
+
+    def set_sor_cache(self, enable: bool):
+        """Cached per-cell face weights and diagonal for the SOR sweeps (the
+        default), or evaluation on the fly (enable False frees the caches);
+        the sweeps give the same bits either way."""
+        _check(lib().lgpu_op_set_sor_cache(self.h, int(bool(enable))))
+        return self
 
     def to_csr(self, dtype=np.float64) -> "CsrMatrix":
         h = _vp()

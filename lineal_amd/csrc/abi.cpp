@@ -210,6 +210,11 @@ struct lgpu_op_s {
   int64_t cmap_cells = 0;
   // intermediate vector of SSOR (the forward sweep's output), f64-sized
   DevBuf ssor_aux;
+  // per-cell face weights and diagonal read by the stencil SOR sweeps, per
+  // precision, built on first use (sor_cache = false: the sweeps evaluate
+  // them on the fly and no cache is kept)
+  DevBuf sor_w64, sor_w32;
+  bool sor_cache = true;
 
   int64_t rows() const { return is_csr ? csr.n : desc.nx * desc.ny * desc.nz; }
   void ensure_f32() {
@@ -218,6 +223,20 @@ struct lgpu_op_s {
     c32.alloc(size_t(n) * sizeof(float));
     vec_convert(c64.ptr, false, c32.ptr, true, n, ctx->stream);
     desc.c32 = c32.as<float>();
+  }
+  // the SOR weights of a stencil operator in that precision: built here when
+  // `build` (and the cache is on), else only when already there
+  const void* sor_weights(bool f32, bool build) {
+    if (is_csr || !sor_cache) return nullptr;
+    DevBuf& w = f32 ? sor_w32 : sor_w64;
+    if (!w.ptr && build) {
+      const size_t bytes = sor_weights_bytes(desc, f32);
+      if (bytes == 0) return nullptr;
+      if (f32) ensure_f32();
+      w.alloc(bytes);
+      stencil_sor_weights(desc, f32, w.ptr, ctx->stream);
+    }
+    return w.ptr;
   }
 };
 
@@ -412,8 +431,9 @@ void op_sor(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, 
     s.order = op->sor_order.as<uint32_t>();
     s.ncols = op->sor_cols;
     s.mailbox = op->sor_mailbox.ptr;
+    const void* w = op->sor_weights(f32, true);
     KTimed kt(op->ctx, fwd ? LGPU_KT_STENCIL_SOR_FWD : LGPU_KT_STENCIL_SOR_BWD);
-    stencil_sor(op->desc, f32, x_in, x_out, b, omega, fwd, ultra, s, st);
+    stencil_sor(op->desc, f32, x_in, x_out, b, omega, fwd, ultra, s, st, w);
   }
 }
 
@@ -583,7 +603,9 @@ void ssor_sweep_into(lgpu_op op, bool f32, const void* x_in, void* x_out, const 
       s.order = op->sor_order.as<uint32_t>();
       s.ncols = op->sor_cols;
       s.mailbox = op->sor_mailbox.ptr;
-      stencil_sor(op->desc, f32, in, out, b, omega, fwd != 0, ultra, s, st);
+      // the cached weights when they exist (building them would allocate)
+      stencil_sor(op->desc, f32, in, out, b, omega, fwd != 0, ultra, s, st,
+                  op->sor_weights(f32, false));
     }
   }
 }
@@ -1101,6 +1123,18 @@ extern "C" int lgpu_op_sor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b
   API_END
 }
 
+extern "C" int lgpu_op_set_sor_cache(lgpu_op op, int enable) {
+  API_BEGIN
+  LGPU_REQUIRE(op, "NULL operator");
+  op->sor_cache = enable != 0;
+  if (!op->sor_cache) {
+    LGPU_HIP_CHECK(hipStreamSynchronize(op->ctx->stream));
+    op->sor_w64.release();
+    op->sor_w32.release();
+  }
+  API_END
+}
+
 extern "C" int lgpu_op_ssor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
                             int ultra) {
   API_BEGIN
@@ -1349,6 +1383,8 @@ void upload_hierarchy(lgpu_amg amg) {
     amg->r32.alloc(size_t(h.sizes[0]) * sizeof(float));
     amg->z32.alloc(size_t(h.sizes[0]) * sizeof(float));
   }
+  // the finest smoother's cached weights, built with the hierarchy
+  amg->finest->sor_weights(amg->f32, true);
   LGPU_HIP_CHECK(hipDeviceSynchronize());
 }
 

@@ -157,8 +157,16 @@ struct SorSched {
 size_t sor_sched_ints(const StencilDesc& s);
 size_t sor_mailbox_bytes(const StencilDesc& s);
 void sor_sched_order(const StencilDesc& s, std::vector<uint32_t>& order);
+// weights: the per-cell face weights and diagonal of stencil_sor_weights (the
+// sweep reads them), or nullptr (the sweep evaluates them)
 void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, const void* b,
-                 double omega, bool forward, bool ultra, const SorSched& sched, hipStream_t st);
+                 double omega, bool forward, bool ultra, const SorSched& sched, hipStream_t st,
+                 const void* weights = nullptr);
+// face weights and diagonal of every cell in the cached-coefficient sweep's
+// order (eight values per cell slot, in the sweep's precision); 0 bytes when
+// the box is too deep in z for them
+size_t sor_weights_bytes(const StencilDesc& s, bool f32);
+void stencil_sor_weights(const StencilDesc& s, bool f32, void* w, hipStream_t st);
 
 // CSR kernels
 int64_t csr_partials(const CsrDesc& a);

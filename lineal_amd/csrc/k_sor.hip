@@ -118,12 +118,15 @@ __device__ __forceinline__ E rload(Rsrc r, uint32_t off) {
   else if constexpr (sizeof(E) == 4) return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0u, 0));
   else return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0u, 0));
 }
-template <typename T>
+// POL: cache policy bits (16: sc1, the write-through that relaxed agent-scope
+// atomic stores use; a naturally aligned dword or dwordx2 store is
+// single-copy atomic)
+template <typename T, int POL = 0>
 __device__ __forceinline__ void rstore(T v, Rsrc r, uint32_t off) {
-  if constexpr (sizeof(T) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0u, 0);
+  if constexpr (sizeof(T) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0u, POL);
   else {
     using V2 = unsigned int __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0u, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0u, POL);
   }
 }
 
@@ -281,96 +284,113 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
   // the b = 7 lines
   const int64_t ncols = ((nx + CB - 1) / CB) * cy;
   T* const ybox = mbox + ncols * CB * nz;
-  T* const out_x = mbox + (col * CB + b) * nz;
-  T* const out_y = ybox + (col * CB + a) * nz;
-  const T* const in_x = mbox + (((I > 0 ? I - 1 : 0) * cy + J) * CB + b) * nz;
-  const T* const in_y = ybox + ((I * cy + (J > 0 ? J - 1 : 0)) * CB + a) * nz;
+  const int64_t col_x = (I > 0 ? I - 1 : 0) * cy + J, col_y = I * cy + (J > 0 ? J - 1 : 0);
+  const T* const in_x = mbox + (col_x * CB + b) * nz;
+  const T* const in_y = ybox + (col_y * CB + a) * nz;
   const bool take_x = line && a == 0 && has_px;
   const bool take_y = line && b == 0 && has_py;
   const bool give_x = a == CB - 1, give_y = b == CB - 1;
-  // the whole mailbox (both parts) as a resource, for the prefetch runs
-  const Rsrc rmb = col_rsrc(static_cast<const T*>(mbox), 0, 2 * ncols * CB * nz);
+  // the four mailbox lines a column reads or writes, as resources: its own
+  // X and Y parts and those of its upstream columns (the lines of lane b / a
+  // start at b * nz / a * nz)
+  const Rsrc rox = col_rsrc(static_cast<const T*>(mbox), col * CB * nz, CB * nz);
+  const Rsrc roy = col_rsrc(static_cast<const T*>(ybox), col * CB * nz, CB * nz);
+  const Rsrc rix = col_rsrc(static_cast<const T*>(mbox), col_x * CB * nz, CB * nz);
+  const Rsrc riy = col_rsrc(static_cast<const T*>(ybox), col_y * CB * nz, CB * nz);
+  const int32_t lx_off = int32_t(b * nz), ly_off = int32_t(a * nz);
   T prev = T(0), wpx_prev = T(0), wpy_prev = T(0), wpz_prev = T(0);
   // a chunk's windows; the next chunk's are issued while the current one's
   // steps run (after its mailbox prefetch, so waiting for those words does
   // not wait for the windows).  PF = false loads them at the chunk start.
   constexpr bool PF = true;
-  auto load_windows = [&](Raw(&w_cw)[K + 2], Raw(&w_hx)[K + 2], Raw(&w_hy)[K + 2],
-                          T(&w_xw)[K + 1], T(&w_exx)[K + 1], T(&w_exy)[K + 1], T(&w_bw)[K],
-                          int64_t tc) {
-    const int32_t z0 = int32_t(tc) - a - b;
-    // runs: the lowest element of each window (forward: the first step's;
-    // backward: the last step's), for 16-byte loads when every window a lane
-    // uses starts inside its resource
-    const int32_t zc = FWD ? zorg + z0 - 1 : zorg - z0 - K;      // cw: K + 2 elements
-    const int32_t zx = FWD ? zorg + z0 : zorg - z0 - K;          // xw: K + 1
-    const int32_t zb = FWD ? zorg + z0 : zorg - z0 - (K - 1);    // bw: K
-    bool runs = false;
-    if constexpr (RUNS) {
-      auto out = [emax](int32_t lo, int n) { return lo < 0 || lo + n - 1 > emax; };
-      const bool bad = line && (out(zc, K + 2) || out(zb, K) || (HAS_X && out(zx, K + 1)) ||
-                                (hx_need && out(zc + hxo, K + 2)) ||
-                                (hy_need && out(zc + hyo, K + 2)) ||
-                                (HAS_X && give_x && has_sx && out(zx + hsx, K + 1)) ||
-                                (HAS_X && give_y && has_sy && out(zx + hsy, K + 1)));
-      runs = __builtin_amdgcn_ballot_w64(bad) == 0;
+  // A chunk's windows hold exactly the elements its K steps read (a window
+  // element loaded but never read would let the compiler reuse its register
+  // while the load is still in flight, and wait for it there):
+  //   cw  [K + 1]  coefficients of the own line, logical z0 .. z0 + K
+  //   hx/hy [K]    coefficients of the x / y halo line, z0 .. z0 + K - 1
+  //   xw  [K + 1]  old values of the own line, z0 .. z0 + K
+  //   exx/exy [K]  old values of the +x / +y line beyond the column, z0 ..
+  //   bw  [K]      right-hand side, z0 .. z0 + K - 1
+  // element loads of a lane's windows (clamped into the resource)
+  auto load_elements = [&](Raw(&w_cw)[K + 1], Raw(&w_hx)[K], Raw(&w_hy)[K], T(&w_xw)[K + 1],
+                           T(&w_exx)[K], T(&w_exy)[K], T(&w_bw)[K], int32_t z0) {
+#pragma unroll
+    for (int j = 0; j < K + 1; ++j) {
+      const int32_t e = zorg + dz * (z0 + j);
+      w_cw[j] = rload<Raw>(rc, boff(e, SR));
+      w_xw[j] = HAS_X ? rload<T>(rxi, boff(e, ST)) : T(0);
     }
-    if (runs) {
-      if constexpr (RUNS) {
-        load_run<Raw, K + 2, !FWD>(rc, line ? uint32_t(zc) * SR : kRunOff, w_cw);
-        load_run<Raw, K + 2, !FWD>(rc, line && hx_need ? uint32_t(zc + hxo) * SR : kRunOff, w_hx);
-        load_run<Raw, K + 2, !FWD>(rc, line && hy_need ? uint32_t(zc + hyo) * SR : kRunOff, w_hy);
-        load_run<T, K, !FWD>(rbv, line ? uint32_t(zb) * ST : kRunOff, w_bw);
-        if constexpr (HAS_X) {
-          load_run<T, K + 1, !FWD>(rxi, line ? uint32_t(zx) * ST : kRunOff, w_xw);
-          load_run<T, K + 1, !FWD>(rxi, line && give_x && has_sx ? uint32_t(zx + hsx) * ST : kRunOff, w_exx);
-          load_run<T, K + 1, !FWD>(rxi, line && give_y && has_sy ? uint32_t(zx + hsy) * ST : kRunOff, w_exy);
-        } else {
 #pragma unroll
-          for (int j = 0; j < K + 1; ++j) w_xw[j] = w_exx[j] = w_exy[j] = T(0);
-        }
+    for (int j = 0; j < K; ++j) {
+      const int32_t e = zorg + dz * (z0 + j);
+      w_hx[j] = rload<Raw>(rc, hx_on ? boff(e + hxo, SR) : kOff);
+      w_hy[j] = rload<Raw>(rc, hy_on ? boff(e + hyo, SR) : kOff);
+      w_exx[j] = HAS_X ? rload<T>(rxi, give_x ? boff(e + hsx, ST) : kOff) : T(0);
+      w_exy[j] = HAS_X ? rload<T>(rxi, give_y ? boff(e + hsy, ST) : kOff) : T(0);
+      w_bw[j] = rload<T>(rbv, boff(e, ST));
+    }
+  };
+  // a chunk's windows; the next chunk's are issued while the current one's
+  // steps run (after its mailbox prefetch, so waiting for those words does
+  // not wait for the windows).  Every lane issues the same 16-byte runs; the
+  // few lanes whose windows reach past their column's resource (the first
+  // and last chunks of a line) take zeros from those and reload their
+  // windows element by element in a masked block.  No path skips the runs,
+  // so the waits for data loaded before them can count them.
+  auto load_windows = [&](Raw(&w_cw)[K + 1], Raw(&w_hx)[K], Raw(&w_hy)[K], T(&w_xw)[K + 1],
+                          T(&w_exx)[K], T(&w_exy)[K], T(&w_bw)[K], int64_t tc) {
+    const int32_t z0 = int32_t(tc) - a - b;
+    if constexpr (RUNS) {
+      // the lowest element of each window (forward: the first step's;
+      // backward: the last step's)
+      const int32_t z1 = FWD ? zorg + z0 : zorg - z0 - K;        // K + 1 elements
+      const int32_t zk = FWD ? zorg + z0 : zorg - z0 - (K - 1);  // K elements
+      auto out = [emax](int32_t lo, int n) { return lo < 0 || lo + n - 1 > emax; };
+      const bool bad = line && (out(z1, K + 1) || out(zk, K) ||
+                                (hx_need && out(zk + hxo, K)) || (hy_need && out(zk + hyo, K)) ||
+                                (HAS_X && give_x && has_sx && out(zk + hsx, K)) ||
+                                (HAS_X && give_y && has_sy && out(zk + hsy, K)));
+      const bool ok = line && !bad;
+      load_run<Raw, K + 1, !FWD>(rc, ok ? uint32_t(z1) * SR : kRunOff, w_cw);
+      load_run<Raw, K, !FWD>(rc, ok && hx_need ? uint32_t(zk + hxo) * SR : kRunOff, w_hx);
+      load_run<Raw, K, !FWD>(rc, ok && hy_need ? uint32_t(zk + hyo) * SR : kRunOff, w_hy);
+      load_run<T, K, !FWD>(rbv, ok ? uint32_t(zk) * ST : kRunOff, w_bw);
+      if constexpr (HAS_X) {
+        load_run<T, K + 1, !FWD>(rxi, ok ? uint32_t(z1) * ST : kRunOff, w_xw);
+        load_run<T, K, !FWD>(rxi, ok && give_x && has_sx ? uint32_t(zk + hsx) * ST : kRunOff, w_exx);
+        load_run<T, K, !FWD>(rxi, ok && give_y && has_sy ? uint32_t(zk + hsy) * ST : kRunOff, w_exy);
+      } else {
+#pragma unroll
+        for (int j = 0; j < K + 1; ++j) w_xw[j] = T(0);
+#pragma unroll
+        for (int j = 0; j < K; ++j) w_exx[j] = w_exy[j] = T(0);
       }
+      if (bad) load_elements(w_cw, w_hx, w_hy, w_xw, w_exx, w_exy, w_bw, z0);
     } else {
-#pragma unroll
-      for (int j = 0; j < K + 2; ++j) {
-        const int32_t e = zorg + dz * (z0 - 1 + j);
-        w_cw[j] = rload<Raw>(rc, boff(e, SR));
-        w_hx[j] = rload<Raw>(rc, hx_on ? boff(e + hxo, SR) : kOff);
-        w_hy[j] = rload<Raw>(rc, hy_on ? boff(e + hyo, SR) : kOff);
-      }
-#pragma unroll
-      for (int j = 0; j < K + 1; ++j) {
-        const int32_t e = zorg + dz * (z0 + j);
-        w_xw[j] = HAS_X ? rload<T>(rxi, boff(e, ST)) : T(0);
-        w_exx[j] = HAS_X ? rload<T>(rxi, give_x ? boff(e + hsx, ST) : kOff) : T(0);
-        w_exy[j] = HAS_X ? rload<T>(rxi, give_y ? boff(e + hsy, ST) : kOff) : T(0);
-      }
-#pragma unroll
-      for (int j = 0; j < K; ++j) w_bw[j] = rload<T>(rbv, boff(zorg + dz * (z0 + j), ST));
+      load_elements(w_cw, w_hx, w_hy, w_xw, w_exx, w_exy, w_bw, z0);
     }
   };
   // the next chunk's windows (PF)
-  Raw pcw[K + 2], phx[K + 2], phy[K + 2];
-  T pxw[K + 1], pexx[K + 1], pexy[K + 1], pbw[K];
+  Raw pcw[K + 1], phx[K], phy[K];
+  T pxw[K + 1], pexx[K], pexy[K], pbw[K];
   if constexpr (PF) load_windows(pcw, phx, phy, pxw, pexx, pexy, pbw, 0);
   for (int64_t t0 = 0; t0 < nsteps; t0 += K) {
-    Raw cw[K + 2], hx[K + 2], hy[K + 2];
-    T xw[K + 1], exx[K + 1], exy[K + 1], bw[K];
+    Raw cw[K + 1], hx[K], hy[K];
+    T xw[K + 1], exx[K], exy[K], bw[K];
     if constexpr (PF) {
 #pragma unroll
-      for (int j = 0; j < K + 2; ++j) {
+      for (int j = 0; j < K + 1; ++j) {
         cw[j] = pcw[j];
+        xw[j] = pxw[j];
+      }
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
         hx[j] = phx[j];
         hy[j] = phy[j];
-      }
-#pragma unroll
-      for (int j = 0; j < K + 1; ++j) {
-        xw[j] = pxw[j];
         exx[j] = pexx[j];
         exy[j] = pexy[j];
+        bw[j] = pbw[j];
       }
-#pragma unroll
-      for (int j = 0; j < K; ++j) bw[j] = pbw[j];
     } else {
       load_windows(cw, hx, hy, xw, exx, exy, bw, t0);
     }
@@ -380,21 +400,18 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
     const bool run_store = RUNS && __builtin_amdgcn_ballot_w64(line && (z0 < 0 || z0 + K > nz)) == 0;
     T xo[K];
     // ---- new border values of the upstream columns: prefetched from their
-    // mailboxes now (every lane reads a word of its upstream line, clamped
-    // into it, and lanes that take none discard it), polled again at the step
-    // that needs them if still unset ----
-    // f32 words inside every taking line's range come as 16-byte coherent
-    // runs (each 4-byte word is read whole; a word still unset reads as the
-    // sentinel and is polled at its step)
+    // mailboxes now, polled again at the step that needs them if still
+    // unset ----
+    // Words inside the taking lines' ranges come as 16-byte sc1 runs (each
+    // aligned 4- or 8-byte word is read whole; a word still unset reads as
+    // the sentinel and is polled at its step).  The lanes of a chunk that
+    // reaches past the line's ends read zeros from the runs and their words
+    // one by one in a masked block, so no path skips the runs.
     T nwx[K], nwy[K];
-    const bool mruns = sizeof(T) == 4 &&
-                       __builtin_amdgcn_ballot_w64((take_x || take_y) && (z0 < 0 || z0 + K > nz)) == 0;
-    if (mruns) {
-      if constexpr (sizeof(T) == 4) {
-        load_run<T, K, false, 16>(rmb, take_x ? uint32_t(in_x - mbox + z0) * ST : kRunOff, nwx);
-        load_run<T, K, false, 16>(rmb, take_y ? uint32_t(in_y - mbox + z0) * ST : kRunOff, nwy);
-      }
-    } else {
+    const bool m_in = z0 >= 0 && z0 + K <= nz;
+    load_run<T, K, false, 16>(rix, take_x && m_in ? uint32_t(lx_off + z0) * ST : kRunOff, nwx);
+    load_run<T, K, false, 16>(riy, take_y && m_in ? uint32_t(ly_off + z0) * ST : kRunOff, nwy);
+    if ((take_x || take_y) && !m_in) {
 #pragma unroll
       for (int j = 0; j < K; ++j) {
         const int32_t lz = z0 + j;
@@ -406,9 +423,12 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
         nwy[j] = take_y && ok ? vy : T(0);
       }
     }
-    if constexpr (PF) {
-      if (t0 + K < nsteps) load_windows(pcw, phx, phy, pxw, pexx, pexy, pbw, t0 + K);
-    }
+    // issued on every chunk, the last one included (its windows lie beyond
+    // the line and load clamped elements that no step uses): every chunk then
+    // issues the same sequence of vector memory instructions, so the waits
+    // for the mailbox words below count the window loads behind them and do
+    // not wait for those
+    if constexpr (PF) load_windows(pcw, phx, phy, pxw, pexx, pexy, pbw, t0 + K);
     // ---- K steps from registers ----
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -429,11 +449,15 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
         }
         if (take_x) nwx[k] = mailbox_poll(in_x + lz, err);
         if (take_y) nwy[k] = mailbox_poll(in_y + lz, err);
+        // leave with no load of this block in flight: otherwise the registers
+        // it loaded would count as pending on the path that skips it too, and
+        // their later reuse would wait for the next chunk's windows there
+        __builtin_amdgcn_s_waitcnt(0);
       }
       // successor data of the neighbouring lanes (same logical z)
       // (lane +-1 by DPP whole-wave shifts, lane +-8 by permutes)
-      Raw csx = shfl_raw(cw[k + 2], lane + CB < 64 ? lane + CB : lane);
-      Raw csy = shift<kWaveShl1>(cw[k + 2], cw[k + 2]);
+      Raw csx = shfl_raw(cw[k + 1], lane + CB < 64 ? lane + CB : lane);
+      Raw csy = shift<kWaveShl1>(cw[k + 1], cw[k + 1]);
       T xsx = __shfl(xw[k + 1], lane + CB < 64 ? lane + CB : lane, 64);
       T xsy = shift<kWaveShl1>(xw[k + 1], xw[k + 1]);
       // predecessor results of the neighbouring lanes (one step ago)
@@ -442,22 +466,22 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
       const T wpx_in = __shfl(wpx_prev, lane >= CB ? lane - CB : lane, 64);
       const T wpy_in = shift<kWaveShr1>(wpy_prev, wpy_prev);
       if (give_x) {
-        csx = hx[k + 1];
+        csx = hx[k];
         xsx = exx[k];
       }
       if (give_y) {
-        csy = hy[k + 1];
+        csy = hy[k];
         xsy = exy[k];
       }
-      const Raw cc = cw[k + 1];
+      const Raw cc = cw[k];
       // face weights: successor faces computed here, predecessor faces shared
       // (computed unconditionally, absent faces selected to 0)
       const bool has_sz = lz + 1 < nz, has_pz = lz > 0;
       const T c_sx = conn_of<T, KIND, FD>(s, cc, csx, qx, 0);
       const T c_sy = conn_of<T, KIND, FD>(s, cc, csy, qy, 1);
-      const T c_sz = conn_of<T, KIND, FD>(s, cc, cw[k + 2], qz, 2);
-      const T c_px = conn_of<T, KIND, FD>(s, cc, hx[k + 1], qx, 0);
-      const T c_py = conn_of<T, KIND, FD>(s, cc, hy[k + 1], qy, 1);
+      const T c_sz = conn_of<T, KIND, FD>(s, cc, cw[k + 1], qz, 2);
+      const T c_px = conn_of<T, KIND, FD>(s, cc, hx[k], qx, 0);
+      const T c_py = conn_of<T, KIND, FD>(s, cc, hy[k], qy, 1);
       const T w_sx = has_sx ? c_sx : T(0);
       const T w_sy = has_sy ? c_sy : T(0);
       const T w_sz = has_sz ? c_sz : T(0);
@@ -509,10 +533,12 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
         xn = fma(omega, quot, m1w * xi);
       }
       xo[k] = xn;
-      if (!run_store) rstore<T>(xn, rxo, valid ? boff(zorg + dz * lz, ST) : kOff);
-      // border lines publish their values to the downstream columns
-      if (valid && give_x) __hip_atomic_store(out_x + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (valid && give_y) __hip_atomic_store(out_y + lz, xn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // stores without branches (lanes that store nothing use kOff), so that
+      // every step issues the same vector memory instructions; the border
+      // lines publish their values to the downstream columns with sc1 stores
+      rstore<T>(xn, rxo, valid && !run_store ? boff(zorg + dz * lz, ST) : kOff);
+      rstore<T, 16>(xn, rox, valid && give_x ? uint32_t(lx_off + lz) * ST : kOff);
+      rstore<T, 16>(xn, roy, valid && give_y ? uint32_t(ly_off + lz) * ST : kOff);
       // a lane's values of a step outside its line are read only by steps
       // outside the box
       prev = xn;
@@ -520,9 +546,376 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
       wpy_prev = w_sy;
       wpz_prev = w_sz;
     }
-    if constexpr (RUNS) {
-      if (run_store) store_run<T, K, !FWD>(xo, rxo, line ? uint32_t(zb) * ST : kRunOff);
+    if constexpr (RUNS) store_run<T, K, !FWD>(xo, rxo, line && run_store ? uint32_t(zb) * ST : kRunOff);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same sweep with the operator's coefficients cached in the sweep's own
+// order.  Every cell the forward sweep visits gets a record of eight values
+// {w-x, w+x, w-y, w+y, w-z, w+z, d, 0}: the weights of its six physical faces
+// (0 where there is no neighbour) and the diagonal, computed by
+// sor_records_kernel with the operations the on-the-fly sweep performs (face
+// weights are symmetric bit for bit; the diagonal sums in the same order), so
+// both sweeps give the same bits.  The records are stored per column, per
+// chunk of K = 4 steps, in 16-byte slices interleaved over the 64 lanes:
+//     [column][chunk][slice q][lane][16 bytes]
+// so one 16-byte load of a wave reads 1 KiB of consecutive memory (a lane's
+// K records are its NQ slices).  The backward sweep reads the same records:
+// with its column grid anchored at the box's far corner (its first logical
+// column starts rx = (8 - nx % 8) % 8 lines before the box), backward column
+// C, lane L and step t' visit the cell of forward column ncols - 1 - C, lane
+// 63 - L, step nsteps - 1 - t'.  Forward chunks start at step -s with
+// s = (4 - nsteps % 4) % 4, so that the chunks of both sweeps are the same
+// sets of four steps.  A step then takes its six face weights and diagonal
+// from registers: what remains on its chain is the hand-off of the new
+// values, the fused multiply-adds and the division.
+
+constexpr int kRecK = 4;     // steps per chunk of the record sweep
+constexpr int kRecVals = 8;  // values per record
+
+struct RecGeom {
+  int64_t cx, cy, ncols, nsteps, s, nc, rx, ry;
+};
+__host__ __device__ inline RecGeom rec_geom(const StencilDesc& d) {
+  RecGeom g;
+  g.cx = (d.nx + CB - 1) / CB;
+  g.cy = (d.ny + CB - 1) / CB;
+  g.ncols = g.cx * g.cy;
+  g.nsteps = d.nz + 2 * (CB - 1);
+  g.s = (kRecK - g.nsteps % kRecK) % kRecK;
+  g.nc = (g.nsteps + g.s) / kRecK;
+  g.rx = (CB - d.nx % CB) % CB;
+  g.ry = (CB - d.ny % CB) % CB;
+  return g;
+}
+// 16-byte slices per lane and chunk
+template <typename T>
+constexpr int kRecSlices = kRecK * kRecVals * int(sizeof(T)) / 16;
+
+template <typename T, int KIND, bool FD>
+__global__ __launch_bounds__(256) void sor_records_kernel(StencilDesc s, RecGeom g, T* __restrict__ w) {
+  using CI = CellInfo<T, KIND>;
+  constexpr int NQ = kRecSlices<T>;
+  const int64_t nx = s.nx, ny = s.ny, nz = s.nz, nyz = ny * nz;
+  const T qx = T(s.q[0]), qy = T(s.q[1]), qz = T(s.q[2]);
+  const T two_q0 = T(2) * qx;
+  const T adv = T(s.adv);
+  const int64_t total = g.ncols * g.nc * kRecK * 64;
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < total;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const int L = int(idx % 64);
+    int64_t r = idx / 64;
+    const int k = int(r % kRecK);
+    r /= kRecK;
+    const int64_t c = r % g.nc, col = r / g.nc;
+    const int64_t I = col / g.cy, J = col % g.cy;
+    const int a = L / CB, b = L % CB;
+    const int64_t x = I * CB + a, y = J * CB + b, z = kRecK * c + k - g.s - a - b;
+    T v[kRecVals] = {};
+    if (x < nx && y < ny && z >= 0 && z < nz) {
+      const int64_t p = x * nyz + y * nz + z;
+      const typename CI::Raw cc = CI::load(s, p);
+      auto face = [&](bool has, int64_t q, T qd, int d) {
+        return has ? conn_of<T, KIND, FD>(s, cc, CI::load(s, q), qd, d) : T(0);
+      };
+      const T wmx = face(x > 0, p - nyz, qx, 0), wpx = face(x + 1 < nx, p + nyz, qx, 0);
+      const T wmy = face(y > 0, p - nz, qy, 1), wpy = face(y + 1 < ny, p + nz, qy, 1);
+      const T wmz = face(z > 0, p - 1, qz, 2), wpz = face(z + 1 < nz, p + 1, qz, 2);
+      // the diagonal in the reference's visiting order (algorithm.hpp:106-226)
+      const T ci = CI::value(s, cc);
+      T d = T(0);
+      if (x == 0) d += two_q0 * ci;
+      if (x + 1 == nx) d += two_q0 * ci;
+      d += wmx;
+      d += wmy;
+      d += wmz;
+      d += wpz;
+      d += wpy;
+      d += wpx;
+      d += adv;
+      if (s.zero_to_one && d == T(0)) d = T(1);
+      v[0] = wmx;
+      v[1] = wpx;
+      v[2] = wmy;
+      v[3] = wpy;
+      v[4] = wmz;
+      v[5] = wpz;
+      v[6] = d;
     }
+    constexpr int per16 = 16 / int(sizeof(T));
+#pragma unroll
+    for (int j = 0; j < kRecVals; ++j) {
+      const int i = k * kRecVals + j;  // value index within the lane's chunk
+      const int q = i / per16, within = i % per16;
+      w[(((col * g.nc + c) * NQ + q) * 64 + L) * per16 + within] = v[j];
+    }
+  }
+}
+
+template <typename T, bool FWD, bool ULTRA, bool HAS_X>
+__global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
+                                                      const T* __restrict__ w,
+                                                      const T* __restrict__ xin,
+                                                      T* __restrict__ xout,
+                                                      const T* __restrict__ bv, T omega,
+                                                      int* state, int* err,
+                                                      const uint32_t* order, T* mbox) {
+  constexpr int K = kRecK;
+  constexpr int NQ = kRecSlices<T>;
+  constexpr int W = int(sizeof(T) / 4);  // dwords per value
+  constexpr uint32_t ST = sizeof(T);
+  const int lane = threadIdx.x;
+  const int a = lane / CB, b = lane % CB;
+  __shared__ int s_ticket;
+  if (lane == 0) s_ticket = atomicAdd(state, 1);
+  __syncthreads();
+  const int64_t cy = g.cy;
+  const int64_t col = order[s_ticket];
+  const int64_t I = col / cy, J = col % cy;
+  const int64_t nx = s.nx, ny = s.ny, nz = s.nz, nyz = ny * nz;
+  // logical line coordinates (the backward grid starts rx / ry lines before
+  // the box)
+  const int64_t lx = I * CB + a - (FWD ? 0 : g.rx), ly = J * CB + b - (FWD ? 0 : g.ry);
+  const bool line = lx >= 0 && lx < nx && ly >= 0 && ly < ny;
+  const int64_t x = FWD ? lx : nx - 1 - lx;  // physical x
+  const bool has_sx = lx + 1 < nx, has_px = lx > 0;
+  const bool has_sy = ly + 1 < ny, has_py = ly > 0;
+  // the physical planes of logical x in [lx(a = 0) - 1, lx(a = 7) + 1]
+  const int64_t lxa = I * CB - (FWD ? 0 : g.rx);
+  const int64_t l0 = lxa > 0 ? lxa - 1 : 0;
+  const int64_t l1 = lxa + CB < nx - 1 ? lxa + CB : nx - 1;
+  const int64_t p0 = FWD ? l0 : nx - 1 - l1, p1 = FWD ? l1 : nx - 1 - l0;
+  const int64_t relems = (p1 - p0 + 1) * nyz;
+  const Rsrc rxi = col_rsrc(xin, p0 * nyz, HAS_X ? relems : 0);
+  const Rsrc rbv = col_rsrc(bv, p0 * nyz, relems);
+  const Rsrc rxo = col_rsrc(static_cast<const T*>(xout), p0 * nyz, relems);
+  // the column's records (forward column ncols - 1 - col when backward) and
+  // this lane's slot in them
+  const int64_t rcol = FWD ? col : g.ncols - 1 - col;
+  const int64_t rbytes = g.nc * NQ * 64 * 16;
+  const Rsrc rrec = col_rsrc(w, rcol * (rbytes / ST), rbytes / ST);
+  const uint32_t rlane = uint32_t(FWD ? lane : 63 - lane) * 16u;
+  // element offset (in the resources) of logical z = 0 on this line (lines
+  // outside the box are clamped into it), and the element steps of logical
+  // +z / +x / +y
+  const int64_t cxl = lx < 0 ? 0 : (lx < nx ? lx : nx - 1);
+  const int64_t cyl = ly < 0 ? 0 : (ly < ny ? ly : ny - 1);
+  const int64_t pxl = FWD ? cxl : nx - 1 - cxl, pyl = FWD ? cyl : ny - 1 - cyl;
+  const int32_t dz = FWD ? 1 : -1;
+  const int32_t zorg = int32_t((pxl - p0) * nyz + pyl * nz + (FWD ? 0 : nz - 1));
+  const int32_t hsx = int32_t(FWD ? nyz : -nyz), hsy = int32_t(FWD ? nz : -nz);
+  const int32_t emax = int32_t(relems - 1);
+  auto boff = [emax](int32_t e, uint32_t sz) {
+    return uint32_t(e < 0 ? 0 : (e > emax ? emax : e)) * sz;
+  };
+  const T m1w = T(1) - omega;
+  const T adv = T(s.adv);  // upwind advection: the -x coupling
+  // Mailboxes: 15 lines of nz words per column, the new values of its
+  // downstream border lines: line b for lane (7, b), line 8 + a for lane
+  // (a, 7) with a < 7 (lane (7, 7)'s values, line 7, serve both
+  // directions).  One store per step publishes both borders.
+  const int32_t nzi = int32_t(nz);
+  const int64_t col_x = (I > 0 ? I - 1 : 0) * cy + J, col_y = I * cy + (J > 0 ? J - 1 : 0);
+  const bool take_x = line && a == 0 && has_px;
+  const bool take_y = line && b == 0 && has_py;
+  const bool give_x = a == CB - 1, give_y = b == CB - 1;
+  const int out_line = give_x ? b : (give_y ? CB + a : -1);
+  const int in_y_line = a < CB - 1 ? CB + a : CB - 1;
+  const Rsrc rmo = col_rsrc(static_cast<const T*>(mbox), col * 15 * nz, 15 * nz);
+  const Rsrc rmx = col_rsrc(static_cast<const T*>(mbox), col_x * 15 * nz, 15 * nz);
+  const Rsrc rmy = col_rsrc(static_cast<const T*>(mbox), col_y * 15 * nz, 15 * nz);
+  const T* const in_x = mbox + col_x * 15 * nz + int64_t(b) * nz;
+  const T* const in_y = mbox + col_y * 15 * nz + int64_t(in_y_line) * nz;
+  const int32_t ix_off = b * nzi, iy_off = in_y_line * nzi, o_off = out_line * nzi;
+  T prev = T(0);
+  // A chunk's windows (steps t0 .. t0 + K - 1, logical z0 = t0 - a - b):
+  //   rec [K][8]   the cells' records
+  //   xw [K + 1]   old values of the own line, z0 .. z0 + K
+  //   exx, exy [K] old values of the +x / +y line beyond the column
+  //   bw [K]       right-hand side
+  struct Win {
+    T rec[K][kRecVals];
+    T xw[K + 1], exx[K], exy[K], bw[K];
+  };
+  auto load_elements = [&](Win& v, int32_t z0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int32_t e = zorg + dz * (z0 + j);
+      v.bw[j] = rload<T>(rbv, boff(e, ST));
+      v.exx[j] = HAS_X ? rload<T>(rxi, give_x && has_sx ? boff(e + hsx, ST) : kOff) : T(0);
+      v.exy[j] = HAS_X ? rload<T>(rxi, give_y && has_sy ? boff(e + hsy, ST) : kOff) : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < K + 1; ++j)
+      v.xw[j] = HAS_X ? rload<T>(rxi, boff(zorg + dz * (z0 + j), ST)) : T(0);
+  };
+  // chunk index c (0 .. nc - 1) -> its windows; the natural-layout windows
+  // come as 16-byte runs, patched element by element in a masked block for
+  // the lanes whose runs reach past the column's resource (no path skips
+  // the runs)
+  auto load_windows = [&](Win& v, int64_t c) {
+    const int64_t t0 = FWD ? kRecK * c - g.s : kRecK * c;
+    {
+      const int64_t fc = FWD ? c : g.nc - 1 - c;
+      uint32_t d[NQ * 4];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        using V4 = unsigned int __attribute__((ext_vector_type(4)));
+        const V4 u = __builtin_amdgcn_raw_buffer_load_b128(
+            rrec, uint32_t(((fc * NQ + q) * 64) * 16) + rlane, 0u, 0);
+        d[4 * q] = u[0];
+        d[4 * q + 1] = u[1];
+        d[4 * q + 2] = u[2];
+        d[4 * q + 3] = u[3];
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int j = 0; j < kRecVals; ++j) {
+          const int m = ((FWD ? k : K - 1 - k) * kRecVals + j) * W;
+          if constexpr (W == 1) v.rec[k][j] = __builtin_bit_cast(T, d[m]);
+          else v.rec[k][j] = __builtin_bit_cast(T, uint64_t(d[m]) | (uint64_t(d[m + 1]) << 32));
+        }
+    }
+    const int32_t z0 = int32_t(t0) - a - b;
+    const int32_t zk = FWD ? zorg + z0 : zorg - z0 - (K - 1);  // K elements
+    const int32_t z1 = FWD ? zorg + z0 : zorg - z0 - K;        // xw: K + 1
+    auto out = [emax](int32_t lo, int n) { return lo < 0 || lo + n - 1 > emax; };
+    const bool bad = line && (out(zk, K) || (HAS_X && out(z1, K + 1)) ||
+                              (HAS_X && give_x && has_sx && out(zk + hsx, K)) ||
+                              (HAS_X && give_y && has_sy && out(zk + hsy, K)));
+    const bool ok = line && !bad;
+    load_run<T, K, !FWD>(rbv, ok ? uint32_t(zk) * ST : kRunOff, v.bw);
+    if constexpr (HAS_X) {
+      load_run<T, K + 1, !FWD>(rxi, ok ? uint32_t(z1) * ST : kRunOff, v.xw);
+      load_run<T, K, !FWD>(rxi, ok && give_x && has_sx ? uint32_t(zk + hsx) * ST : kRunOff, v.exx);
+      load_run<T, K, !FWD>(rxi, ok && give_y && has_sy ? uint32_t(zk + hsy) * ST : kRunOff, v.exy);
+    } else {
+#pragma unroll
+      for (int j = 0; j < K + 1; ++j) v.xw[j] = T(0);
+#pragma unroll
+      for (int j = 0; j < K; ++j) v.exx[j] = v.exy[j] = T(0);
+    }
+    if (bad) load_elements(v, z0);
+  };
+  // One chunk: its steps from `cur` while the windows two chunks ahead load
+  // into `fill` (two chunks in flight).  Three buffers rotate by name in the
+  // loop below, so no window is copied between registers.
+  auto chunk = [&](const Win& cur, Win& fill, int64_t c) {
+    const int64_t t0 = FWD ? kRecK * c - g.s : kRecK * c;
+    const int32_t z0 = int32_t(t0) - a - b;
+    const int32_t zb = FWD ? zorg + z0 : zorg - z0 - (K - 1);  // lowest result element
+    // lanes whose K steps are all in their line store the chunk as one run;
+    // the others store element by element in a masked block
+    const bool full = line && z0 >= 0 && z0 + K <= nz;
+    T xo[K];
+    // new border values of the upstream columns, prefetched (as in
+    // sor_wavefront)
+    T nwx[K], nwy[K];
+    const bool m_in = z0 >= 0 && z0 + K <= nz;
+    load_run<T, K, false, 16>(rmx, take_x && m_in ? uint32_t(ix_off + z0) * ST : kRunOff, nwx);
+    load_run<T, K, false, 16>(rmy, take_y && m_in ? uint32_t(iy_off + z0) * ST : kRunOff, nwy);
+    if ((take_x || take_y) && !m_in) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int32_t lz = z0 + j;
+        const bool okz = lz >= 0 && lz < nz;
+        const int32_t cz = lz < 0 ? 0 : (lz >= nz ? nzi - 1 : lz);
+        const T vx = __hip_atomic_load(in_x + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const T vy = __hip_atomic_load(in_y + cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        nwx[j] = take_x && okz ? vx : T(0);
+        nwy[j] = take_y && okz ? vy : T(0);
+      }
+    }
+    load_windows(fill, c + 2);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int32_t lz = z0 + k;
+      const bool valid = line && lz >= 0 && lz < nz;
+      if (valid && ((take_x && is_sentinel(nwx[k])) || (take_y && is_sentinel(nwy[k])))) {
+#pragma unroll
+        for (int j = k + 1; j < K; ++j) {
+          const int32_t lzj = z0 + j;
+          const bool okj = lzj >= 0 && lzj < nz;
+          const int32_t czj = lzj < 0 ? 0 : (lzj >= nz ? nzi - 1 : lzj);
+          const T vx = __hip_atomic_load(in_x + czj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const T vy = __hip_atomic_load(in_y + czj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (take_x && okj && is_sentinel(nwx[j])) nwx[j] = vx;
+          if (take_y && okj && is_sentinel(nwy[j])) nwy[j] = vy;
+        }
+        if (take_x) nwx[k] = mailbox_poll(in_x + lz, err);
+        if (take_y) nwy[k] = mailbox_poll(in_y + lz, err);
+        // leave with no load of this block in flight (see sor_wavefront)
+        __builtin_amdgcn_s_waitcnt(0);
+      }
+      // old values of the logical +x / +y neighbours (same logical z) and the
+      // new values of the logical -x / -y neighbours (one step ago)
+      T xsx = __shfl(cur.xw[k + 1], lane + CB < 64 ? lane + CB : lane, 64);
+      T xsy = shift<kWaveShl1>(cur.xw[k + 1], cur.xw[k + 1]);
+      const T npx = __shfl(prev, lane >= CB ? lane - CB : lane, 64);
+      const T npy = shift<kWaveShr1>(prev, prev);
+      if (give_x) xsx = cur.exx[k];
+      if (give_y) xsy = cur.exy[k];
+      const bool has_sz = lz + 1 < nz, has_pz = lz > 0;
+      // (backward, the grid's first column may hold lanes left of the box:
+      // a lane at logical x or y = 0 takes no value from its neighbour)
+      T n_px = a == 0 ? nwx[k] : npx;
+      T n_py = b == 0 ? nwy[k] : npy;
+      if constexpr (!FWD) {
+        n_px = has_px ? n_px : T(0);
+        n_py = has_py ? n_py : T(0);
+      }
+      const T n_pz = has_pz ? prev : T(0);
+      const T o_sx = has_sx ? xsx : T(0);
+      const T o_sy = has_sy ? xsy : T(0);
+      const T o_sz = has_sz ? cur.xw[k + 1] : T(0);
+      const T vmx = FWD ? n_px : o_sx, vpx = FWD ? o_sx : n_px;
+      const T vmy = FWD ? n_py : o_sy, vpy = FWD ? o_sy : n_py;
+      const T vmz = FWD ? n_pz : o_sz, vpz = FWD ? o_sz : n_pz;
+      const T* r = cur.rec[k];
+      const T xi = cur.xw[k];
+      T acc = cur.bw[k];
+      acc = fma(x > 0 ? r[0] + adv : T(0), vmx, acc);
+      acc = fma(r[2], vmy, acc);
+      acc = fma(r[4], vmz, acc);
+      acc = fma(r[5], vpz, acc);
+      acc = fma(r[3], vpy, acc);
+      acc = fma(r[1], vpx, acc);
+      const T d = r[6];
+      T xn;
+      if (ULTRA) {
+        acc = fma(-d, xi, acc);
+        const T ad = d < T(0) ? -d : d;
+        xn = xi + (omega * acc) / (ad + T(0.5) * T(0));
+      } else {
+        const T quot = acc / d;
+        xn = fma(omega, quot, m1w * xi);
+      }
+      xo[k] = xn;
+      rstore<T, 16>(xn, rmo, valid && out_line >= 0 ? uint32_t(o_off + lz) * ST : kOff);
+      prev = xn;
+    }
+    store_run<T, K, !FWD>(xo, rxo, full ? uint32_t(zb) * ST : kRunOff);
+    if (line && !full) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int32_t lz = z0 + k;
+        rstore<T>(xo[k], rxo, lz >= 0 && lz < nz ? boff(zorg + dz * lz, ST) : kOff);
+      }
+    }
+  };
+  Win w0, w1, w2;
+  load_windows(w0, 0);
+  load_windows(w1, 1);
+  const int64_t nc = g.nc;
+  for (int64_t c = 0;; c += 3) {
+    chunk(w0, w2, c);
+    if (c + 1 >= nc) break;
+    chunk(w1, w0, c + 1);
+    if (c + 2 >= nc) break;
+    chunk(w2, w1, c + 2);
+    if (c + 3 >= nc) break;
   }
 }
 
@@ -565,7 +958,68 @@ void launch2(const StencilDesc& s, const void* x_in, void* x_out, const void* b,
   }
 }
 
+template <typename T, bool FWD, bool ULTRA>
+void launch_r(const StencilDesc& s, const void* w, const void* x_in, void* x_out, const void* b,
+              double omega, const SorSched& sched, hipStream_t st) {
+  const RecGeom g = rec_geom(s);
+  const dim3 grid(unsigned(sched.ncols)), block(64);
+  if (x_in) {
+    hipLaunchKernelGGL((sor_wavefront_r<T, FWD, ULTRA, true>), grid, block, 0, st, s, g,
+                       static_cast<const T*>(w), static_cast<const T*>(x_in), static_cast<T*>(x_out),
+                       static_cast<const T*>(b), T(omega), sched.state, sched.err, sched.order,
+                       static_cast<T*>(sched.mailbox));
+  } else {
+    hipLaunchKernelGGL((sor_wavefront_r<T, FWD, ULTRA, false>), grid, block, 0, st, s, g,
+                       static_cast<const T*>(w), static_cast<const T*>(x_in), static_cast<T*>(x_out),
+                       static_cast<const T*>(b), T(omega), sched.state, sched.err, sched.order,
+                       static_cast<T*>(sched.mailbox));
+  }
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_r2(const StencilDesc& s, const void* w, const void* x_in, void* x_out, const void* b,
+               double omega, bool fwd, bool ultra, const SorSched& sched, hipStream_t st) {
+  if (fwd) {
+    if (ultra) launch_r<T, true, true>(s, w, x_in, x_out, b, omega, sched, st);
+    else launch_r<T, true, false>(s, w, x_in, x_out, b, omega, sched, st);
+  } else {
+    if (ultra) launch_r<T, false, true>(s, w, x_in, x_out, b, omega, sched, st);
+    else launch_r<T, false, false>(s, w, x_in, x_out, b, omega, sched, st);
+  }
+}
+
+template <typename T, int KIND>
+void launch_records(const StencilDesc& s, void* w, hipStream_t st) {
+  const RecGeom g = rec_geom(s);
+  const int64_t n = g.ncols * g.nc * kRecK * 64;
+  const dim3 grid(unsigned(std::min<int64_t>((n + 255) / 256, 65536))), block(256);
+  const bool fd = KIND == 0 && (sizeof(T) == 8 ? s.fast_div64 : s.fast_div32);
+  if (fd) hipLaunchKernelGGL((sor_records_kernel<T, KIND, true>), grid, block, 0, st, s, g, static_cast<T*>(w));
+  else hipLaunchKernelGGL((sor_records_kernel<T, KIND, false>), grid, block, 0, st, s, g, static_cast<T*>(w));
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
+
+// a record per column, chunk slot and lane; 0 when a column's records would
+// not fit 32-bit byte offsets (the sweeps then evaluate the stencil)
+size_t sor_weights_bytes(const StencilDesc& s, bool f32) {
+  const RecGeom g = rec_geom(s);
+  const int64_t per_col = g.nc * kRecK * 64 * kRecVals * int64_t(f32 ? sizeof(float) : sizeof(double));
+  if (per_col >= (int64_t(1) << 31)) return 0;
+  return size_t(g.ncols * per_col);
+}
+
+void stencil_sor_weights(const StencilDesc& s, bool f32, void* w, hipStream_t st) {
+  if (f32) {
+    if (s.kind == 0) launch_records<float, 0>(s, w, st);
+    else launch_records<float, 1>(s, w, st);
+  } else {
+    if (s.kind == 0) launch_records<double, 0>(s, w, st);
+    else launch_records<double, 1>(s, w, st);
+  }
+}
 
 size_t sor_sched_ints(const StencilDesc&) { return 2; }
 
@@ -589,7 +1043,8 @@ void sor_sched_order(const StencilDesc& s, std::vector<uint32_t>& order) {
 }
 
 void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, const void* b,
-                 double omega, bool forward, bool ultra, const SorSched& sched, hipStream_t st) {
+                 double omega, bool forward, bool ultra, const SorSched& sched, hipStream_t st,
+                 const void* weights) {
   LGPU_REQUIRE(x_in != x_out, "SOR sweep needs distinct input and output vectors");
   // a column's ten planes are addressed with 32-bit byte offsets below 2^31
   LGPU_REQUIRE(10 * s.ny * s.nz * int64_t(f32 ? 4 : 8) < (int64_t(1) << 31),
@@ -597,13 +1052,16 @@ void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, 
   // reset the ticket; the error word (sched.err) is sticky until the host
   // clears it
   vec_fill_int(sched.state, 1, 0, st);
+  // one word per border-line cell, of the sweep's precision
   const int64_t words = int64_t(sor_mailbox_bytes(s) / sizeof(double));
-  const int64_t fw = f32 ? 2 * words : words;
-  const dim3 rg(unsigned(std::min<int64_t>((fw + 255) / 256, 4096))), rb(256);
-  if (f32) hipLaunchKernelGGL(mailbox_reset<float>, rg, rb, 0, st, static_cast<float*>(sched.mailbox), fw);
-  else hipLaunchKernelGGL(mailbox_reset<double>, rg, rb, 0, st, static_cast<double*>(sched.mailbox), fw);
+  const dim3 rg(unsigned(std::min<int64_t>((words + 255) / 256, 4096))), rb(256);
+  if (f32) hipLaunchKernelGGL(mailbox_reset<float>, rg, rb, 0, st, static_cast<float*>(sched.mailbox), words);
+  else hipLaunchKernelGGL(mailbox_reset<double>, rg, rb, 0, st, static_cast<double*>(sched.mailbox), words);
   LGPU_HIP_CHECK(hipGetLastError());
-  if (f32) {
+  if (weights) {
+    if (f32) launch_r2<float>(s, weights, x_in, x_out, b, omega, forward, ultra, sched, st);
+    else launch_r2<double>(s, weights, x_in, x_out, b, omega, forward, ultra, sched, st);
+  } else if (f32) {
     if (s.kind == 0) launch2<float, 0>(s, x_in, x_out, b, omega, forward, ultra, sched, st);
     else launch2<float, 1>(s, x_in, x_out, b, omega, forward, ultra, sched, st);
   } else {

@@ -33,12 +33,15 @@ def main(argv=None):
     ap.add_argument("kernels", nargs="+")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--sor-cache", type=int, default=1,
+                    help="1: SOR sweeps read cached face weights and diagonal; 0: evaluated on the fly")
     args = ap.parse_args(argv)
     n = args.n
     dims = (n, n, n)
     N = n ** 3
     ctx = la.default_context()
     A = la.StencilMatrix.values(la.BoxInfo(dims), problems.lognormal(dims, 1.0, 1), ctx)
+    A.set_sor_cache(bool(args.sor_cache))
     rot = 3
     vec = {}
 

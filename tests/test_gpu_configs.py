@@ -111,9 +111,13 @@ def test_jacobi_256_ten_sweeps(field256, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_sor_256_ten_sweeps(field256, dtype):
-    """10 lexicographic SOR sweeps (omega 1), alternating forward and backward."""
+@pytest.mark.parametrize("cache", [True, False])
+def test_sor_256_ten_sweeps(field256, dtype, cache):
+    """10 lexicographic SOR sweeps (omega 1), alternating forward and backward,
+    with the cached face weights and diagonal and with them evaluated on the
+    fly."""
     dims, c, A, S = field256
+    A.set_sor_cache(cache)
     x = problems.uniform(S.N, 2).astype(dtype)
     b = S.rhs(dtype)
     bv, xa, xb = la.DenseVector(b), la.DenseVector(x), la.DenseVector(S.N, dtype)
@@ -124,6 +128,7 @@ def test_sor_256_ten_sweeps(field256, dtype):
         xa, xb = xb, xa
         ref = O.sor(S, ref, b, 1.0, fwd)
     np.testing.assert_array_equal(xa.numpy(), ref)
+    A.set_sor_cache(True)
 
 
 @pytest.mark.timeout(600)

@@ -67,8 +67,12 @@ def test_jacobi(dims, kind, dtype, with_x):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("forward", [True, False])
 @pytest.mark.parametrize("ultra", [False, True])
-def test_sor_wavefront(dims, kind, dtype, forward, ultra):
+@pytest.mark.parametrize("cache", [True, False])
+def test_sor_wavefront(dims, kind, dtype, forward, ultra, cache):
+    """Both sweep kernels: cached per-cell face weights and diagonal (the
+    default) and the weights evaluated on the fly."""
     A, S = make_pair(dims, kind)
+    A.set_sor_cache(cache)
     n = S.N
     x = problems.uniform(n, 2, dtype)
     b = problems.uniform(n, 3, dtype)
@@ -114,6 +118,11 @@ def test_values_coefficient_ranges(dims, which, dtype):
         same(A.residual(la.DenseVector(b), xd, la.DenseVector(n, dtype)).numpy(), O.residual(S, b, x))
         same(A.jacobi(xd, la.DenseVector(n, dtype), la.DenseVector(b), 0.7).numpy(),
              O.jacobi(S, x, b, dtype(0.7)))
+        for cache in (True, False):
+            A.set_sor_cache(cache)
+            for forward in (True, False):
+                out = A.sor(xd, la.DenseVector(n, dtype), la.DenseVector(b), 1.1, forward)
+                same(out.numpy(), O.sor(S, x, b, dtype(1.1), forward, False))
 
 
 @pytest.mark.parametrize("dims", [(4, 9, 7), (13, 17, 5), (33, 17, 65)])
@@ -217,9 +226,11 @@ def test_advection_kernels(dims, kind, dtype):
     same(A.rhs(dtype).numpy(), S.rhs(dtype))
     same(A.residual(la.DenseVector(b), xd, la.DenseVector(n, dtype)).numpy(), O.residual(S, b, x))
     same(A.jacobi(xd, la.DenseVector(n, dtype), la.DenseVector(b), 0.7).numpy(), O.jacobi(S, x, b, dtype(0.7)))
-    for forward in (True, False):
-        for ultra in (False, True):
-            for with_x in (True, False):
-                out = A.sor(xd if with_x else None, la.DenseVector(n, dtype), la.DenseVector(b), 1.1,
-                            forward, ultra)
-                same(out.numpy(), O.sor(S, x if with_x else None, b, dtype(1.1), forward, ultra))
+    for cache in (True, False):
+        A.set_sor_cache(cache)
+        for forward in (True, False):
+            for ultra in (False, True):
+                for with_x in (True, False):
+                    out = A.sor(xd if with_x else None, la.DenseVector(n, dtype), la.DenseVector(b), 1.1,
+                                forward, ultra)
+                    same(out.numpy(), O.sor(S, x if with_x else None, b, dtype(1.1), forward, ultra))
