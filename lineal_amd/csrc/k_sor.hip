@@ -552,27 +552,35 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
 
 // ---------------------------------------------------------------------------
 // The same sweep with the operator's coefficients cached in the sweep's own
-// order.  Every cell the forward sweep visits gets a record of eight values
-// {w-x, w+x, w-y, w+y, w-z, w+z, d, 0}: the weights of its six physical faces
-// (0 where there is no neighbour) and the diagonal, computed by
-// sor_records_kernel with the operations the on-the-fly sweep performs (face
-// weights are symmetric bit for bit; the diagonal sums in the same order), so
-// both sweeps give the same bits.  The records are stored per column, per
-// chunk of K = 4 steps, in 16-byte slices interleaved over the 64 lanes:
+// order.  Every cell the forward sweep visits gets a record {w+x, w+y, w+z,
+// d}: the weights of the faces to its physical +x / +y / +z neighbours (0
+// where there is none) and its diagonal, computed by sor_records_kernel with
+// the operations the on-the-fly sweep performs (face weights are symmetric
+// bit for bit; the diagonal sums in the same order), so both sweeps give the
+// same bits.  A cell's -x / -y / -z faces are the +x / +y / +z faces of its
+// neighbours: inside a column they come from the neighbouring lane (its
+// record one step earlier forward, one step later backward) or from the own
+// record of the previous / next step; the lanes on a column's -x and -y
+// borders (physical) read them from a halo array.
+//
+// Records are stored per column and per chunk of K = 4 steps, in 16-byte
+// slices interleaved over the 64 lanes:
 //     [column][chunk][slice q][lane][16 bytes]
-// so one 16-byte load of a wave reads 1 KiB of consecutive memory (a lane's
-// K records are its NQ slices).  The backward sweep reads the same records:
-// with its column grid anchored at the box's far corner (its first logical
-// column starts rx = (8 - nx % 8) % 8 lines before the box), backward column
-// C, lane L and step t' visit the cell of forward column ncols - 1 - C, lane
-// 63 - L, step nsteps - 1 - t'.  Forward chunks start at step -s with
+// so that one 16-byte load of a wave reads 1 KiB of consecutive memory (a
+// lane's K records are its NQ slices).  Halo values are stored per column and
+// chunk as 16 entries of K values: entry b is the -x face of lane (0, b),
+// entry 8 + a the -y face of lane (a, 0), so eight lanes read 128 consecutive
+// bytes.  The backward sweep reads the same arrays: with its column grid
+// anchored at the box's far corner (its first logical column starts
+// rx = (8 - nx % 8) % 8 lines before the box), backward column C, lane L and
+// step t' visit the cell of forward column ncols - 1 - C, lane 63 - L, step
+// nsteps - 1 - t'.  Forward chunks start at step -s with
 // s = (4 - nsteps % 4) % 4, so that the chunks of both sweeps are the same
-// sets of four steps.  A step then takes its six face weights and diagonal
-// from registers: what remains on its chain is the hand-off of the new
-// values, the fused multiply-adds and the division.
+// sets of four steps.
 
 constexpr int kRecK = 4;     // steps per chunk of the record sweep
-constexpr int kRecVals = 8;  // values per record
+constexpr int kRecVals = 4;  // values per record
+constexpr int kHaloEntries = 2 * CB;
 
 struct RecGeom {
   int64_t cx, cy, ncols, nsteps, s, nc, rx, ry;
@@ -592,15 +600,24 @@ __host__ __device__ inline RecGeom rec_geom(const StencilDesc& d) {
 // 16-byte slices per lane and chunk
 template <typename T>
 constexpr int kRecSlices = kRecK * kRecVals * int(sizeof(T)) / 16;
+// elements of the record array; the halo array follows it
+__host__ __device__ inline int64_t rec_elems(const RecGeom& g) {
+  return g.ncols * g.nc * kRecK * 64 * kRecVals;
+}
+__host__ __device__ inline int64_t halo_elems(const RecGeom& g) {
+  return g.ncols * g.nc * kHaloEntries * kRecK;
+}
 
 template <typename T, int KIND, bool FD>
 __global__ __launch_bounds__(256) void sor_records_kernel(StencilDesc s, RecGeom g, T* __restrict__ w) {
   using CI = CellInfo<T, KIND>;
   constexpr int NQ = kRecSlices<T>;
+  constexpr int per16 = 16 / int(sizeof(T));
   const int64_t nx = s.nx, ny = s.ny, nz = s.nz, nyz = ny * nz;
   const T qx = T(s.q[0]), qy = T(s.q[1]), qz = T(s.q[2]);
   const T two_q0 = T(2) * qx;
   const T adv = T(s.adv);
+  T* const halo = w + rec_elems(g);
   const int64_t total = g.ncols * g.nc * kRecK * 64;
   for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < total;
        idx += int64_t(gridDim.x) * blockDim.x) {
@@ -613,6 +630,7 @@ __global__ __launch_bounds__(256) void sor_records_kernel(StencilDesc s, RecGeom
     const int a = L / CB, b = L % CB;
     const int64_t x = I * CB + a, y = J * CB + b, z = kRecK * c + k - g.s - a - b;
     T v[kRecVals] = {};
+    T hmx = T(0), hmy = T(0);
     if (x < nx && y < ny && z >= 0 && z < nz) {
       const int64_t p = x * nyz + y * nz + z;
       const typename CI::Raw cc = CI::load(s, p);
@@ -635,21 +653,22 @@ __global__ __launch_bounds__(256) void sor_records_kernel(StencilDesc s, RecGeom
       d += wpx;
       d += adv;
       if (s.zero_to_one && d == T(0)) d = T(1);
-      v[0] = wmx;
-      v[1] = wpx;
-      v[2] = wmy;
-      v[3] = wpy;
-      v[4] = wmz;
-      v[5] = wpz;
-      v[6] = d;
+      v[0] = wpx;
+      v[1] = wpy;
+      v[2] = wpz;
+      v[3] = d;
+      hmx = wmx;
+      hmy = wmy;
     }
-    constexpr int per16 = 16 / int(sizeof(T));
 #pragma unroll
     for (int j = 0; j < kRecVals; ++j) {
       const int i = k * kRecVals + j;  // value index within the lane's chunk
       const int q = i / per16, within = i % per16;
       w[(((col * g.nc + c) * NQ + q) * 64 + L) * per16 + within] = v[j];
     }
+    const int64_t hbase = (col * g.nc + c) * kHaloEntries;
+    if (a == 0) halo[(hbase + b) * kRecK + k] = hmx;
+    if (b == 0) halo[(hbase + CB + a) * kRecK + k] = hmy;
   }
 }
 
@@ -690,12 +709,18 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
   const Rsrc rxi = col_rsrc(xin, p0 * nyz, HAS_X ? relems : 0);
   const Rsrc rbv = col_rsrc(bv, p0 * nyz, relems);
   const Rsrc rxo = col_rsrc(static_cast<const T*>(xout), p0 * nyz, relems);
-  // the column's records (forward column ncols - 1 - col when backward) and
-  // this lane's slot in them
+  // the column's records and halo values (forward column ncols - 1 - col
+  // when backward) and this lane's slots in them
   const int64_t rcol = FWD ? col : g.ncols - 1 - col;
   const int64_t rbytes = g.nc * NQ * 64 * 16;
   const Rsrc rrec = col_rsrc(w, rcol * (rbytes / ST), rbytes / ST);
   const uint32_t rlane = uint32_t(FWD ? lane : 63 - lane) * 16u;
+  const int64_t hcol = g.nc * kHaloEntries * kRecK;  // halo elements per column
+  const Rsrc rhalo = col_rsrc(w + rec_elems(g), rcol * hcol, hcol);
+  // the lanes on the physical -x / -y border and their halo entries
+  const bool hx_lane = FWD ? a == 0 : a == CB - 1;
+  const bool hy_lane = FWD ? b == 0 : b == CB - 1;
+  const int hx_entry = FWD ? b : CB - 1 - b, hy_entry = CB + (FWD ? a : CB - 1 - a);
   // element offset (in the resources) of logical z = 0 on this line (lines
   // outside the box are clamped into it), and the element steps of logical
   // +z / +x / +y
@@ -729,13 +754,19 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
   const T* const in_y = mbox + col_y * 15 * nz + int64_t(in_y_line) * nz;
   const int32_t ix_off = b * nzi, iy_off = in_y_line * nzi, o_off = out_line * nzi;
   T prev = T(0);
+  // carried from the previous step (forward): the own +x / +y / +z face
+  // weights, which the +x / +y lanes and the next step read as their -x /
+  // -y / -z faces
+  T wpx_prev = T(0), wpy_prev = T(0), wpz_prev = T(0);
   // A chunk's windows (steps t0 .. t0 + K - 1, logical z0 = t0 - a - b):
-  //   rec [K][8]   the cells' records
+  //   rec [K][4]   the cells' records, in step order
+  //   hx, hy [K]   -x / -y faces of the border lanes (halo)
   //   xw [K + 1]   old values of the own line, z0 .. z0 + K
   //   exx, exy [K] old values of the +x / +y line beyond the column
   //   bw [K]       right-hand side
   struct Win {
     T rec[K][kRecVals];
+    T hx[K], hy[K];
     T xw[K + 1], exx[K], exy[K], bw[K];
   };
   auto load_elements = [&](Win& v, int32_t z0) {
@@ -756,8 +787,8 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
   // the runs)
   auto load_windows = [&](Win& v, int64_t c) {
     const int64_t t0 = FWD ? kRecK * c - g.s : kRecK * c;
+    const int64_t fc = FWD ? c : g.nc - 1 - c;
     {
-      const int64_t fc = FWD ? c : g.nc - 1 - c;
       uint32_t d[NQ * 4];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
@@ -777,6 +808,11 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
           if constexpr (W == 1) v.rec[k][j] = __builtin_bit_cast(T, d[m]);
           else v.rec[k][j] = __builtin_bit_cast(T, uint64_t(d[m]) | (uint64_t(d[m + 1]) << 32));
         }
+    }
+    {
+      const int64_t hb = fc * kHaloEntries;
+      load_run<T, K, !FWD>(rhalo, hx_lane ? uint32_t((hb + hx_entry) * kRecK) * ST : kRunOff, v.hx);
+      load_run<T, K, !FWD>(rhalo, hy_lane ? uint32_t((hb + hy_entry) * kRecK) * ST : kRunOff, v.hy);
     }
     const int32_t z0 = int32_t(t0) - a - b;
     const int32_t zk = FWD ? zorg + z0 : zorg - z0 - (K - 1);  // K elements
@@ -799,10 +835,11 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
     }
     if (bad) load_elements(v, z0);
   };
-  // One chunk: its steps from `cur` while the windows two chunks ahead load
-  // into `fill` (two chunks in flight).  Three buffers rotate by name in the
-  // loop below, so no window is copied between registers.
-  auto chunk = [&](const Win& cur, Win& fill, int64_t c) {
+  // One chunk: its steps from `cur` (the backward sweep also reads the first
+  // record of `nxt`) while the windows two chunks ahead load into `fill`.
+  // Three buffers rotate by name in the loop below, so no window is copied
+  // between registers.
+  auto chunk = [&](const Win& cur, const Win& nxt, Win& fill, int64_t c) {
     const int64_t t0 = FWD ? kRecK * c - g.s : kRecK * c;
     const int32_t z0 = int32_t(t0) - a - b;
     const int32_t zb = FWD ? zorg + z0 : zorg - z0 - (K - 1);  // lowest result element
@@ -873,16 +910,34 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
       const T vmx = FWD ? n_px : o_sx, vpx = FWD ? o_sx : n_px;
       const T vmy = FWD ? n_py : o_sy, vpy = FWD ? o_sy : n_py;
       const T vmz = FWD ? n_pz : o_sz, vpz = FWD ? o_sz : n_pz;
+      // physical faces: +x / +y / +z and the diagonal from the own record;
+      // -x / -y from the lane holding that neighbour's record (forward: lane
+      // -8 / -1, one step ago; backward: lane +8 / +1, one step later) or the
+      // halo; -z from the own record one step ago (forward) / later
+      // (backward).  Records of cells outside the box are 0.
       const T* r = cur.rec[k];
+      const T wpx = r[0], wpy = r[1], wpz = r[2], d = r[3];
+      T wmx, wmy, wmz;
+      if constexpr (FWD) {
+        wmx = __shfl(wpx_prev, lane >= CB ? lane - CB : lane, 64);
+        wmy = shift<kWaveShr1>(wpy_prev, wpy_prev);
+        wmz = wpz_prev;
+      } else {
+        const T* rn = k + 1 < K ? cur.rec[k + 1 < K ? k + 1 : 0] : nxt.rec[0];
+        wmx = __shfl(rn[0], lane + CB < 64 ? lane + CB : lane, 64);
+        wmy = shift<kWaveShl1>(rn[1], rn[1]);
+        wmz = rn[2];
+      }
+      if (hx_lane) wmx = cur.hx[k];
+      if (hy_lane) wmy = cur.hy[k];
       const T xi = cur.xw[k];
       T acc = cur.bw[k];
-      acc = fma(x > 0 ? r[0] + adv : T(0), vmx, acc);
-      acc = fma(r[2], vmy, acc);
-      acc = fma(r[4], vmz, acc);
-      acc = fma(r[5], vpz, acc);
-      acc = fma(r[3], vpy, acc);
-      acc = fma(r[1], vpx, acc);
-      const T d = r[6];
+      acc = fma(x > 0 ? wmx + adv : T(0), vmx, acc);
+      acc = fma(wmy, vmy, acc);
+      acc = fma(wmz, vmz, acc);
+      acc = fma(wpz, vpz, acc);
+      acc = fma(wpy, vpy, acc);
+      acc = fma(wpx, vpx, acc);
       T xn;
       if (ULTRA) {
         acc = fma(-d, xi, acc);
@@ -895,6 +950,11 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
       xo[k] = xn;
       rstore<T, 16>(xn, rmo, valid && out_line >= 0 ? uint32_t(o_off + lz) * ST : kOff);
       prev = xn;
+      if constexpr (FWD) {
+        wpx_prev = wpx;
+        wpy_prev = wpy;
+        wpz_prev = wpz;
+      }
     }
     store_run<T, K, !FWD>(xo, rxo, full ? uint32_t(zb) * ST : kRunOff);
     if (line && !full) {
@@ -910,11 +970,11 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
   load_windows(w1, 1);
   const int64_t nc = g.nc;
   for (int64_t c = 0;; c += 3) {
-    chunk(w0, w2, c);
+    chunk(w0, w1, w2, c);
     if (c + 1 >= nc) break;
-    chunk(w1, w0, c + 1);
+    chunk(w1, w2, w0, c + 1);
     if (c + 2 >= nc) break;
-    chunk(w2, w1, c + 2);
+    chunk(w2, w0, w1, c + 2);
     if (c + 3 >= nc) break;
   }
 }
@@ -1006,9 +1066,10 @@ void launch_records(const StencilDesc& s, void* w, hipStream_t st) {
 // not fit 32-bit byte offsets (the sweeps then evaluate the stencil)
 size_t sor_weights_bytes(const StencilDesc& s, bool f32) {
   const RecGeom g = rec_geom(s);
-  const int64_t per_col = g.nc * kRecK * 64 * kRecVals * int64_t(f32 ? sizeof(float) : sizeof(double));
+  const int64_t es = int64_t(f32 ? sizeof(float) : sizeof(double));
+  const int64_t per_col = g.nc * kRecK * 64 * kRecVals * es;
   if (per_col >= (int64_t(1) << 31)) return 0;
-  return size_t(g.ncols * per_col);
+  return size_t((rec_elems(g) + halo_elems(g)) * es);
 }
 
 void stencil_sor_weights(const StencilDesc& s, bool f32, void* w, hipStream_t st) {
