@@ -28,7 +28,17 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the finest-level f32 SOR sweeps at 256^3: 2 x
 # FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc runs (the factor 2 is what a plain
 # copy kernel of known size gives), profiles/stencil_kernels_256_hbm_counters.md
-TRAFFIC_256 = {"stencil_sor_fwd": 722_531_635, "stencil_sor_bwd": 1_024_084_070}
+TRAFFIC_256 = {"stencil_sor_fwd": 812_460_032, "stencil_sor_bwd": 966_311_936}
+
+
+def sor_cache_bytes(n: int, esize: int) -> int:
+    """Bytes of the stencil SOR sweeps' coefficient cache for an n^3 box (each
+    sweep reads it once): per 8 x 8 column and chunk of 4 steps, a 4-value
+    record per lane and step plus 16 halo entries of 4 values."""
+    cx = (n + 7) // 8
+    nsteps = n + 14
+    nc = (nsteps + (4 - nsteps % 4) % 4) // 4
+    return cx * cx * nc * (4 * 64 * 4 + 16 * 4) * esize
 
 
 def log(msg):
@@ -259,13 +269,16 @@ def main():
 
     # The dominant kernel of the mixed solve: the finest-level SOR sweep in
     # f32 (a forward sweep from zero as pre-smoother, a backward sweep from the
-    # prolonged iterate as post-smoother).  Algorithmic bytes per launch: the
-    # f32 coefficient and right-hand side read, x_out written, plus x_in read
-    # by the backward sweep, each cell once.
+    # prolonged iterate as post-smoother).  Algorithmic bytes per launch (the
+    # matrix-free figure): the f32 coefficient and right-hand side read, x_out
+    # written, plus x_in read by the backward sweep, each cell once.  The
+    # sweep reads its face weights and diagonal from the operator's cache
+    # instead of the coefficients; kernel_bytes counts those.
     dom = max(("stencil_sor_fwd", "stencil_sor_bwd"), key=lambda k: classes[k]["ms_total_per_solve"])
     dom_bytes = (12 if dom == "stencil_sor_fwd" else 16) * N
     dom_ms = classes[dom]["ms_per_launch"]
     achieved = dom_bytes / dom_ms / 1e6
+    kernel_bytes = dom_bytes - 4 * N + sor_cache_bytes(n, 4)
     # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
     # the same kernels at 256^3 (python -m lineal_amd.profile_kernels), FETCH_SIZE
     # scaled by the factor a plain copy kernel of known size gives
@@ -311,11 +324,13 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": {"stencil_sor_fwd": "sor_wavefront<float, values, forward, regular, x_in = 0> "
+            "kernel": {"stencil_sor_fwd": "sor_wavefront_r<float, forward, regular, x_in = 0> "
                                           "(finest-level pre-smoother)",
-                       "stencil_sor_bwd": "sor_wavefront<float, values, backward, regular, x_in> "
+                       "stencil_sor_bwd": "sor_wavefront_r<float, backward, regular, x_in> "
                                           "(finest-level post-smoother)"}[dom],
             "algorithmic_bytes": dom_bytes,
+            "kernel_bytes": kernel_bytes,
+            "kernel_bytes_frac": kernel_bytes / dom_ms / 1e6 / HBM_PEAK_GBS,
             "ms_per_launch": dom_ms,
         },
     }
