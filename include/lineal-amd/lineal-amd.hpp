@@ -234,6 +234,9 @@ class GpuStencilMatrixBase : public GpuMatrixBase {
     return b;
   }
   [[nodiscard]] const BoxInfo& info() const { return box_; }
+  // SOR sweeps over cached per-cell face weights and diagonal (the default)
+  // or with them evaluated on the fly (false frees the caches); same bits
+  void set_sor_cache(bool enable) { check(lgpu_op_set_sor_cache(h_.get(), enable ? 1 : 0)); }
 
  protected:
   GpuStencilMatrixBase(const GpuContext& ctx, lgpu_op h, BoxInfo box)

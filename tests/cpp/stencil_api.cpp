@@ -197,6 +197,27 @@ int main(int argc, char** argv) {
       (void)r;
     }
 
+    // the SOR sweeps' coefficient cache changes no bit
+    {
+      const auto rhs = A.rhs();
+      lg::GpuSsorSolver<double> ssor(1.0);
+      const auto si = ssor.instantiate(A);
+      lg::GpuDenseVector<double> y0(n, 0.0, ctx), y1(n, 0.0, ctx);
+      for (int k = 0; k < 5; ++k) si.apply(&y0, y0, rhs);
+      A.set_sor_cache(false);
+      for (int k = 0; k < 5; ++k) si.apply(&y1, y1, rhs);
+      A.set_sor_cache(true);
+      const auto h0 = y0.to_host(), h1 = y1.to_host();
+      double diff = 0.0;
+      bool same = true;
+      for (size_t i = 0; i < n; ++i)
+        if (h0[i] != h1[i]) {
+          same = false;
+          diff = std::fmax(diff, std::fabs(h0[i] - h1[i]));
+        }
+      expect(same, "ssor with and without the sor coefficient cache", diff);
+    }
+
     // CSR copy of the stencil
     {
       lg::GpuCsrMatrix<double> C(A);

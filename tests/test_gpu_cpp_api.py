@@ -23,4 +23,4 @@ def test_cpp_api_program():
     res = subprocess.run([EXE, golden], capture_output=True, text=True, timeout=120)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
-    assert res.stdout.count("ok  ") == 13
+    assert res.stdout.count("ok  ") == 14
