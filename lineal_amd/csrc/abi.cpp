@@ -202,6 +202,7 @@ struct lgpu_op_s {
   std::vector<uint8_t> host_mat;
   DevBuf sor_state, sor_order, sor_mailbox;
   int64_t sor_cols = 0;
+  SorBoxState sor_box;
   // csr
   CsrDev csr;
   setup::RowMatrix host_csr;
@@ -314,7 +315,19 @@ void init_sor_sched(lgpu_op op) {
   const size_t ints = sor_sched_ints(op->desc);
   op->sor_state.alloc(ints * sizeof(int));
   LGPU_HIP_CHECK(hipMemset(op->sor_state.ptr, 0, ints * sizeof(int)));
-  op->sor_mailbox.alloc(sor_mailbox_bytes(op->desc));
+  op->sor_mailbox.alloc(2 * sor_mailbox_bytes(op->desc));
+  op->sor_box = SorBoxState{};
+}
+
+SorSched sor_sched_of(lgpu_op op) {
+  SorSched s;
+  s.state = op->sor_state.as<int>();
+  s.err = op->ctx->err();
+  s.order = op->sor_order.as<uint32_t>();
+  s.ncols = op->sor_cols;
+  s.mailbox = op->sor_mailbox.ptr;
+  s.box = &op->sor_box;
+  return s;
 }
 
 // Host evaluation of the stencil rows in f64 in iterate order (-x,-y,-z,+z,+y,+x,
@@ -425,12 +438,7 @@ void op_sor(lgpu_op op, bool f32, const void* x_in, void* x_out, const void* b, 
     csr_sweep(op->ctx, op->csr, x_in, x_out, b, omega, fwd, ultra);
   } else {
     if (f32) op->ensure_f32();
-    SorSched s;
-    s.state = op->sor_state.as<int>();
-    s.err = op->ctx->err();
-    s.order = op->sor_order.as<uint32_t>();
-    s.ncols = op->sor_cols;
-    s.mailbox = op->sor_mailbox.ptr;
+    const SorSched s = sor_sched_of(op);
     const void* w = op->sor_weights(f32, true);
     KTimed kt(op->ctx, fwd ? LGPU_KT_STENCIL_SOR_FWD : LGPU_KT_STENCIL_SOR_BWD);
     stencil_sor(op->desc, f32, x_in, x_out, b, omega, fwd, ultra, s, st, w);
@@ -597,12 +605,7 @@ void ssor_sweep_into(lgpu_op op, bool f32, const void* x_in, void* x_out, const 
     if (op->is_csr) {
       csr_sweep(op->ctx, op->csr, in, out, b, omega, fwd != 0, ultra);
     } else {
-      SorSched s;
-      s.state = op->sor_state.as<int>();
-      s.err = op->ctx->err();
-      s.order = op->sor_order.as<uint32_t>();
-      s.ncols = op->sor_cols;
-      s.mailbox = op->sor_mailbox.ptr;
+      const SorSched s = sor_sched_of(op);
       // the cached weights when they exist (building them would allocate)
       stencil_sor(op->desc, f32, in, out, b, omega, fwd != 0, ultra, s, st,
                   op->sor_weights(f32, false));

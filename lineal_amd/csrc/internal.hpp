@@ -147,12 +147,20 @@ void stencil_apply(const StencilDesc& s, int mode, bool f32, const void* x, cons
 void stencil_rhs(const StencilDesc& s, bool f32, void* b, hipStream_t st);
 
 // lexicographic SOR sweep on the stencil (tile-column wavefront)
+// host-side state of a stencil operator's two mailbox sets (with their
+// tickets): the set the next sweep uses, and the word size (bytes) each set is
+// cleared for: 0 not cleared, 4 f32 words, 8 f64 words (which covers f32)
+struct SorBoxState {
+  int next = 0;
+  int clean[2] = {0, 0};
+};
 struct SorSched {
-  int* state = nullptr;        // [0] ticket
+  int* state = nullptr;        // [0], [1] the tickets of the two mailbox sets
   int* err = nullptr;          // sticky wait-error word (the context's)
   const uint32_t* order = nullptr;  // ticket -> column id (forward topological order)
   int64_t ncols = 0;
-  void* mailbox = nullptr;     // border-line values passed between columns
+  void* mailbox = nullptr;     // two sets of sor_mailbox_bytes: border-line values passed between columns
+  SorBoxState* box = nullptr;  // which set is next and what is cleared
 };
 size_t sor_sched_ints(const StencilDesc& s);
 size_t sor_mailbox_bytes(const StencilDesc& s);

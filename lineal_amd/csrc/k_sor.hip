@@ -47,18 +47,21 @@ constexpr int SPIN_LIMIT = 1 << 24;
 // Mailbox words start as a NaN with a payload that no arithmetic produces
 // (arithmetic NaNs are the canonical quiet NaN or carry an input's payload);
 // a word is published with one single-copy-atomic store, so a reader that
-// sees anything else sees the final value and needs no fence.
+// sees anything else sees the final value and needs no fence.  The f64
+// sentinel is the f32 one twice, so a set cleared in either precision reads
+// as cleared in f32.
+constexpr uint32_t kSentinel32 = 0xFFFABADBu;
 template <typename T>
 struct Sentinel;
 template <>
 struct Sentinel<float> {
   using Bits = uint32_t;
-  static constexpr Bits bits = 0x7FBADBADu;
+  static constexpr Bits bits = kSentinel32;
 };
 template <>
 struct Sentinel<double> {
   using Bits = uint64_t;
-  static constexpr Bits bits = 0x7FF8BADBADBADBADull;
+  static constexpr Bits bits = (uint64_t(kSentinel32) << 32) | kSentinel32;
 };
 
 template <typename T>
@@ -85,13 +88,13 @@ __device__ __forceinline__ T mailbox_poll(const T* p, int* err) {
   return v;
 }
 
-template <typename T>
-__global__ void mailbox_reset(T* box, int64_t n) {
-  using Bits = typename Sentinel<T>::Bits;
-  Bits* p = reinterpret_cast<Bits*>(box);
+// the whole of one mailbox set (n dwords) and its ticket, when no sweep has
+// cleared them for the coming one
+__global__ void mailbox_reset(uint32_t* box, int64_t n, int* ticket) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
-    p[i] = Sentinel<T>::bits;
+    box[i] = kSentinel32;
 }
 
 template <typename R>
@@ -128,6 +131,34 @@ __device__ __forceinline__ void rstore(T v, Rsrc r, uint32_t off) {
     using V2 = unsigned int __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0u, POL);
   }
+}
+
+// Two mailbox sets (and two tickets) alternate between sweeps: a sweep uses
+// one and leaves the other cleared for the next, so no reset kernel runs
+// between sweeps.  The wave that draws ticket 0 zeroes the other ticket; every
+// column wave, once its steps are done, clears its 16 nz words of the other
+// set (any column layout of either sweep kernel lies in [0, 16 nz ncols)
+// words).  Nothing reads the other set during this sweep, and the next sweep
+// starts after this one ends.
+__device__ __forceinline__ int claim_column(int* ticket, int* ticket_next) {
+  __shared__ int s_ticket;
+  if (threadIdx.x == 0) {
+    const int t = atomicAdd(ticket, 1);
+    if (t == 0) *ticket_next = 0;
+    s_ticket = t;
+  }
+  __syncthreads();
+  return s_ticket;
+}
+template <typename T>
+__device__ __forceinline__ void clear_next_box(T* next, int64_t col, int64_t nz, int lane) {
+  using V4 = unsigned int __attribute__((ext_vector_type(4)));
+  // 16 nz words of T are nz * sizeof(T) 16-byte units
+  const int32_t units = int32_t(nz * int64_t(sizeof(T)));
+  const Rsrc r = col_rsrc(next, col * 16 * nz, 16 * nz);
+  const V4 v = {kSentinel32, kSentinel32, kSentinel32, kSentinel32};
+  for (int32_t i = lane; i < units; i += 64)
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, uint32_t(i) * 16u, 0u, 0);
 }
 
 // Runs of consecutive elements with 16-byte loads and stores: a lane's K-step
@@ -223,18 +254,16 @@ template <typename T, int KIND, bool FWD, bool ULTRA, bool HAS_X, bool FD>
 __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __restrict__ xin,
                                                     T* __restrict__ xout,
                                                     const T* __restrict__ bv, T omega,
-                                                    int* state, int* err,
-                                                    const uint32_t* order, int64_t cy, T* mbox) {
+                                                    int* ticket, int* ticket_next, int* err,
+                                                    const uint32_t* order, int64_t cy, T* mbox,
+                                                    T* mbox_next) {
   using CI = CellInfo<T, KIND>;
   using Raw = typename CI::Raw;
   constexpr int K = kChunk<T>;
   constexpr uint32_t SR = sizeof(Raw), ST = sizeof(T);
   const int lane = threadIdx.x;
   const int a = lane / CB, b = lane % CB;
-  __shared__ int s_ticket;
-  if (lane == 0) s_ticket = atomicAdd(state, 1);
-  __syncthreads();
-  const int64_t col = order[s_ticket];
+  const int64_t col = order[claim_column(ticket, ticket_next)];
   const int64_t I = col / cy, J = col % cy;
   const int64_t nx = s.nx, ny = s.ny, nz = s.nz, nyz = ny * nz;
   const int64_t lx = I * CB + a, ly = J * CB + b;  // logical line coordinates
@@ -548,6 +577,7 @@ __global__ __launch_bounds__(64) void sor_wavefront(StencilDesc s, const T* __re
     }
     if constexpr (RUNS) store_run<T, K, !FWD>(xo, rxo, line && run_store ? uint32_t(zb) * ST : kRunOff);
   }
+  clear_next_box(mbox_next, col, nz, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -678,19 +708,16 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
                                                       const T* __restrict__ xin,
                                                       T* __restrict__ xout,
                                                       const T* __restrict__ bv, T omega,
-                                                      int* state, int* err,
-                                                      const uint32_t* order, T* mbox) {
+                                                      int* ticket, int* ticket_next, int* err,
+                                                      const uint32_t* order, T* mbox, T* mbox_next) {
   constexpr int K = kRecK;
   constexpr int NQ = kRecSlices<T>;
   constexpr int W = int(sizeof(T) / 4);  // dwords per value
   constexpr uint32_t ST = sizeof(T);
   const int lane = threadIdx.x;
   const int a = lane / CB, b = lane % CB;
-  __shared__ int s_ticket;
-  if (lane == 0) s_ticket = atomicAdd(state, 1);
-  __syncthreads();
   const int64_t cy = g.cy;
-  const int64_t col = order[s_ticket];
+  const int64_t col = order[claim_column(ticket, ticket_next)];
   const int64_t I = col / cy, J = col % cy;
   const int64_t nx = s.nx, ny = s.ny, nz = s.nz, nyz = ny * nz;
   // logical line coordinates (the backward grid starts rx / ry lines before
@@ -977,30 +1004,42 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
     chunk(w2, w0, w1, c + 2);
     if (c + 3 >= nc) break;
   }
+  clear_next_box(mbox_next, col, nz, lane);
 }
+
+// one sweep's tickets and mailbox sets: its own and the other (cleared by it)
+struct SweepBoxes {
+  int* ticket;
+  int* ticket_next;
+  int* err;
+  const uint32_t* order;
+  int64_t ncols;
+  void* mailbox;
+  void* mailbox_next;
+};
 
 template <typename T, int KIND, bool FWD, bool ULTRA, bool FD>
 void launch5(const StencilDesc& s, const void* x_in, void* x_out, const void* b, double omega,
-             const SorSched& sched, hipStream_t st) {
+             const SweepBoxes& sched, hipStream_t st) {
   const int64_t cy = (s.ny + CB - 1) / CB;
   const dim3 grid(unsigned(sched.ncols)), block(64);
   if (x_in) {
     hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, true, FD>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched.state, sched.err, sched.order, cy,
-                       static_cast<T*>(sched.mailbox));
+                       static_cast<const T*>(b), T(omega), sched.ticket, sched.ticket_next, sched.err,
+                       sched.order, cy, static_cast<T*>(sched.mailbox), static_cast<T*>(sched.mailbox_next));
   } else {
     hipLaunchKernelGGL((sor_wavefront<T, KIND, FWD, ULTRA, false, FD>), grid, block, 0, st, s,
                        static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched.state, sched.err, sched.order, cy,
-                       static_cast<T*>(sched.mailbox));
+                       static_cast<const T*>(b), T(omega), sched.ticket, sched.ticket_next, sched.err,
+                       sched.order, cy, static_cast<T*>(sched.mailbox), static_cast<T*>(sched.mailbox_next));
   }
   LGPU_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T, int KIND, bool FWD, bool ULTRA>
 void launch4(const StencilDesc& s, const void* x_in, void* x_out, const void* b, double omega,
-             const SorSched& sched, hipStream_t st) {
+             const SweepBoxes& sched, hipStream_t st) {
   const bool fd = KIND == 0 && (sizeof(T) == 8 ? s.fast_div64 : s.fast_div32);
   if (fd) launch5<T, KIND, FWD, ULTRA, true>(s, x_in, x_out, b, omega, sched, st);
   else launch5<T, KIND, FWD, ULTRA, false>(s, x_in, x_out, b, omega, sched, st);
@@ -1008,7 +1047,7 @@ void launch4(const StencilDesc& s, const void* x_in, void* x_out, const void* b,
 
 template <typename T, int KIND>
 void launch2(const StencilDesc& s, const void* x_in, void* x_out, const void* b, double omega,
-             bool fwd, bool ultra, const SorSched& sched, hipStream_t st) {
+             bool fwd, bool ultra, const SweepBoxes& sched, hipStream_t st) {
   if (fwd) {
     if (ultra) launch4<T, KIND, true, true>(s, x_in, x_out, b, omega, sched, st);
     else launch4<T, KIND, true, false>(s, x_in, x_out, b, omega, sched, st);
@@ -1020,26 +1059,26 @@ void launch2(const StencilDesc& s, const void* x_in, void* x_out, const void* b,
 
 template <typename T, bool FWD, bool ULTRA>
 void launch_r(const StencilDesc& s, const void* w, const void* x_in, void* x_out, const void* b,
-              double omega, const SorSched& sched, hipStream_t st) {
+              double omega, const SweepBoxes& sched, hipStream_t st) {
   const RecGeom g = rec_geom(s);
   const dim3 grid(unsigned(sched.ncols)), block(64);
   if (x_in) {
     hipLaunchKernelGGL((sor_wavefront_r<T, FWD, ULTRA, true>), grid, block, 0, st, s, g,
                        static_cast<const T*>(w), static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched.state, sched.err, sched.order,
-                       static_cast<T*>(sched.mailbox));
+                       static_cast<const T*>(b), T(omega), sched.ticket, sched.ticket_next, sched.err,
+                       sched.order, static_cast<T*>(sched.mailbox), static_cast<T*>(sched.mailbox_next));
   } else {
     hipLaunchKernelGGL((sor_wavefront_r<T, FWD, ULTRA, false>), grid, block, 0, st, s, g,
                        static_cast<const T*>(w), static_cast<const T*>(x_in), static_cast<T*>(x_out),
-                       static_cast<const T*>(b), T(omega), sched.state, sched.err, sched.order,
-                       static_cast<T*>(sched.mailbox));
+                       static_cast<const T*>(b), T(omega), sched.ticket, sched.ticket_next, sched.err,
+                       sched.order, static_cast<T*>(sched.mailbox), static_cast<T*>(sched.mailbox_next));
   }
   LGPU_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
 void launch_r2(const StencilDesc& s, const void* w, const void* x_in, void* x_out, const void* b,
-               double omega, bool fwd, bool ultra, const SorSched& sched, hipStream_t st) {
+               double omega, bool fwd, bool ultra, const SweepBoxes& sched, hipStream_t st) {
   if (fwd) {
     if (ultra) launch_r<T, true, true>(s, w, x_in, x_out, b, omega, sched, st);
     else launch_r<T, true, false>(s, w, x_in, x_out, b, omega, sched, st);
@@ -1084,8 +1123,8 @@ void stencil_sor_weights(const StencilDesc& s, bool f32, void* w, hipStream_t st
 
 size_t sor_sched_ints(const StencilDesc&) { return 2; }
 
-// two mailbox lines (the a = 7 and b = 7 border lines) of 8 z-lines per column,
-// sized for f64 words
+// one mailbox set: 16 lines of nz words per column (the kernels use 15 or 16),
+// sized for f64 words; an operator holds two sets
 size_t sor_mailbox_bytes(const StencilDesc& s) {
   const int64_t cx = (s.nx + CB - 1) / CB, cy = (s.ny + CB - 1) / CB;
   return size_t(2 * cx * cy * CB * s.nz) * sizeof(double);
@@ -1110,25 +1149,37 @@ void stencil_sor(const StencilDesc& s, bool f32, const void* x_in, void* x_out, 
   // a column's ten planes are addressed with 32-bit byte offsets below 2^31
   LGPU_REQUIRE(10 * s.ny * s.nz * int64_t(f32 ? 4 : 8) < (int64_t(1) << 31),
                "SOR sweep: the y-z plane is too large for 32-bit column offsets");
-  // reset the ticket; the error word (sched.err) is sticky until the host
-  // clears it
-  vec_fill_int(sched.state, 1, 0, st);
-  // one word per border-line cell, of the sweep's precision
-  const int64_t words = int64_t(sor_mailbox_bytes(s) / sizeof(double));
-  const dim3 rg(unsigned(std::min<int64_t>((words + 255) / 256, 4096))), rb(256);
-  if (f32) hipLaunchKernelGGL(mailbox_reset<float>, rg, rb, 0, st, static_cast<float*>(sched.mailbox), words);
-  else hipLaunchKernelGGL(mailbox_reset<double>, rg, rb, 0, st, static_cast<double*>(sched.mailbox), words);
-  LGPU_HIP_CHECK(hipGetLastError());
-  if (weights) {
-    if (f32) launch_r2<float>(s, weights, x_in, x_out, b, omega, forward, ultra, sched, st);
-    else launch_r2<double>(s, weights, x_in, x_out, b, omega, forward, ultra, sched, st);
-  } else if (f32) {
-    if (s.kind == 0) launch2<float, 0>(s, x_in, x_out, b, omega, forward, ultra, sched, st);
-    else launch2<float, 1>(s, x_in, x_out, b, omega, forward, ultra, sched, st);
-  } else {
-    if (s.kind == 0) launch2<double, 0>(s, x_in, x_out, b, omega, forward, ultra, sched, st);
-    else launch2<double, 1>(s, x_in, x_out, b, omega, forward, ultra, sched, st);
+  LGPU_REQUIRE(sched.box != nullptr, "SOR sweep: no mailbox state");
+  SorBoxState& bs = *sched.box;
+  const int set = bs.next;
+  const int words = f32 ? 4 : 8;
+  const size_t set_bytes = sor_mailbox_bytes(s);
+  char* const box0 = static_cast<char*>(sched.mailbox);
+  SweepBoxes sb{sched.state + set, sched.state + (1 - set), sched.err, sched.order, sched.ncols,
+                box0 + size_t(set) * set_bytes, box0 + size_t(1 - set) * set_bytes};
+  // the set is cleared for this precision by the previous sweep, or here (the
+  // error word, sched.err, is sticky until the host clears it)
+  if (bs.clean[set] < words) {
+    const int64_t dwords = int64_t(set_bytes / sizeof(uint32_t));
+    const dim3 rg(unsigned(std::min<int64_t>((dwords + 255) / 256, 4096))), rb(256);
+    hipLaunchKernelGGL(mailbox_reset, rg, rb, 0, st, static_cast<uint32_t*>(sb.mailbox), dwords,
+                       sb.ticket);
+    LGPU_HIP_CHECK(hipGetLastError());
   }
+  bs.clean[0] = bs.clean[1] = 0;
+  if (weights) {
+    if (f32) launch_r2<float>(s, weights, x_in, x_out, b, omega, forward, ultra, sb, st);
+    else launch_r2<double>(s, weights, x_in, x_out, b, omega, forward, ultra, sb, st);
+  } else if (f32) {
+    if (s.kind == 0) launch2<float, 0>(s, x_in, x_out, b, omega, forward, ultra, sb, st);
+    else launch2<float, 1>(s, x_in, x_out, b, omega, forward, ultra, sb, st);
+  } else {
+    if (s.kind == 0) launch2<double, 0>(s, x_in, x_out, b, omega, forward, ultra, sb, st);
+    else launch2<double, 1>(s, x_in, x_out, b, omega, forward, ultra, sb, st);
+  }
+  // the sweep leaves the other set and its ticket cleared in its word size
+  bs.clean[1 - set] = words;
+  bs.next = 1 - set;
 }
 
 }  // namespace lgpu
