@@ -763,13 +763,21 @@ class GpuMultiGridSolver {
     p.cycle_kind = int(cycle_.kind);
     p.cycle_iterations = cycle_.iterations;
     p.setup_on_device = params_.setup_on_device ? 1 : 0;
+    if (sequential_rows_ >= 0) p.sequential_rows = sequential_rows_;
     return p;
+  }
+  // coarse levels with at most `rows` rows are swept in one segment (0:
+  // none; the library's default otherwise)
+  GpuMultiGridSolver& sequential_rows(int64_t rows) {
+    sequential_rows_ = rows;
+    return *this;
   }
 
  private:
   HierarchyWrightParams params_;
   SorVariant variant_;
   int64_t coarse_segment_rows_;
+  int64_t sequential_rows_ = -1;
   double relax_ = 1.0;
   DefaultCycle cycle_{};
 };

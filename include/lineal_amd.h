@@ -184,8 +184,8 @@ int lgpu_op_ssor(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double o
 /* One SOR sweep of a CSR operator with the reference's thread-segment
  * semantics (gauss-seidel-variant/sweep.hpp:153-169; the segments of
  * parallel/execution-policy/adaptive.hpp:266-279): rows are cut into segments
- * of segment_rows (1..64) consecutive rows, each swept in order with new values
- * only from its own segment. */
+ * of segment_rows (1..8192) consecutive rows, each swept in order with new
+ * values only from its own segment. */
 int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, lgpu_vec b, double omega,
                           int forward, int ultra, int64_t segment_rows);
 
@@ -223,7 +223,7 @@ typedef struct {
                                    smoothers and level operators, FP64 coarsest solve */
   int ultra;                    /* SOR variant of the smoothers */
   int64_t coarse_segment_rows;  /* coarse-level (CSR) smoothers: rows per thread segment
-                                   (1..64, default 64); 0 sweeps each coarse level in one
+                                   (default 64); 0 sweeps each coarse level in one
                                    segment (lexicographic). The finest level is always
                                    swept lexicographically. */
   /* MultiGridSolver{relax, DefaultCycle{cycle_kind, cycle_iterations}, wright}
@@ -239,6 +239,12 @@ typedef struct {
    * greedy aggregation runs on the host either way, and the hierarchy is the
    * same bit for bit */
   int setup_on_device;
+  /* coarse levels with at most this many rows are swept in one segment
+   * (lexicographically, as the reference with one thread) whatever
+   * coarse_segment_rows says; 0 (default): none.  Fewer CG iterations
+   * (256^3 lognormal: 98 -> 94 with 1024), but on the GPU such a sweep runs
+   * one dependency level at a time and costs more than the iterations save */
+  int64_t sequential_rows;
 } lgpu_amg_params;
 enum { LGPU_CYCLE_REGULAR = 0, LGPU_CYCLE_DECREASING = 1 };
 

@@ -85,10 +85,14 @@ class _AmgParams(C.Structure):
         ("cycle_kind", C.c_int),
         ("cycle_iterations", C.c_int),
         ("setup_on_device", C.c_int),
+        ("sequential_rows", C.c_int64),
     ]
 
 
 CYCLE_REGULAR, CYCLE_DECREASING = 0, 1
+# lgpu_amg_params.sequential_rows by default: coarse levels of at most this
+# many rows are swept in one segment (0: none)
+SEQUENTIAL_ROWS = 0
 
 
 class _AmgLevel(C.Structure):
@@ -665,12 +669,13 @@ class HierarchyWrightParams:
     setup_on_device: bool = True
 
     def _struct(self, precision: int, ultra: bool, coarse_segment_rows: int, relax=1.0,
-                cycle=(CYCLE_REGULAR, 1)):
+                cycle=(CYCLE_REGULAR, 1), sequential_rows=None):
+        seq = SEQUENTIAL_ROWS if sequential_rows is None else sequential_rows
         return _AmgParams(self.max_level_num, self.min_coarsen_size, self.min_coarsen_factor,
                           self.strong_threshold, self.aggressive_strong_threshold,
                           self.min_aggregate_size, self.max_aggregate_size, precision, int(ultra),
                           int(coarse_segment_rows), float(relax), int(cycle[0]), int(cycle[1]),
-                          int(self.setup_on_device))
+                          int(self.setup_on_device), int(seq))
 
 
 class MultiGridSolver:
@@ -681,16 +686,19 @@ class MultiGridSolver:
     smoothers and level operators with an FP64 coarsest solve.
     coarse_segment_rows: rows per thread segment of the coarse-level smoothers
     (0: one lexicographic segment per level).
+    sequential_rows: coarse levels with at most this many rows are swept in one
+    segment whatever coarse_segment_rows says (0: none).
     relax: the coarse correction is added scaled by 1/relax.
     cycle: (CYCLE_REGULAR or CYCLE_DECREASING, iterations); (REGULAR, 1) is the V-cycle.
     """
 
     def __init__(self, wright: HierarchyWrightParams | None = None, precision=np.float64, ultra=False,
-                 coarse_segment_rows=64, relax=1.0, cycle=(CYCLE_REGULAR, 1)):
+                 coarse_segment_rows=64, relax=1.0, cycle=(CYCLE_REGULAR, 1), sequential_rows=None):
         self.wright = wright or HierarchyWrightParams()
         self.precision = precision
         self.ultra = ultra
         self.coarse_segment_rows = coarse_segment_rows
+        self.sequential_rows = SEQUENTIAL_ROWS if sequential_rows is None else int(sequential_rows)
         self.relax = relax
         self.cycle = tuple(cycle)
 
@@ -716,7 +724,7 @@ class MultiGridInstance:
     def __init__(self, lhs: _Matrix, solver: MultiGridSolver, levels: list[AmgLevel] | None = None):
         self.lhs = lhs
         p = solver.wright._struct(_dtype_code(solver.precision), solver.ultra, solver.coarse_segment_rows,
-                                  solver.relax, solver.cycle)
+                                  solver.relax, solver.cycle, solver.sequential_rows)
         h = _vp()
         if levels is None:
             _check(lib().lgpu_amg_create(lhs.h, C.byref(p), C.byref(h)))
@@ -772,6 +780,12 @@ class MultiGridInstance:
         n = C.c_int64()
         _check(lib().lgpu_amg_level_size(self.h, level, C.byref(n)))
         return n.value
+
+    def level_nnz(self, level: int) -> int:
+        """Stored entries of level >= 1's matrix."""
+        nnz = C.c_int64()
+        _check(lib().lgpu_amg_matrix(self.h, level, C.byref(nnz), None, None, None))
+        return nnz.value
 
     def aggregates(self, level: int) -> np.ndarray:
         out = np.empty(self.level_size(level), np.uint32)

@@ -137,9 +137,26 @@ struct CsrDev {
   int epoch = 0;
   int64_t seg_entries = 0, blk_entries = 0;
   int seg_new_max = 0, seg_trail_max = 0;
+  // the scheduled segment sweeps' positions (build_schedule)
+  int64_t sched_seg = 0;
+  int sched_max_len = 0;
+  DevBuf sched_fwd, sched_bwd;
+  int64_t nnz = 0;
   CsrDesc desc() const {
-    return CsrDesc{n,          rowptr.as<int64_t>(), cols.as<int32_t>(), vals.ptr, f32 ? 1 : 0,
-                   seg_entries, blk_entries,          seg_new_max,       seg_trail_max};
+    return CsrDesc{n,           rowptr.as<int64_t>(), cols.as<int32_t>(), vals.ptr, f32 ? 1 : 0,
+                   seg_entries, blk_entries,          seg_new_max,       seg_trail_max,
+                   sched_seg,   sched_fwd.as<int2>(), sched_bwd.as<int2>(), sched_max_len,
+                   nnz};
+  }
+  // positions of the scheduled sweeps with segments of `seg` rows
+  void build_schedule(const setup::RowMatrix& m, int64_t seg) {
+    std::vector<int2> fwd, bwd;
+    sched_max_len = sched_positions(m.n, m.rowptr.data(), m.cols.data(), seg, fwd, bwd);
+    sched_fwd.alloc(fwd.size() * sizeof(int2));
+    sched_bwd.alloc(bwd.size() * sizeof(int2));
+    LGPU_HIP_CHECK(hipMemcpy(sched_fwd.ptr, fwd.data(), fwd.size() * sizeof(int2), hipMemcpyHostToDevice));
+    LGPU_HIP_CHECK(hipMemcpy(sched_bwd.ptr, bwd.data(), bwd.size() * sizeof(int2), hipMemcpyHostToDevice));
+    sched_seg = seg;
   }
   void upload(const setup::RowMatrix& m, bool as_f32) {
     n = m.n;
@@ -173,6 +190,7 @@ struct CsrDev {
     rowptr.alloc(size_t(n + 1) * sizeof(int64_t));
     LGPU_HIP_CHECK(hipMemcpy(rowptr.ptr, m.rowptr.data(), size_t(n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     const size_t nnz = m.cols.size();
+    this->nnz = int64_t(nnz);
     cols.alloc(nnz * sizeof(int32_t));
     LGPU_HIP_CHECK(hipMemcpy(cols.ptr, m.cols.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice));
     if (as_f32) {
@@ -243,6 +261,7 @@ struct lgpu_op_s {
 
 struct AmgLevel {
   CsrDev a;            // level operator (levels >= 1)
+  int64_t seg = 0;     // rows per smoother segment (levels >= 1; 0: one segment)
   DevBuf agg;          // this level -> next level map
   DevBuf c2f_ptr, c2f_idx;
   int64_t c2f_blk = 0;  // most list entries of any 256 consecutive coarse rows
@@ -499,14 +518,17 @@ void dot_into(lgpu_ctx ctx, const void* a, const void* b, bool f32, int64_t n, d
   reduce_partials(part, np, dst, ctx->stream);
 }
 
-// smoother of a coarse level: thread segments of coarse_segment_rows rows, or
-// one lexicographic segment
-void coarse_sweep(lgpu_amg amg, CsrDev& a, const void* x_in, void* x_out, const void* b,
-                  double omega, bool fwd, bool ultra) {
-  const int64_t seg = amg->params.coarse_segment_rows;
+// smoother of a coarse level: thread segments of the level's segment length
+// (level_segment_rows), 0 meaning one lexicographic segment
+void coarse_sweep(lgpu_amg amg, CsrDev& a, int64_t seg, const void* x_in, void* x_out,
+                  const void* b, double omega, bool fwd, bool ultra) {
   KTimed kt(amg->ctx, LGPU_KT_CSR_SOR);
-  if (seg > 0) csr_sor_segmented(a.desc(), x_in, x_out, b, omega, fwd, ultra, int(seg), amg->ctx->stream);
-  else csr_sweep(amg->ctx, a, x_in, x_out, b, omega, fwd, ultra);
+  if (a.sched_seg > 0)
+    csr_sor_scheduled(a.desc(), x_in, x_out, b, omega, fwd, ultra, amg->ctx->stream);
+  else if (seg > 0)
+    csr_sor_segmented(a.desc(), x_in, x_out, b, omega, fwd, ultra, int(seg), amg->ctx->stream);
+  else
+    csr_sweep(amg->ctx, a, x_in, x_out, b, omega, fwd, ultra);
 }
 
 // One multigrid cycle in the hierarchy's precision from zero: z = M r
@@ -537,7 +559,7 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine, const void* x0 =
       op_sor(fine, f32, x0, xs(0), bs(0), omega, true, ultra);
       op_apply(fine, SM_RESID, f32, xs(0), bs(0), cur.tmp.ptr, 0.0, nullptr);
     } else {
-      coarse_sweep(amg, cur.a, nullptr, xs(l), bs(l), omega, true, ultra);
+      coarse_sweep(amg, cur.a, cur.seg, nullptr, xs(l), bs(l), omega, true, ultra);
       KTimed kt(amg->ctx, LGPU_KT_CSR_PASS);
       csr_apply(cur.a.desc(), SM_RESID, xs(l), bs(l), cur.tmp.ptr, 0.0, nullptr, st);
     }
@@ -550,7 +572,7 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine, const void* x0 =
     prolong_add(xs(l), amg->lv[size_t(l + 1)].sol.ptr, f32, cur.agg.as<uint32_t>(), cur.n,
                 inv_relax, cur.aux.ptr, st);
     if (l == 0) op_sor(fine, f32, cur.aux.ptr, xs(0), bs(0), omega, false, ultra);
-    else coarse_sweep(amg, cur.a, cur.aux.ptr, xs(l), bs(l), omega, false, ultra);
+    else coarse_sweep(amg, cur.a, cur.seg, cur.aux.ptr, xs(l), bs(l), omega, false, ultra);
   };
   auto coarsest = [&] {
     AmgLevel& c = amg->lv[size_t(L - 1)];
@@ -1207,7 +1229,7 @@ extern "C" int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, 
                                      double omega, int forward, int ultra, int64_t segment_rows) {
   API_BEGIN
   LGPU_REQUIRE(op && op->is_csr, "segmented SOR needs a CSR operator");
-  LGPU_REQUIRE(segment_rows >= 1 && segment_rows <= kMaxSegmentRows, "segment_rows must lie in 1..64");
+  LGPU_REQUIRE(segment_rows >= 1 && segment_rows <= kSchedMaxRows, "segment_rows must lie in 1..8192");
   check_vec(x_out, op->rows(), "x_out");
   check_vec(b, op->rows(), "b");
   if (x_in) {
@@ -1215,8 +1237,16 @@ extern "C" int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, 
     LGPU_REQUIRE(x_in->f32 == x_out->f32 && x_in->ptr != x_out->ptr, "x_in must match x_out and not alias it");
   }
   LGPU_REQUIRE(b->f32 == x_out->f32 && op->csr.f32 == x_out->f32, "vectors and values must share a dtype");
-  csr_sor_segmented(op->csr.desc(), x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega, forward != 0,
-                    ultra != 0, int(segment_rows), op->ctx->stream);
+  if (segment_rows <= kMaxSegmentRows) {
+    csr_sor_segmented(op->csr.desc(), x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega,
+                      forward != 0, ultra != 0, int(segment_rows), op->ctx->stream);
+  } else {
+    // longer segments: the scheduled sweep, its positions built on first use
+    const int64_t seg = std::min(segment_rows, op->rows());
+    if (op->csr.sched_seg != seg) op->csr.build_schedule(op->host_csr, seg);
+    csr_sor_scheduled(op->csr.desc(), x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega,
+                      forward != 0, ultra != 0, op->ctx->stream);
+  }
   API_END
 }
 
@@ -1317,6 +1347,7 @@ extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {
   p->cycle_kind = LGPU_CYCLE_REGULAR;
   p->cycle_iterations = 1;
   p->setup_on_device = 1;
+  p->sequential_rows = 0;
 }
 
 namespace {
@@ -1324,13 +1355,21 @@ namespace {
 void check_amg_params(lgpu_op finest, const lgpu_amg_params* params) {
   LGPU_REQUIRE(params->precision == LGPU_F64 || params->precision == LGPU_F32, "unknown precision");
   LGPU_REQUIRE(params->max_aggregate_size >= 1 && params->min_aggregate_size >= 1, "invalid aggregate sizes");
-  LGPU_REQUIRE(params->coarse_segment_rows >= 0 && params->coarse_segment_rows <= kMaxSegmentRows,
-               "coarse_segment_rows must lie in 0..64");
+  LGPU_REQUIRE(params->coarse_segment_rows >= 0 && params->coarse_segment_rows <= kSchedMaxRows,
+               "coarse_segment_rows must lie in 0..8192");
+  LGPU_REQUIRE(params->sequential_rows >= 0, "sequential_rows must not be negative");
   LGPU_REQUIRE(!finest->is_csr || !finest->csr.f32, "the finest CSR operator must hold f64 values");
   LGPU_REQUIRE(std::isfinite(params->relax) && params->relax != 0.0, "relax must be finite and nonzero");
   LGPU_REQUIRE(params->cycle_kind == LGPU_CYCLE_REGULAR || params->cycle_kind == LGPU_CYCLE_DECREASING,
                "unknown cycle kind");
   LGPU_REQUIRE(params->cycle_iterations >= 1, "cycle_iterations must be at least 1");
+}
+
+// Rows per smoother segment of a coarse level of n rows (0: one segment):
+// one segment when n is at most sequential_rows, else coarse_segment_rows.
+int64_t level_segment_rows(const lgpu_amg_params& p, int64_t n) {
+  if (n <= p.sequential_rows) return 0;
+  return p.coarse_segment_rows >= n ? 0 : p.coarse_segment_rows;
 }
 
 // Uploads the host hierarchy amg->host (levels, transfers, coarse matrices
@@ -1347,7 +1386,13 @@ void upload_hierarchy(lgpu_amg amg) {
     AmgLevel& a = amg->lv[size_t(l)];
     a.n = h.sizes[size_t(l)];
     if (l > 0) {
-      a.a.upload(h.coarse[size_t(l - 1)], amg->f32);
+      const setup::RowMatrix& m = h.coarse[size_t(l - 1)];
+      a.a.upload(m, amg->f32);
+      a.seg = level_segment_rows(amg->params, a.n);
+      // a level swept in one segment that fits one workgroup's LDS, and
+      // segments longer than a wavefront, run the scheduled sweep
+      if (a.n > 0 && ((a.seg == 0 && a.n <= kSchedMaxRows) || a.seg > kMaxSegmentRows))
+        a.a.build_schedule(m, a.seg == 0 ? a.n : a.seg);
       // the level vectors persist between cycles, as the reference's
       // hierarchy vectors do (hierarchy/wright.hpp:227-228): a DECREASING
       // cycle prolongs a coarse solution no visit of the same cycle wrote.

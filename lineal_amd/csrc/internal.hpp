@@ -122,6 +122,14 @@ struct CsrDesc {
   // its own segment, on the swept side) and of its trailing old-value
   // entries, over both sweep directions
   int seg_new_max, seg_trail_max;
+  // scheduled segment sweeps (csr_sor_scheduled): segment length (the whole
+  // matrix when it is n), {row, level} of every position for each direction,
+  // and the longest row
+  int64_t sched_seg = 0;
+  const int2* sched_fwd = nullptr;
+  const int2* sched_bwd = nullptr;
+  int sched_max_len = 0;
+  int64_t nnz = 0;  // stored entries
 };
 // rows per block of the CSR passes
 constexpr int kCsrBlockRows = 256;
@@ -180,6 +188,19 @@ void stencil_sor_weights(const StencilDesc& s, bool f32, void* w, hipStream_t st
 int64_t csr_partials(const CsrDesc& a);
 void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y, double omega,
                double* partials, hipStream_t st);
+// the longest segment a scheduled sweep keeps in one workgroup's LDS (its new
+// values, f64)
+constexpr int64_t kSchedMaxRows = 8192;
+// thread-segment SOR over the matrix's sweep schedule (segments of
+// a.sched_seg rows, each swept in order: rows grouped by dependency depth)
+void csr_sor_scheduled(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
+                       double omega, bool forward, bool ultra, hipStream_t st);
+// {row, level} positions of the scheduled sweeps of a CSR matrix: per segment
+// of `seg` rows, the rows sorted by their depth in the segment's dependency
+// graph for the forward (new values below the row) and the backward sweep;
+// returns the longest row
+int sched_positions(int64_t n, const int64_t* rowptr, const int32_t* cols, int64_t seg,
+                    std::vector<int2>& fwd, std::vector<int2>& bwd);
 void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
              bool forward, bool ultra, int* sched, int* err, int* flags, int epoch, hipStream_t st);
 // thread-segment SOR: segments of `seg` consecutive rows, 1 <= seg <= kMaxSegmentRows

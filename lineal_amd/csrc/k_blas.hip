@@ -167,7 +167,20 @@ __global__ __launch_bounds__(NT) void restrict_staged_kernel(const T* fine, cons
   const int64_t c0 = blockIdx.x * int64_t(NT);
   const int64_t c1 = c0 + NT < nc ? c0 + NT : nc;
   const int64_t e0 = ptr[c0], e1 = ptr[c1];
-  for (int64_t k = e0 + threadIdx.x; k < e1; k += NT) sv[k - e0] = fine[idx[k]];
+  // batches of 8 entries per thread: the index loads, then the gathers, in
+  // flight together
+  constexpr int B = 8;
+  for (int64_t kb = e0 + threadIdx.x; kb < e1; kb += B * NT) {
+    int32_t j[B];
+    T v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) j[u] = idx[kb + u * NT < e1 ? kb + u * NT : e1 - 1];
+#pragma unroll
+    for (int u = 0; u < B; ++u) v[u] = fine[j[u]];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (kb + u * NT < e1) sv[kb + u * NT - e0] = v[u];
+  }
   __syncthreads();
   const int64_t c = c0 + threadIdx.x;
   if (c < nc) {

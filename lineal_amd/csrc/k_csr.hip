@@ -10,6 +10,8 @@
 // the current sweep's epoch in its flag, then computes, publishes its value and
 // raises its own flag.  Dependencies only point to rows claimed earlier, so
 // every wait ends; waits are bounded and report a timeout on the error word.
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace lgpu {
@@ -18,6 +20,10 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int SPIN_LIMIT = 1 << 22;
+// entries a thread stages per batch into LDS: a batch's loads are issued
+// together, so staging costs one memory round trip (two with the gathers)
+// per batch rather than per entry
+constexpr int kStageBatch = 8;
 
 template <typename T, typename V, int MODE>
 __global__ __launch_bounds__(NT) void csr_kernel(int64_t n, const int64_t* __restrict__ rp,
@@ -104,11 +110,29 @@ __global__ __launch_bounds__(NT) void csr_staged_kernel(int64_t n, const int64_t
   const int64_t r0 = blockIdx.x * int64_t(NT);
   const int64_t r1 = min(r0 + NT, n);
   const int64_t e0 = rp[r0], e1 = rp[r1];
-  for (int64_t k = e0 + threadIdx.x; k < e1; k += NT) {
-    const int32_t j = ci[k];
-    sv[k - e0] = va[k];
-    sx[k - e0] = x ? x[j] : T(0);
-    if constexpr (MODE == SM_JACOBI) sc[k - e0] = j;
+  // batches of kStageBatch entries per thread: the column and value loads of a
+  // batch, then its gathers, are all in flight together
+  for (int64_t kb = e0 + threadIdx.x; kb < e1; kb += kStageBatch * NT) {
+    int32_t j[kStageBatch];
+    V v[kStageBatch];
+    T xv[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int64_t k = kb + u * NT < e1 ? kb + u * NT : e1 - 1;
+      j[u] = ci[k];
+      v[u] = va[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) xv[u] = x ? x[j[u]] : T(0);
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int64_t k = kb + u * NT;
+      if (k < e1) {
+        sv[k - e0] = v[u];
+        sx[k - e0] = xv[u];
+        if constexpr (MODE == SM_JACOBI) sc[k - e0] = j[u];
+      }
+    }
   }
   __syncthreads();
   const int64_t i = r0 + threadIdx.x;
@@ -528,12 +552,20 @@ __global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
   const int64_t s1 = min(s0 + WSEG, n);
   const int64_t e0 = rp[s0], e1 = rp[s1];
   if constexpr (STAGED) {
-    for (int64_t k = e0 + lane; k < e1; k += WSEG) {
-      S e;
-      e.col = ci[k];
-      e.val = va[k];
-      e.old = HAS_X ? xin[e.col] : T(0);
-      st.put(k - e0, e);
+    // batches of kStageBatch entries per lane (see csr_staged_kernel)
+    for (int64_t kb = e0 + lane; kb < e1; kb += kStageBatch * WSEG) {
+      S e[kStageBatch];
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u) {
+        const int64_t k = kb + u * WSEG < e1 ? kb + u * WSEG : e1 - 1;
+        e[u].col = ci[k];
+        e[u].val = va[k];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u) e[u].old = HAS_X ? xin[e[u].col] : T(0);
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u)
+        if (kb + u * WSEG < e1) st.put(kb + u * WSEG - e0, e[u]);
     }
   }
   __syncthreads();
@@ -654,6 +686,410 @@ __global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     done = __ballot(finished);
+  }
+}
+
+// Scheduled segment sweep: the thread-segment SOR of any segment length (one
+// segment for the whole level included), one workgroup per segment.  The host
+// sorts each segment's rows by their depth in the segment's dependency graph
+// (level 1: no new-value entry; a row's level is one more than the deepest
+// row whose new value it reads), `pos` holding {row, level} per position.
+// Round r computes the rows of level r, and a barrier closes every round, so
+// each row finds the new values it reads in LDS.  Thread t takes positions
+// s0 + t, s0 + t + SCHED_NT, ... in order and loads a position's row (its
+// entries classified as old value, new value or diagonal, the old values
+// gathered) as soon as it has finished the previous one, so the loads run
+// while the rounds in between pass.  Each row performs the oracle's
+// operations in its column order (gauss-seidel-variant/sweep.hpp:134-200
+// with the segment's own new values): padded slots hold value 0 and old value
+// 0, and fma(-0, 0, a) == a bit for bit.
+constexpr int SCHED_NT = 256;
+
+template <typename T>
+struct BitsOf;
+template <>
+struct BitsOf<float> {
+  using type = uint32_t;
+};
+template <>
+struct BitsOf<double> {
+  using type = uint64_t;
+};
+
+// One row of a scheduled sweep in registers: SLOTS entry slots (rows with more
+// entries read them from memory when they run).
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, int SLOTS>
+struct SchedRow {
+  int64_t i = -1;   // row (-1: none)
+  int lev = 0;
+  int len = 0;
+  T acc = T(0), d = T(1), div = T(0), xi = T(0);
+  T val[SLOTS];
+  T src[SLOTS];     // old value, or the LDS slot of a new value (bits)
+  uint64_t newmask = 0;
+
+  // the row's loads are issued together (no load waits on a branch), then
+  // the entries are classified and the old values gathered
+  __device__ __forceinline__ void load(int64_t ii, int lv, int64_t s0, int64_t s1,
+                                       const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                       const V* __restrict__ va, const T* __restrict__ xin,
+                                       const T* __restrict__ b) {
+    using Bits = typename BitsOf<T>::type;
+    i = ii;
+    lev = lv;
+    const int64_t k0 = rp[i];
+    len = int(rp[i + 1] - k0);
+    acc = b[i];
+    xi = HAS_X ? xin[i] : T(0);
+    d = T(1);
+    div = T(0);
+    newmask = 0;
+    const int m = len <= SLOTS ? len : 0;
+    // first the entries (src holds the column's bits until classified)
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) {
+      const int64_t k = k0 + (u < m ? u : 0);
+      src[u] = __builtin_bit_cast(T, static_cast<Bits>(m > 0 ? ci[k] : int32_t(ii)));
+      val[u] = m > 0 ? T(va[k]) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < SLOTS; ++u) {
+      const int64_t j = int64_t(static_cast<int32_t>(__builtin_bit_cast(Bits, src[u])));
+      const T a = val[u];
+      const bool valid = u < m;
+      const bool inside = j >= s0 && j < s1;
+      const bool is_diag = valid && j == i;
+      const bool is_new = valid && !is_diag && inside && (FWD ? j < i : j > i);
+      const T old = HAS_X ? xin[j] : T(0);
+      d = is_diag ? a : d;
+      if (ULTRA) div += (valid && !is_diag && !inside) ? (a < T(0) ? -a : a) : T(0);
+      // the diagonal's slot: for ultra acc - a_ii x_i at its place, else nothing
+      val[u] = valid && (!is_diag || ULTRA) ? a : T(0);
+      src[u] = is_new ? __builtin_bit_cast(T, static_cast<Bits>(j - s0))
+                      : (is_diag ? (ULTRA ? xi : T(0)) : (valid ? old : T(0)));
+      newmask |= uint64_t(is_new) << u;
+    }
+  }
+};
+
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, int SLOTS>
+__global__ __launch_bounds__(SCHED_NT) void csr_sor_sched_kernel(
+    int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const V* __restrict__ va, const T* __restrict__ xin, T* __restrict__ xout,
+    const T* __restrict__ b, T omega, int64_t seg, const int2* __restrict__ pos) {
+  using Bits = typename BitsOf<T>::type;
+  extern __shared__ double sched_lds[];
+  T* const xnew = reinterpret_cast<T*>(sched_lds);
+  const int64_t s0 = blockIdx.x * seg;
+  const int64_t s1 = min(s0 + seg, n);
+  // positions are sorted by level: the last one holds the segment's depth
+  const int depth = pos[s1 - 1].y;
+  const T m1w = T(1) - omega;
+  SchedRow<T, V, FWD, ULTRA, HAS_X, SLOTS> row;
+  int64_t p = s0 + threadIdx.x;
+  if (p < s1) {
+    const int2 q = pos[p];
+    row.load(q.x, q.y, s0, s1, rp, ci, va, xin, b);
+  }
+  for (int r = 1; r <= depth; ++r) {
+    while (row.i >= 0 && row.lev == r) {
+      T a = row.acc, d = row.d, div = row.div;
+      const int self = int(row.i - s0);
+      if (row.len <= SLOTS) {
+#pragma unroll
+        for (int u = 0; u < SLOTS; u += 4) {
+          if (u < row.len) {
+#pragma unroll
+            for (int w = u; w < u + 4; ++w) {
+              const bool is_new = (row.newmask >> w) & 1u;
+              const T nv = xnew[is_new ? int(__builtin_bit_cast(Bits, row.src[w])) : self];
+              a = fma(-row.val[w], is_new ? nv : row.src[w], a);
+            }
+          }
+        }
+      } else {
+        // a row longer than the register slots: its entries in column order
+        // from memory
+        const int64_t k0 = rp[row.i];
+        for (int u = 0; u < row.len; ++u) {
+          const int64_t j = ci[k0 + u];
+          const T aij = T(va[k0 + u]);
+          if (j == row.i) {
+            d = aij;
+            if (ULTRA) a = fma(-aij, row.xi, a);
+            continue;
+          }
+          const bool inside = j >= s0 && j < s1;
+          if (ULTRA && !inside) div += aij < T(0) ? -aij : aij;
+          const bool use_new = inside && (FWD ? j < row.i : j > row.i);
+          const T xj = use_new ? xnew[j - s0] : (HAS_X ? xin[j] : T(0));
+          a = fma(-aij, xj, a);
+        }
+      }
+      T xn;
+      if (ULTRA) {
+        const T ad = d < T(0) ? -d : d;
+        xn = row.xi + (omega * a) / (ad + T(0.5) * div);
+      } else {
+        const T quot = a / d;
+        xn = fma(omega, quot, m1w * row.xi);
+      }
+      xnew[self] = xn;
+      xout[row.i] = xn;
+      p += SCHED_NT;
+      if (p < s1) {
+        const int2 q = pos[p];
+        row.load(q.x, q.y, s0, s1, rp, ci, va, xin, b);
+      } else {
+        row.i = -1;
+      }
+    }
+    // a barrier that orders only the LDS: the next rows' loads stay in flight
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  }
+}
+
+// One segment for the whole level, swept by one wavefront with the level
+// staged in LDS: the matrix (values, 16-bit columns), b, and the vector,
+// which the sweep updates in place.  The schedule's levels order every row
+// after the rows whose new values it reads and before the rows whose old
+// values it reads, so x[j] holds exactly the value the sequential sweep
+// reads.  Positions carry {row | level << 16, first entry | length << 24}.
+// Lane t takes positions t, t + 64, ... and loads a position's entries into
+// registers as soon as it has finished the previous one; a round is then
+// the reads of x at those columns, the FMA chain in column order and the
+// division.  Rows longer than the register slots read their entries from LDS.
+constexpr int LDS_NT = 64;
+constexpr int kLdsSlots = 32;
+
+__host__ __device__ inline size_t lds_sweep_bytes(int64_t n, int64_t nnz, size_t es) {
+  // (16 bytes of slack: a padded slot of an empty last row reads one entry
+  // past the columns)
+  return size_t(2 * n + 1 + nnz) * es + size_t(n) * sizeof(int2) + size_t(nnz) * sizeof(uint16_t) + 16;
+}
+
+template <typename T, bool FWD, bool ULTRA, bool HAS_X>
+__global__ __launch_bounds__(LDS_NT) void csr_sor_lds_kernel(int64_t n, const int64_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ ci,
+                                                           const T* __restrict__ va, const T* __restrict__ xin,
+                                                           T* __restrict__ xout, const T* __restrict__ b,
+                                                           T omega, const int2* __restrict__ pos) {
+  extern __shared__ double lds_sweep[];
+  const int lane = threadIdx.x;
+  const int64_t e0 = rp[0], nnz = rp[n] - e0;
+  T* const x = reinterpret_cast<T*>(lds_sweep);  // n + 1
+  T* const bs = x + n + 1;
+  T* const vals = bs + n;
+  int2* const info = reinterpret_cast<int2*>(vals + nnz);
+  uint16_t* const cols = reinterpret_cast<uint16_t*>(info + n);
+  // staged in batches of kStage loads per lane, all issued before the first
+  // store, so that each batch costs one memory round trip
+  constexpr int kStage = 16;
+  for (int64_t k0 = lane; k0 < nnz; k0 += kStage * LDS_NT) {
+    T v[kStage];
+    int32_t c[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int64_t k = k0 + u * LDS_NT < nnz ? k0 + u * LDS_NT : nnz - 1;
+      v[u] = va[e0 + k];
+      c[u] = ci[e0 + k];
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int64_t k = k0 + u * LDS_NT;
+      if (k < nnz) {
+        vals[k] = v[u];
+        cols[k] = uint16_t(c[u]);
+      }
+    }
+  }
+  for (int64_t p0 = lane; p0 < n; p0 += kStage * LDS_NT) {
+    T xv[kStage], bv[kStage];
+    int2 q[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int64_t p = p0 + u * LDS_NT < n ? p0 + u * LDS_NT : n - 1;
+      xv[u] = HAS_X ? xin[p] : T(0);
+      bv[u] = b[p];
+      q[u] = pos[p];
+    }
+    int64_t r0[kStage], r1[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      r0[u] = rp[q[u].x];
+      r1[u] = rp[q[u].x + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int64_t p = p0 + u * LDS_NT;
+      if (p < n) {
+        x[p] = xv[u];
+        bs[p] = bv[u];
+        info[p] = int2{q[u].x | (q[u].y << 16), int((r0[u] - e0) | ((r1[u] - r0[u]) << 24))};
+      }
+    }
+  }
+  if (lane == 0) x[n] = T(0);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  const int depth = info[n - 1].x >> 16;
+  const T m1w = T(1) - omega;
+  // the lane's next row: position, row, level, entries
+  int64_t p = lane;
+  int i = 0, lev = 0x7FFFFFFF, k0 = 0, len = 0;
+  int col[kLdsSlots];
+  T val[kLdsSlots];
+  auto fetch = [&]() {
+    if (p < n) {
+      const int2 q = info[p];
+      i = q.x & 0xFFFF;
+      lev = q.x >> 16;
+      k0 = q.y & 0xFFFFFF;
+      len = int(uint32_t(q.y) >> 24);
+    } else {
+      lev = 0x7FFFFFFF;
+      len = 0;
+    }
+    // every slot reads (the entry itself, or the row's first one), then
+    // selects: no branch, so the reads are all in flight together
+#pragma unroll
+    for (int u = 0; u < kLdsSlots; ++u) {
+      const int k = k0 + (u < len ? u : 0);
+      const int c = int(cols[k]);
+      const T v = vals[k];
+      col[u] = u < len ? c : int(n);  // padding reads x[n] = 0 with value 0
+      val[u] = u < len ? v : T(0);
+    }
+  };
+  fetch();
+  for (int r = 1; r <= depth; ++r) {
+    while (lev == r) {
+      T acc = bs[i], d = T(1);
+      if (len <= kLdsSlots) {
+        T xv[kLdsSlots];
+#pragma unroll
+        for (int u = 0; u < kLdsSlots; ++u) xv[u] = x[col[u]];
+#pragma unroll
+        for (int u = 0; u < kLdsSlots; u += 4) {
+          if (u < len) {
+#pragma unroll
+            for (int w = u; w < u + 4; ++w) {
+              // the diagonal: d, and for ultra acc - a_ii x_i at its place
+              // (x[i] still holds the old value)
+              const bool diag = col[w] == i;
+              d = diag ? val[w] : d;
+              const T t = fma(-val[w], xv[w], acc);
+              acc = (diag && !ULTRA) ? acc : t;
+            }
+          }
+        }
+      } else {
+        for (int k = k0; k < k0 + len; ++k) {
+          const int j = cols[k];
+          const T a = vals[k];
+          if (j == i) {
+            d = a;
+            if (!ULTRA) continue;
+          }
+          acc = fma(-a, x[j], acc);
+        }
+      }
+      const T xi = x[i];
+      T xn;
+      if (ULTRA) {
+        const T ad = d < T(0) ? -d : d;
+        xn = xi + (omega * acc) / (ad + T(0.5) * T(0));
+      } else {
+        const T quot = acc / d;
+        xn = fma(omega, quot, m1w * xi);
+      }
+      x[i] = xn;
+      p += LDS_NT;
+      fetch();
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  for (int64_t q = lane; q < n; q += LDS_NT) xout[q] = x[q];
+}
+
+template <typename T, bool FWD, bool ULTRA, bool HAS_X>
+void launch_lds3(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+                 const int2* pos, hipStream_t st) {
+  const size_t lds = lds_sweep_bytes(a.n, a.nnz, sizeof(T));
+  hipLaunchKernelGGL((csr_sor_lds_kernel<T, FWD, ULTRA, HAS_X>), dim3(1), dim3(LDS_NT), lds, st, a.n,
+                     a.rowptr, a.cols, static_cast<const T*>(a.vals), static_cast<const T*>(x_in),
+                     static_cast<T*>(x_out), static_cast<const T*>(b), T(omega), pos);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void dispatch_lds(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+                  bool fwd, bool ultra, hipStream_t st) {
+  const int2* pos = fwd ? a.sched_fwd : a.sched_bwd;
+  if (fwd) {
+    if (ultra) {
+      if (x_in) launch_lds3<T, true, true, true>(a, x_in, x_out, b, omega, pos, st);
+      else launch_lds3<T, true, true, false>(a, x_in, x_out, b, omega, pos, st);
+    } else {
+      if (x_in) launch_lds3<T, true, false, true>(a, x_in, x_out, b, omega, pos, st);
+      else launch_lds3<T, true, false, false>(a, x_in, x_out, b, omega, pos, st);
+    }
+  } else {
+    if (ultra) {
+      if (x_in) launch_lds3<T, false, true, true>(a, x_in, x_out, b, omega, pos, st);
+      else launch_lds3<T, false, true, false>(a, x_in, x_out, b, omega, pos, st);
+    } else {
+      if (x_in) launch_lds3<T, false, false, true>(a, x_in, x_out, b, omega, pos, st);
+      else launch_lds3<T, false, false, false>(a, x_in, x_out, b, omega, pos, st);
+    }
+  }
+}
+
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
+void launch_sched3(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+                   const int2* pos, hipStream_t st) {
+  const int64_t seg = a.sched_seg;
+  const int64_t nseg = (a.n + seg - 1) / seg;
+  const size_t lds = size_t(std::min(seg, a.n)) * sizeof(T);
+  const auto xi = static_cast<const T*>(x_in);
+  const auto xo = static_cast<T*>(x_out);
+  const auto bp = static_cast<const T*>(b);
+  const auto vp = static_cast<const V*>(a.vals);
+  // rows longer than the slots read their entries when they run
+  if (a.sched_max_len <= 16)
+    hipLaunchKernelGGL((csr_sor_sched_kernel<T, V, FWD, ULTRA, HAS_X, 16>), dim3(nseg), dim3(SCHED_NT),
+                       lds, st, a.n, a.rowptr, a.cols, vp, xi, xo, bp, T(omega), seg, pos);
+  else
+    hipLaunchKernelGGL((csr_sor_sched_kernel<T, V, FWD, ULTRA, HAS_X, 32>), dim3(nseg), dim3(SCHED_NT),
+                       lds, st, a.n, a.rowptr, a.cols, vp, xi, xo, bp, T(omega), seg, pos);
+  LGPU_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, typename V>
+void dispatch_sched(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,
+                    bool fwd, bool ultra, hipStream_t st) {
+  const int2* pos = fwd ? a.sched_fwd : a.sched_bwd;
+  if (fwd) {
+    if (ultra) {
+      if (x_in) launch_sched3<T, V, true, true, true>(a, x_in, x_out, b, omega, pos, st);
+      else launch_sched3<T, V, true, true, false>(a, x_in, x_out, b, omega, pos, st);
+    } else {
+      if (x_in) launch_sched3<T, V, true, false, true>(a, x_in, x_out, b, omega, pos, st);
+      else launch_sched3<T, V, true, false, false>(a, x_in, x_out, b, omega, pos, st);
+    }
+  } else {
+    if (ultra) {
+      if (x_in) launch_sched3<T, V, false, true, true>(a, x_in, x_out, b, omega, pos, st);
+      else launch_sched3<T, V, false, true, false>(a, x_in, x_out, b, omega, pos, st);
+    } else {
+      if (x_in) launch_sched3<T, V, false, false, true>(a, x_in, x_out, b, omega, pos, st);
+      else launch_sched3<T, V, false, false, false>(a, x_in, x_out, b, omega, pos, st);
+    }
   }
 }
 
@@ -816,6 +1252,89 @@ void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const vo
   if (a.n == 0) return;
   if (a.f32) dispatch_segment<float, float>(a, x_in, x_out, b, omega, forward, ultra, seg, st);
   else dispatch_segment<double, double>(a, x_in, x_out, b, omega, forward, ultra, seg, st);
+}
+
+int sched_positions(int64_t n, const int64_t* rowptr, const int32_t* cols, int64_t seg,
+                    std::vector<int2>& fwd, std::vector<int2>& bwd) {
+  LGPU_REQUIRE(seg > 0, "segment length must be positive");
+  LGPU_REQUIRE(n < (int64_t(1) << 31), "too many rows for a sweep schedule");
+  std::vector<int32_t> lev(static_cast<size_t>(n), 0);
+  int longest = 0;
+  for (int64_t i = 0; i < n; ++i) longest = std::max(longest, int(rowptr[i + 1] - rowptr[i]));
+  auto order = [&](std::vector<int2>& out) {
+    out.resize(size_t(n));
+    for (int64_t s0 = 0; s0 < n; s0 += seg) {
+      const int64_t s1 = std::min(s0 + seg, n);
+      int depth = 0;
+      for (int64_t i = s0; i < s1; ++i) depth = std::max(depth, lev[size_t(i)]);
+      // counting sort by level, rows ascending within a level
+      std::vector<int64_t> start(size_t(depth) + 2, 0);
+      for (int64_t i = s0; i < s1; ++i) ++start[size_t(lev[size_t(i)]) + 1];
+      for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
+      for (int64_t i = s0; i < s1; ++i) {
+        const int l = lev[size_t(i)];
+        out[size_t(s0 + start[size_t(l)]++)] = int2{int(i), l};
+      }
+    }
+  };
+  // Levels also keep every row that reads the old value of a row of its
+  // segment before that row (the sweep order's write-after-read), so a sweep
+  // may update one vector in place.  after[i]: the deepest level of a row
+  // that reads i's old value.
+  std::vector<int32_t> after(static_cast<size_t>(n), 0);
+  // forward: new values below the row, old values above it
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t s0 = i / seg * seg, s1 = std::min(s0 + seg, n);
+    int l = after[size_t(i)] + 1;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int64_t j = cols[k];
+      if (j >= s0 && j < i) l = std::max(l, lev[size_t(j)] + 1);
+    }
+    lev[size_t(i)] = l;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int64_t j = cols[k];
+      if (j > i && j < s1) after[size_t(j)] = std::max(after[size_t(j)], l);
+    }
+  }
+  order(fwd);
+  // backward: new values above the row, old values below it
+  std::fill(after.begin(), after.end(), 0);
+  for (int64_t i = n - 1; i >= 0; --i) {
+    const int64_t s0 = i / seg * seg, s1 = std::min(s0 + seg, n);
+    int l = after[size_t(i)] + 1;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int64_t j = cols[k];
+      if (j > i && j < s1) l = std::max(l, lev[size_t(j)] + 1);
+    }
+    lev[size_t(i)] = l;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int64_t j = cols[k];
+      if (j >= s0 && j < i) after[size_t(j)] = std::max(after[size_t(j)], l);
+    }
+  }
+  order(bwd);
+  return longest;
+}
+
+// rows and levels fit 15 bits of the packed positions, entry offsets 24 bits
+static bool csr_lds_sweep_fits(int64_t n, int64_t nnz, bool f32) {
+  return n <= 0x7FFF && nnz < (int64_t(1) << 24) &&
+         lds_sweep_bytes(n, nnz, f32 ? sizeof(float) : sizeof(double)) <= wave_lds_budget();
+}
+
+void csr_sor_scheduled(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
+                       double omega, bool forward, bool ultra, hipStream_t st) {
+  LGPU_REQUIRE(x_in != x_out, "SOR sweep needs distinct input and output vectors");
+  LGPU_REQUIRE(a.sched_seg > 0 && a.sched_fwd && a.sched_bwd, "no sweep schedule for this matrix");
+  if (a.n == 0) return;
+  // one segment that fits one workgroup's LDS: one wavefront over the LDS copy
+  if (a.sched_seg >= a.n && a.sched_max_len < 256 && csr_lds_sweep_fits(a.n, a.nnz, a.f32)) {
+    if (a.f32) dispatch_lds<float>(a, x_in, x_out, b, omega, forward, ultra, st);
+    else dispatch_lds<double>(a, x_in, x_out, b, omega, forward, ultra, st);
+    return;
+  }
+  if (a.f32) dispatch_sched<float, float>(a, x_in, x_out, b, omega, forward, ultra, st);
+  else dispatch_sched<double, double>(a, x_in, x_out, b, omega, forward, ultra, st);
 }
 
 void csr_sor(const CsrDesc& a, const void* x_in, void* x_out, const void* b, double omega,

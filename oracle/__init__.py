@@ -122,6 +122,7 @@ def lib():
             "or_amg_cholesky": (cp, [hp]),
             "or_amg_stats": (None, [hp, C.c_int, i64p]),
             "or_amg_set_coarse_segment": (None, [hp, C.c_int64]),
+            "or_amg_set_level_segment": (None, [hp, C.c_int, C.c_int64]),
             "or_amg_set_cycle": (None, [hp, C.c_double, C.c_int, C.c_int]),
             "or_cholesky_solve": (None, [cp, f64p, f64p]),
             "or_cholesky_decompose": (C.c_int64, [cp, i64p, i64p, f64p]),
@@ -330,6 +331,11 @@ def strength_graph(A, theta=1.0 / 3.0):
     return props[:ne], iso[: A.N]
 
 
+# coarse levels of at most this many rows are swept in one segment (the
+# library's default lgpu_amg_params.sequential_rows; 0: none)
+SEQUENTIAL_ROWS = 0
+
+
 def default_params(coarse_f32=False):
     """Parameters of test/aux/default.hpp:170-188."""
     return AmgParams(16, 32, 1.2, 1.0 / 3.0, 1.0 / 3.0, 6, 9, int(coarse_f32))
@@ -337,15 +343,24 @@ def default_params(coarse_f32=False):
 
 class Hierarchy:
     def __init__(self, A, params: AmgParams | None = None, coarse_f32=False, coarse_segment=0,
-                 relax=1.0, cycle_kind=0, cycle_iterations=1):
+                 relax=1.0, cycle_kind=0, cycle_iterations=1, level_segments=None,
+                 sequential_rows=SEQUENTIAL_ROWS):
         """relax, cycle_kind (0 REGULAR, 1 DECREASING) and cycle_iterations:
-        MultiGridSolver{relax, DefaultCycle{kind, iterations}}."""
+        MultiGridSolver{relax, DefaultCycle{kind, iterations}}.  coarse_segment:
+        rows per smoother segment on the coarse levels (0: one segment);
+        coarse levels of at most sequential_rows rows are swept in one segment;
+        level_segments: {level: rows} overriding both on single levels."""
         self.A = A  # keep the finest operator alive
         self.params = params or default_params(coarse_f32)
         self.h = lib().or_amg_setup(C.byref(A.mat), C.byref(self.params))
         if not self.h:
             raise ValueError("the coarsest factorisation failed")
         lib().or_amg_set_coarse_segment(self.h, int(coarse_segment))
+        for lvl in range(1, self.level_num):
+            if self.size(lvl) <= sequential_rows:
+                lib().or_amg_set_level_segment(self.h, lvl, 0)
+        for lvl, rows in (level_segments or {}).items():
+            lib().or_amg_set_level_segment(self.h, int(lvl), int(rows))
         lib().or_amg_set_cycle(self.h, float(relax), int(cycle_kind), int(cycle_iterations))
 
     def __del__(self):

@@ -64,6 +64,9 @@ struct or_hierarchy {
   double* lu_l;       /* unit lower factor, dense row major */
   double* lu_u;       /* upper factor, dense row major */
   int64_t coarse_seg; /* rows per smoother segment on levels >= 1 (0: one segment) */
+  /* per-level segment lengths overriding coarse_seg where level_seg_set[l] */
+  int64_t level_seg[32];
+  int level_seg_set[32];
   /* MultiGridSolver{relax, DefaultCycle{kind, iterations}}: 0 in relax means
    * 1 (the calloc'ed default), kind 0 REGULAR / 1 DECREASING, 0 in
    * iterations means 1 */
@@ -87,7 +90,8 @@ struct or_hierarchy {
  * uses new values only inside the row's own segment). */
 static int64_t* or_level_segments(const or_hierarchy* h, int l, int* nseg) {
   const int64_t n = h->size[l];
-  const int64_t len = (l == 0 || h->coarse_seg <= 0) ? n : h->coarse_seg;
+  const int64_t rows = h->level_seg_set[l] ? h->level_seg[l] : h->coarse_seg;
+  const int64_t len = (l == 0 || rows <= 0) ? n : rows;
   const int64_t k = (n + len - 1) / len;
   int64_t* seg = (int64_t*)malloc(sizeof(int64_t) * (size_t)(k + 1));
   for (int64_t s = 0; s < k; ++s) seg[s] = s * len;
@@ -97,6 +101,12 @@ static int64_t* or_level_segments(const or_hierarchy* h, int l, int* nseg) {
 }
 
 void or_amg_set_coarse_segment(or_hierarchy* h, int64_t rows) { h->coarse_seg = rows; }
+void or_amg_set_level_segment(or_hierarchy* h, int level, int64_t rows) {
+  if (level >= 1 && level < 32) {
+    h->level_seg[level] = rows;
+    h->level_seg_set[level] = 1;
+  }
+}
 void or_amg_set_cycle(or_hierarchy* h, double relax, int kind, int iterations) {
   h->relax = relax;
   h->cycle_kind = kind;

@@ -157,6 +157,9 @@ void or_amg_stats(const or_hierarchy* h, int level, int64_t out[5]);
 
 /* rows per thread segment of the smoothers on levels >= 1 (0: one segment) */
 void or_amg_set_coarse_segment(or_hierarchy* h, int64_t rows);
+/* rows per smoother segment of one coarse level (0: one segment), overriding
+ * or_amg_set_coarse_segment there */
+void or_amg_set_level_segment(or_hierarchy* h, int level, int64_t rows);
 /* MultiGridSolver{relax, DefaultCycle{kind, iterations}} (multigrid/solver.hpp:183-186,
  * multigrid/cycle.hpp:24-79): kind 0 REGULAR, 1 DECREASING; iterations >= 1.
  * The default is relax 1 and DefaultCycle{REGULAR, 1} (a V-cycle). */

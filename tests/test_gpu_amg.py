@@ -64,6 +64,26 @@ def test_vcycle_equals_oracle(dims, kind, ultra):
         np.testing.assert_array_equal(z, ref)
 
 
+@pytest.mark.parametrize("dims,kind", [((24, 20, 16), "values"), ((24, 24, 24), "spheres"), ((40, 40, 40), "values")])
+@pytest.mark.parametrize("segments", [(64, 0), (64, 4096), (200, 0), (1000, 300), (0, 0)])
+@pytest.mark.parametrize("ultra", [False, True])
+def test_vcycle_segment_policies(dims, kind, segments, ultra):
+    """The coarse smoothers' segments: coarse_segment_rows rows per segment (64:
+    one wavefront per segment; longer: the scheduled sweep) and one segment on
+    the levels of at most sequential_rows rows; the cycle equals the oracle's
+    with the same segments bit for bit, f64 and f32."""
+    seg, seq = segments
+    A, S = pair(dims, kind)
+    for f32 in (False, True):
+        prec = np.float32 if f32 else np.float64
+        H = O.Hierarchy(S, coarse_f32=f32, coarse_segment=seg, sequential_rows=seq)
+        M = la.MultiGridSolver(precision=prec, ultra=ultra, coarse_segment_rows=seg,
+                               sequential_rows=seq).instantiate(A)
+        r = problems.uniform(S.N, 9).astype(prec)
+        z = M.apply(la.DenseVector(r), la.DenseVector(S.N, prec)).numpy()
+        np.testing.assert_array_equal(z, H.vcycle(r, ultra))
+
+
 @pytest.mark.parametrize("dims,kind", CASES[:2])
 def test_vcycle_lexicographic_coarse_levels(dims, kind):
     """coarse_segment_rows = 0: every level swept in one segment (the oracle with one thread)."""

@@ -27,10 +27,10 @@ def make_pair(dims, kind, seed=1, sigma=1.0, p_start=1.0, p_end=0.0, mat_coeffs=
             O.Stencil(dims, materials=m, mat_coeffs=mc, p_start=p_start, p_end=p_end))
 
 
-def same(a, b):
+def same(a, b, case=""):
     assert a.shape == b.shape
     bad = np.flatnonzero(~((a == b) | (np.isnan(a) & np.isnan(b))))
-    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: {a[bad[:5]]} vs {b[bad[:5]]}"
+    assert bad.size == 0, f"{case}{bad.size} mismatches, first at {bad[:5]}: {a[bad[:5]]} vs {b[bad[:5]]}"
 
 
 @pytest.mark.parametrize("dims", SHAPES)
@@ -200,6 +200,33 @@ def test_csr_segment_sor_register_and_fallback_paths(long_rows, dtype):
                 out = G.sor(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype),
                             la.DenseVector(b), 1.1, forward, ultra, segment_rows=64)
                 same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.1), forward, ultra, bounds))
+
+
+@pytest.mark.parametrize("long_rows", [(), (70, 130, 400), (3, 64, 100, 190, 250, 599)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_csr_scheduled_segment_sor(long_rows, dtype):
+    """Segments longer than a wavefront (the scheduled sweep: rows grouped by
+    dependency depth, one workgroup per segment), one segment for the whole
+    matrix included: bit for bit the oracle's thread-segment SOR, forward and
+    backward, regular and ultra, with and without x_in; rows of up to 60
+    entries take the path that reads their entries from memory."""
+    n = 600
+    rp, cols, vals = long_row_csr(n, 23 + len(long_rows), set(long_rows))
+    if dtype == np.float32:
+        vals = vals.astype(np.float32).astype(np.float64)
+    C = O.Csr(rp, cols, vals, symmetric=False, is_f32=dtype == np.float32)
+    G = la.CsrMatrix.from_arrays(rp, cols, vals, dtype, symmetric=False)
+    x = problems.uniform(n, 6, dtype)
+    b = problems.uniform(n, 7, dtype)
+    for seg in (65, 100, 256, 600, 5000):
+        bounds = list(range(0, n, seg)) + [n]
+        for forward in (True, False):
+            for ultra in (False, True):
+                for with_x in (True, False):
+                    out = G.sor(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype),
+                                la.DenseVector(b), 0.9, forward, ultra, segment_rows=seg)
+                    same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(0.9), forward, ultra, bounds),
+                         f"segment {seg}, forward {forward}, ultra {ultra}, x_in {with_x}: ")
 
 
 @pytest.mark.parametrize("dims", [(13, 17, 5), (33, 17, 65), (6, 21, 130)])
