@@ -58,7 +58,7 @@ struct lgpu_ctx_s {
   int64_t partial_cap = 0;
   DevBuf scal;            // SC_COUNT device scalars
   double* host_scal = nullptr;  // pinned mirror
-  hipEvent_t scal_ev = nullptr;  // marks the latest mirror copy (post_scalars)
+  hipEvent_t scal_ev = nullptr;  // marks the latest copy of the scalars to the host mirror
   // opt-in timing of kernel classes (lgpu_ctx_kernel_timing): start/stop
   // event pairs recorded around every launch of a class whose bit is set
   int ktime_mask = 0;
@@ -493,14 +493,6 @@ double read_scalar(lgpu_ctx ctx, int slot) {
   LGPU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return ctx->host_scal[0];
 }
-
-// All device scalars into the pinned mirror, without waiting: work queued
-// after this call overlaps the copy and the host's wait_scalars().
-void post_scalars(lgpu_ctx ctx) {
-  LGPU_HIP_CHECK(hipMemcpyAsync(ctx->host_scal, ctx->scal.ptr, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  LGPU_HIP_CHECK(hipEventRecord(ctx->scal_ev, ctx->stream));
-}
-void wait_scalars(lgpu_ctx ctx) { LGPU_HIP_CHECK(hipEventSynchronize(ctx->scal_ev)); }
 
 // r = b - A x (f64), ||r||^2 into the device scalar *dst
 void residual_norm(lgpu_op op, const double* b, const double* x, double* r, double* dst) {

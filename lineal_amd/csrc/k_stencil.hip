@@ -42,7 +42,6 @@ constexpr int kRows = sizeof(T) == 4 ? 4 : 4;
 template <typename T>
 constexpr int kSlots = sizeof(T) == 4 ? 3 : 3;
 
-__device__ __forceinline__ int read_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ float read_lane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
