@@ -280,6 +280,11 @@ struct lgpu_amg_s {
   double setup_seconds = 0.0;
   bool f32 = false;
   DevBuf r32, z32;  // finest f32 copies for the mixed-precision cycle
+  // the V-cycle's tail (levels tail_first .. coarsest in one workgroup,
+  // amg_tail): its TailLevel array on the device, 0 levels when none
+  DevBuf tail;
+  int tail_first = 0, tail_count = 0, tail_waves = 0;
+  size_t tail_wave_lds = 0;
   // an SSOR preconditioner instead of a hierarchy (lgpu_ssor_precon_create):
   // ssor_iters applications of a forward and a backward sweep from zero
   int ssor_iters = 0;
@@ -574,6 +579,19 @@ void vcycle(lgpu_amg amg, const void* r, void* z, lgpu_op fine, const void* x0 =
       dense_cholesky_solve(amg->chol.as<double>(), c.n, c.rhs.ptr, c.sol.ptr, f32, st);
   };
   const bool regular = amg->params.cycle_kind == LGPU_CYCLE_REGULAR;
+  if (regular && amg->params.cycle_iterations == 1 && amg->tail_count >= 2) {
+    // the V-cycle with its small levels in one workgroup
+    const int t0 = amg->tail_first;
+    for (int l = 0; l < t0; ++l) down(l);
+    {
+      KTimed kt(amg->ctx, LGPU_KT_CSR_SOR);
+      amg_tail(amg->tail.as<TailLevel>(), amg->tail_count, amg->lv[size_t(L - 1)].n, amg->chol.as<double>(),
+               amg->host.lu ? amg->upper.as<double>() : nullptr, f32, ultra, inv_relax, amg->tail_waves,
+               amg->tail_wave_lds, st);
+    }
+    for (int l = t0 - 1; l >= 0; --l) up(l);
+    return;
+  }
   std::vector<int> iter(size_t(L), 0), num(size_t(L), 0);
   int top = 0;
   iter[0] = num[0] = amg->params.cycle_iterations;
@@ -1364,6 +1382,53 @@ int64_t level_segment_rows(const lgpu_amg_params& p, int64_t n) {
   return p.coarse_segment_rows >= n ? 0 : p.coarse_segment_rows;
 }
 
+// The V-cycle's tail: the longest run of levels before the coarsest whose
+// smoothers are 64-row wavefront segments staged in LDS and that have at most
+// kTailMaxRows rows, plus the coarsest; amg_tail runs them in one workgroup.
+void build_tail(lgpu_amg amg) {
+  amg->tail_count = 0;
+  const int L = int(amg->lv.size());
+  if (L < 2 || amg->lv[size_t(L - 1)].n <= 0) return;
+  const size_t budget = tail_lds_budget();
+  int t0 = L - 1;
+  size_t stride = 0;
+  while (t0 - 1 >= 1) {
+    const AmgLevel& a = amg->lv[size_t(t0 - 1)];
+    if (a.n <= 0 || a.n > kTailMaxRows || a.seg != kMaxSegmentRows || a.a.sched_seg != 0) break;
+    const size_t need = tail_wave_lds_bytes(a.a.seg_entries, amg->f32);
+    if (need > budget) break;
+    stride = std::max(stride, need);
+    --t0;
+  }
+  if (t0 >= L - 1) return;  // no level besides the coarsest
+  const int waves = int(std::min<size_t>(kTailMaxWaves, budget / stride));
+  std::vector<TailLevel> tl;
+  for (int l = t0; l < L; ++l) {
+    AmgLevel& a = amg->lv[size_t(l)];
+    TailLevel t{};
+    t.n = a.n;
+    t.rowptr = a.a.rowptr.as<int64_t>();
+    t.cols = a.a.cols.as<int32_t>();
+    t.vals = a.a.vals.ptr;
+    t.cap = a.a.seg_entries;
+    t.shortlists = a.a.seg_new_max <= kSegNewShort && a.a.seg_trail_max <= kSegTrailShort;
+    t.agg = a.agg.as<uint32_t>();
+    t.c2f_ptr = a.c2f_ptr.as<int64_t>();
+    t.c2f_idx = a.c2f_idx.as<int32_t>();
+    t.rhs = a.rhs.ptr;
+    t.sol = a.sol.ptr;
+    t.tmp = a.tmp.ptr;
+    t.aux = a.aux.ptr;
+    tl.push_back(t);
+  }
+  amg->tail.alloc(tl.size() * sizeof(TailLevel));
+  LGPU_HIP_CHECK(hipMemcpy(amg->tail.ptr, tl.data(), tl.size() * sizeof(TailLevel), hipMemcpyHostToDevice));
+  amg->tail_first = t0;
+  amg->tail_count = int(tl.size());
+  amg->tail_waves = waves;
+  amg->tail_wave_lds = stride;
+}
+
 // Uploads the host hierarchy amg->host (levels, transfers, coarse matrices
 // and the coarsest factor) and allocates the level vectors.
 void upload_hierarchy(lgpu_amg amg) {
@@ -1423,6 +1488,7 @@ void upload_hierarchy(lgpu_amg amg) {
     amg->r32.alloc(size_t(h.sizes[0]) * sizeof(float));
     amg->z32.alloc(size_t(h.sizes[0]) * sizeof(float));
   }
+  build_tail(amg);
   // the finest smoother's cached weights, built with the hierarchy
   amg->finest->sor_weights(amg->f32, true);
   LGPU_HIP_CHECK(hipDeviceSynchronize());

@@ -185,4 +185,60 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t hw, int64_t total) {
   return (hw % kXcd) * per + hw / kXcd;
 }
 
+// Dense direct solves of the coarsest level by one block (any number of
+// threads): `s` is n doubles of LDS.  Cholesky with the dense lower factor,
+// forward substitution column by column (for every row i the updates
+// s_i -= L_ij aux_j happen for j = 0, 1, ... as in the row-oriented
+// substitution of direct/substitution.hpp:27-52; zero entries leave s_i
+// unchanged) and the transposed backward substitution, column oriented in
+// the reference as well (direct/substitution.hpp:82-113).
+template <typename T>
+__device__ __forceinline__ void block_chol_solve(const double* L, int64_t n, const T* b, T* x,
+                                                 double* s) {
+  const int64_t tid = threadIdx.x, nt = blockDim.x;
+  for (int64_t i = tid; i < n; i += nt) s[i] = double(b[i]);
+  __syncthreads();
+  for (int64_t j = 0; j < n; ++j) {
+    const double aj = s[j] / L[j * n + j];
+    __syncthreads();
+    if (tid == 0) s[j] = aj;
+    for (int64_t i = j + 1 + tid; i < n; i += nt) s[i] = s[i] - L[i * n + j] * aj;
+    __syncthreads();
+  }
+  // s holds aux; backward: sol = aux; for j = n-1..0: sol_j /= L_jj; sol_i -= L_ji sol_j (i < j)
+  for (int64_t j = n - 1; j >= 0; --j) {
+    const double sj = s[j] / L[j * n + j];
+    __syncthreads();
+    if (tid == 0) s[j] = sj;
+    for (int64_t i = tid; i < j; i += nt) s[i] = s[i] - L[j * n + i] * sj;
+    __syncthreads();
+  }
+  for (int64_t i = tid; i < n; i += nt) x[i] = T(s[i]);
+}
+
+// Dense LU solve with a unit lower factor L and an upper factor U (f64, row
+// major), column by column: forward, x_j is final once reached and is
+// subtracted from the rows below; backward, x_j = s_j / u_jj is subtracted
+// from the rows above (the oracle's order).
+template <typename T>
+__device__ __forceinline__ void block_lu_solve(const double* L, const double* U, int64_t n, const T* b,
+                                               T* x, double* s) {
+  const int64_t tid = threadIdx.x, nt = blockDim.x;
+  for (int64_t i = tid; i < n; i += nt) s[i] = double(b[i]);
+  __syncthreads();
+  for (int64_t j = 0; j < n; ++j) {
+    const double xj = s[j];
+    for (int64_t i = j + 1 + tid; i < n; i += nt) s[i] = s[i] - L[i * n + j] * xj;
+    __syncthreads();
+  }
+  for (int64_t j = n - 1; j >= 0; --j) {
+    const double xj = s[j] / U[j * n + j];
+    __syncthreads();
+    if (tid == 0) s[j] = xj;
+    for (int64_t i = tid; i < j; i += nt) s[i] = s[i] - U[i * n + j] * xj;
+    __syncthreads();
+  }
+  for (int64_t i = tid; i < n; i += nt) x[i] = T(s[i]);
+}
+
 }  // namespace lgpu

@@ -188,6 +188,44 @@ void stencil_sor_weights(const StencilDesc& s, bool f32, void* w, hipStream_t st
 int64_t csr_partials(const CsrDesc& a);
 void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y, double omega,
                double* partials, hipStream_t st);
+// One level of the multigrid cycle's tail, the small levels that
+// amg_tail runs in one workgroup: the level operator (CSR, values of the
+// cycle's precision) and vectors, its 64-row segment sweeps' staging size and
+// register-list variant, and the transfers to the next level.  The last
+// level of a tail is the coarsest (n, rhs and sol only).
+struct TailLevel {
+  int64_t n;
+  const int64_t* rowptr;
+  const int32_t* cols;
+  const void* vals;
+  int64_t cap;         // entries of the level's largest 64-row segment
+  int shortlists;      // rows fit the short register lists
+  const uint32_t* agg; // this level -> next (prolongation)
+  const int64_t* c2f_ptr;  // next level's coarse -> fine lists (restriction into it)
+  const int32_t* c2f_idx;
+  void* rhs;
+  void* sol;
+  void* tmp;
+  void* aux;
+};
+// register-list sizes of the 64-row segment sweeps' short variant (new-value
+// and trailing entries per row)
+constexpr int kSegNewShort = 8, kSegTrailShort = 16;
+// the tail's levels have at most kTailMaxRows rows; its workgroup at most
+// kTailMaxWaves wavefronts
+constexpr int64_t kTailMaxRows = 512;
+constexpr int kTailMaxWaves = 8;
+// bytes of LDS one wavefront of amg_tail needs for a level's segments
+size_t tail_wave_lds_bytes(int64_t cap, bool f32);
+// The V-cycle below the tail's first level (whose rhs is set): down sweeps,
+// residuals and restrictions, the coarsest solve (Cholesky, or LU with U),
+// prolongations (scaled by inv_relax) and backward sweeps, all in one
+// workgroup of `waves` wavefronts with `wave_lds` bytes of LDS each.
+void amg_tail(const TailLevel* levels, int count, int64_t coarsest_n, const double* L,
+              const double* U, bool f32, bool ultra, double inv_relax, int waves, size_t wave_lds,
+              hipStream_t st);
+size_t tail_lds_budget();
+
 // the longest segment a scheduled sweep keeps in one workgroup's LDS (its new
 // values, f64)
 constexpr int64_t kSchedMaxRows = 8192;

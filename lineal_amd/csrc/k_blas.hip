@@ -202,61 +202,19 @@ __global__ void prolong_kernel(const T* x, const T* xc, const uint32_t* agg, int
   }
 }
 
-// Coarsest solve with a dense copy of the lower Cholesky factor, one block.
-// Forward substitution is evaluated column by column: for every row i the
-// updates s_i -= L_ij aux_j happen for j = 0, 1, ... in the same order as the
-// row-oriented substitution of direct/substitution.hpp:27-52 (zero entries of
-// the dense copy leave s_i unchanged); the transposed backward substitution is
-// column oriented in the reference as well (direct/substitution.hpp:82-113).
+// Coarsest solves, one block (block_chol_solve / block_lu_solve)
 template <typename T>
 __global__ __launch_bounds__(NT) void chol_solve_kernel(const double* L, int64_t n, const T* b,
                                                         T* x) {
   extern __shared__ double sh[];
-  double* s = sh;  // n entries
-  for (int64_t i = threadIdx.x; i < n; i += NT) s[i] = double(b[i]);
-  __syncthreads();
-  for (int64_t j = 0; j < n; ++j) {
-    const double aj = s[j] / L[j * n + j];
-    __syncthreads();
-    if (threadIdx.x == 0) s[j] = aj;
-    for (int64_t i = j + 1 + threadIdx.x; i < n; i += NT) s[i] = s[i] - L[i * n + j] * aj;
-    __syncthreads();
-  }
-  // s holds aux; backward: sol = aux; for j = n-1..0: sol_j /= L_jj; sol_i -= L_ji sol_j (i < j)
-  for (int64_t j = n - 1; j >= 0; --j) {
-    const double sj = s[j] / L[j * n + j];
-    __syncthreads();
-    if (threadIdx.x == 0) s[j] = sj;
-    for (int64_t i = threadIdx.x; i < j; i += NT) s[i] = s[i] - L[j * n + i] * sj;
-    __syncthreads();
-  }
-  for (int64_t i = threadIdx.x; i < n; i += NT) x[i] = T(s[i]);
+  block_chol_solve(L, n, b, x, sh);
 }
 
-// Dense LU solve with a unit lower factor L and an upper factor U (f64, row
-// major), column by column in one block: forward, x_j is final once reached
-// and is subtracted from the rows below; backward, x_j = s_j / u_jj is
-// subtracted from the rows above (the oracle's order).
 template <typename T>
 __global__ __launch_bounds__(NT) void lu_solve_kernel(const double* L, const double* U, int64_t n,
                                                       const T* b, T* x) {
   extern __shared__ double sh[];
-  double* s = sh;  // n entries
-  for (int64_t i = threadIdx.x; i < n; i += NT) s[i] = double(b[i]);
-  __syncthreads();
-  for (int64_t j = 0; j < n; ++j) {
-    const double xj = s[j];
-    for (int64_t i = j + 1 + threadIdx.x; i < n; i += NT) s[i] = s[i] - L[i * n + j] * xj;
-    __syncthreads();
-  }
-  for (int64_t j = n - 1; j >= 0; --j) {
-    const double xj = s[j] / U[j * n + j];
-    __syncthreads();
-    if (threadIdx.x == 0) s[j] = xj;
-    for (int64_t i = threadIdx.x; i < j; i += NT) s[i] = s[i] - U[i * n + j] * xj;
-    __syncthreads();
-  }
-  for (int64_t i = threadIdx.x; i < n; i += NT) x[i] = T(s[i]);
+  block_lu_solve(L, U, n, b, x, sh);
 }
 
 }  // namespace

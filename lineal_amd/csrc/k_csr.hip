@@ -380,7 +380,7 @@ size_t wave_lds_bytes(int64_t entries) {
 // list sizes (multiples of 4) of the two variants: a level whose rows all fit
 // the short lists runs a kernel with fewer registers, so more wavefronts
 // share a SIMD
-constexpr int kNewShort = 8, kTrailShort = 16;
+constexpr int kNewShort = kSegNewShort, kTrailShort = kSegTrailShort;
 constexpr int kNewLong = 16, kTrailLong = 24;
 
 __device__ __forceinline__ int wave_max_int(int v) {
@@ -398,7 +398,7 @@ __device__ __forceinline__ bool wave_segment_registers(const StageView<T, V>& st
                                                        const T* __restrict__ b, T omega) {
   using S = Staged<T, V>;
   static_assert(NM % 4 == 0 && TM % 4 == 0, "lists run in groups of four");
-  const int lane = threadIdx.x;
+  const int lane = int(threadIdx.x % WSEG);
   const int64_t i = s0 + lane;
   const bool live = i < s1;
   T acc = T(0), d = T(1), div = T(0), xi = T(0);
@@ -535,20 +535,22 @@ __device__ __forceinline__ bool wave_segment_registers(const StageView<T, V>& st
   return true;
 }
 
+// The sweep of one 64-row segment [s0, s0 + 64) by one wavefront, with its
+// staging area at `lds` (wave_lds_bytes(cap)); the wavefronts of a block may
+// sweep different segments, each in its own area.
 template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, bool STAGED, bool SHORT>
-__global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
-    int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
-    const V* __restrict__ va, const T* __restrict__ xin, T* __restrict__ xout,
-    const T* __restrict__ b, T omega, int64_t cap) {
+__device__ __forceinline__ void wave_sweep_segment(int64_t s0, int64_t n, const int64_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ ci,
+                                                   const V* __restrict__ va, const T* __restrict__ xin,
+                                                   T* __restrict__ xout, const T* __restrict__ b, T omega,
+                                                   int64_t cap, double* lds) {
   using S = Staged<T, V>;
-  extern __shared__ double wave_lds[];
   const int64_t scap = STAGED ? stage_cap(cap) : 0;
-  V* const svals = reinterpret_cast<V*>(wave_lds);
+  V* const svals = reinterpret_cast<V*>(lds);
   T* const solds = reinterpret_cast<T*>(svals + scap);
   T* const s_new = solds + scap;
   const StageView<T, V> st{svals, solds, reinterpret_cast<int32_t*>(s_new + WSEG)};
-  const int lane = threadIdx.x;
-  const int64_t s0 = blockIdx.x * int64_t(WSEG);
+  const int lane = int(threadIdx.x % WSEG);
   const int64_t s1 = min(s0 + WSEG, n);
   const int64_t e0 = rp[s0], e1 = rp[s1];
   if constexpr (STAGED) {
@@ -568,7 +570,9 @@ __global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
         if (kb + u * WSEG < e1) st.put(kb + u * WSEG - e0, e[u]);
     }
   }
-  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if constexpr (STAGED) {
     constexpr int NM = SHORT ? kNewShort : kNewLong, TM = SHORT ? kTrailShort : kTrailLong;
     if (wave_segment_registers<T, V, FWD, ULTRA, HAS_X, NM, TM>(st, s_new, s0, s1, e0, rp, xin,
@@ -1093,6 +1097,16 @@ void dispatch_sched(const CsrDesc& a, const void* x_in, void* x_out, const void*
   }
 }
 
+template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, bool STAGED, bool SHORT>
+__global__ __launch_bounds__(WSEG) void csr_sor_wave_kernel(
+    int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const V* __restrict__ va, const T* __restrict__ xin, T* __restrict__ xout,
+    const T* __restrict__ b, T omega, int64_t cap) {
+  extern __shared__ double wave_lds[];
+  wave_sweep_segment<T, V, FWD, ULTRA, HAS_X, STAGED, SHORT>(blockIdx.x * int64_t(WSEG), n, rp, ci, va,
+                                                           xin, xout, b, omega, cap, wave_lds);
+}
+
 template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X>
 void launch_wave_segment2(const CsrDesc& a, const void* x_in, void* x_out, const void* b,
                           double omega, hipStream_t st) {
@@ -1166,6 +1180,107 @@ void dispatch_segment(const CsrDesc& a, const void* x_in, void* x_out, const voi
   } else {
     if (ultra) launch_segment<T, V, false, true>(a, x_in, x_out, b, omega, seg, st);
     else launch_segment<T, V, false, false>(a, x_in, x_out, b, omega, seg, st);
+  }
+}
+
+// The cycle's tail in one workgroup (amg_tail): every phase of the small
+// levels' V-cycle in turn, with a barrier between phases, so the tail costs
+// one launch instead of five per level.  Each phase repeats the standalone
+// kernels' arithmetic: the 64-row segment sweeps (wave_sweep_segment, one
+// segment per wavefront at a time, each wavefront in its own LDS area), the
+// residual (csr_staged_kernel's row order), the restriction sums in list
+// order, the dense coarsest solve and the prolongation.
+template <typename T, bool FWD, bool ULTRA, bool HAS_X>
+__device__ __forceinline__ void tail_sweep(const TailLevel& a, const T* xin, T* xout, const T* b,
+                                           double* my_lds) {
+  const int wid = int(threadIdx.x / WSEG), nw = int(blockDim.x / WSEG);
+  const int64_t nseg = (a.n + WSEG - 1) / WSEG;
+  const auto va = static_cast<const T*>(a.vals);
+  for (int64_t sg = wid; sg < nseg; sg += nw) {
+    if (a.shortlists)
+      wave_sweep_segment<T, T, FWD, ULTRA, HAS_X, true, true>(sg * WSEG, a.n, a.rowptr, a.cols, va, xin,
+                                                             xout, b, T(1), a.cap, my_lds);
+    else
+      wave_sweep_segment<T, T, FWD, ULTRA, HAS_X, true, false>(sg * WSEG, a.n, a.rowptr, a.cols, va, xin,
+                                                              xout, b, T(1), a.cap, my_lds);
+  }
+}
+
+template <typename T, bool ULTRA>
+__global__ __launch_bounds__(512) void amg_tail_kernel(const TailLevel* __restrict__ lv, int count,
+                                                       int64_t coarsest_n, const double* L,
+                                                       const double* U, T inv_relax, int64_t wave_lds) {
+  extern __shared__ double tail_lds[];
+  double* const my_lds = tail_lds + int64_t(threadIdx.x / WSEG) * (wave_lds / int64_t(sizeof(double)));
+  const int64_t tid = threadIdx.x, nt = blockDim.x;
+  for (int t = 0; t + 1 < count; ++t) {
+    const TailLevel a = lv[t];
+    T* const x = static_cast<T*>(a.sol);
+    const T* const b = static_cast<const T*>(a.rhs);
+    tail_sweep<T, true, ULTRA, false>(a, nullptr, x, b, my_lds);
+    __syncthreads();
+    // residual (sm_resid: the row's products summed with fma in column
+    // order), the entries in batches whose loads are all in flight together
+    T* const r = static_cast<T*>(a.tmp);
+    const T* const va = static_cast<const T*>(a.vals);
+    for (int64_t i = tid; i < a.n; i += nt) {
+      const int64_t k0 = a.rowptr[i], k1 = a.rowptr[i + 1];
+      T sum = T(0);
+      for (int64_t kb = k0; kb < k1; kb += kStageBatch) {
+        int32_t j[kStageBatch];
+        T v[kStageBatch], xv[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+          const int64_t k = kb + u < k1 ? kb + u : k1 - 1;
+          j[u] = a.cols[k];
+          v[u] = va[k];
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) xv[u] = x[j[u]];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u)
+          if (kb + u < k1) sum = fma(v[u], xv[u], sum);
+      }
+      r[i] = b[i] - sum;
+    }
+    __syncthreads();
+    // restriction into the next level: each aggregate in list order
+    const TailLevel nx = lv[t + 1];
+    T* const bc = static_cast<T*>(nx.rhs);
+    for (int64_t c = tid; c < nx.n; c += nt) {
+      const int64_t k0 = a.c2f_ptr[c], k1 = a.c2f_ptr[c + 1];
+      T sum = T(0);
+      for (int64_t kb = k0; kb < k1; kb += kStageBatch) {
+        T v[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) v[u] = r[a.c2f_idx[kb + u < k1 ? kb + u : k1 - 1]];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u)
+          if (kb + u < k1) sum += v[u];
+      }
+      bc[c] = sum;
+    }
+    __syncthreads();
+  }
+  {
+    const TailLevel c = lv[count - 1];
+    if (U) block_lu_solve(L, U, coarsest_n, static_cast<const T*>(c.rhs), static_cast<T*>(c.sol), tail_lds);
+    else block_chol_solve(L, coarsest_n, static_cast<const T*>(c.rhs), static_cast<T*>(c.sol), tail_lds);
+    __syncthreads();
+  }
+  for (int t = count - 2; t >= 0; --t) {
+    const TailLevel a = lv[t];
+    const T* const xc = static_cast<const T*>(lv[t + 1].sol);
+    T* const x = static_cast<T*>(a.sol);
+    T* const aux = static_cast<T*>(a.aux);
+    for (int64_t i = tid; i < a.n; i += nt) {
+      const uint32_t g = a.agg[i];
+      const T refined = g < 0xFFFFFFFEu ? xc[g] : T(0);
+      aux[i] = x[i] + refined * inv_relax;
+    }
+    __syncthreads();
+    tail_sweep<T, false, ULTRA, true>(a, aux, x, static_cast<const T*>(a.rhs), my_lds);
+    __syncthreads();
   }
 }
 
@@ -1252,6 +1367,29 @@ void csr_sor_segmented(const CsrDesc& a, const void* x_in, void* x_out, const vo
   if (a.n == 0) return;
   if (a.f32) dispatch_segment<float, float>(a, x_in, x_out, b, omega, forward, ultra, seg, st);
   else dispatch_segment<double, double>(a, x_in, x_out, b, omega, forward, ultra, seg, st);
+}
+
+size_t tail_wave_lds_bytes(int64_t cap, bool f32) {
+  const size_t b = f32 ? wave_lds_bytes<float, float>(cap) : wave_lds_bytes<double, double>(cap);
+  return (b + 15) & ~size_t(15);
+}
+
+size_t tail_lds_budget() { return wave_lds_budget(); }
+
+void amg_tail(const TailLevel* levels, int count, int64_t coarsest_n, const double* L,
+              const double* U, bool f32, bool ultra, double inv_relax, int waves, size_t wave_lds,
+              hipStream_t st) {
+  LGPU_REQUIRE(count >= 2 && waves >= 1 && waves <= kTailMaxWaves, "invalid cycle tail");
+  const size_t lds = std::max(size_t(waves) * wave_lds, size_t(coarsest_n) * sizeof(double));
+  const dim3 grid(1), block(unsigned(waves * WSEG));
+  if (f32) {
+    if (ultra) hipLaunchKernelGGL((amg_tail_kernel<float, true>), grid, block, lds, st, levels, count, coarsest_n, L, U, float(inv_relax), int64_t(wave_lds));
+    else hipLaunchKernelGGL((amg_tail_kernel<float, false>), grid, block, lds, st, levels, count, coarsest_n, L, U, float(inv_relax), int64_t(wave_lds));
+  } else {
+    if (ultra) hipLaunchKernelGGL((amg_tail_kernel<double, true>), grid, block, lds, st, levels, count, coarsest_n, L, U, inv_relax, int64_t(wave_lds));
+    else hipLaunchKernelGGL((amg_tail_kernel<double, false>), grid, block, lds, st, levels, count, coarsest_n, L, U, inv_relax, int64_t(wave_lds));
+  }
+  LGPU_HIP_CHECK(hipGetLastError());
 }
 
 int sched_positions(int64_t n, const int64_t* rowptr, const int32_t* cols, int64_t seg,
