@@ -115,6 +115,12 @@ def summary(args):
             lvl = L - 1
             going_up = True
             level = L - 1
+        elif base == "amg_tail_kernel":
+            # the levels after the last one swept down, and the coarsest, in
+            # one launch: the next prolongation is the last down level's
+            lvl = "tail"
+            going_up = True
+            level = (level or 0) + 1
         elif base == "prolong_kernel":
             level = (level if level is not None else L - 1) - 1
             lvl = level
@@ -140,6 +146,8 @@ def summary(args):
             kind, mode = int(targs[1]), int(targs[3])
             c = 1 if kind == 1 else E
             return (c + (2 if mode in (0, 2) else 3) * E) * N
+        if lvl == "tail":
+            return None
         if lvl is None:
             return {"cg_update_kernel": 52 * N, "xpby_scal_kernel": (4 if E == 4 else 8) * N + 16 * N,
                     "dot_mixed_kernel": 12 * N, "dot_kernel": 2 * E * N, "axpy_scal_kernel": 24 * N,
@@ -171,7 +179,12 @@ def summary(args):
     print("| kernel | level (rows) | calls | ms per solve | us per call | MB per call | GB/s | of 8 TB/s |")
     print("|---|---|---:|---:|---:|---:|---:|---:|")
     for ns, k, lvl, calls, avg_us, by, gbs in rows:
-        where = "finest / Krylov" if lvl is None else f"{lvl} ({lv[lvl]['n']})"
+        if lvl is None:
+            where = "finest / Krylov"
+        elif lvl == "tail":
+            where = "tail (the small levels and the coarsest)"
+        else:
+            where = f"{lvl} ({lv[lvl]['n']})"
         mb = f"{by / 1e6:.1f}" if by else "-"
         g = f"{gbs:.0f}" if gbs else "-"
         frac = f"{gbs / HBM_PEAK_GBS * 100:.1f}%" if gbs else "-"
