@@ -24,6 +24,7 @@ constexpr int SPIN_LIMIT = 1 << 22;
 // together, so staging costs one memory round trip (two with the gathers)
 // per batch rather than per entry
 constexpr int kStageBatch = 8;
+constexpr int kWaveStageBatch = 16;
 
 template <typename T, typename V, int MODE>
 __global__ __launch_bounds__(NT) void csr_kernel(int64_t n, const int64_t* __restrict__ rp,
@@ -554,19 +555,21 @@ __device__ __forceinline__ void wave_sweep_segment(int64_t s0, int64_t n, const 
   const int64_t s1 = min(s0 + WSEG, n);
   const int64_t e0 = rp[s0], e1 = rp[s1];
   if constexpr (STAGED) {
-    // batches of kStageBatch entries per lane (see csr_staged_kernel)
-    for (int64_t kb = e0 + lane; kb < e1; kb += kStageBatch * WSEG) {
-      S e[kStageBatch];
+    // batches of kWaveStageBatch entries per lane (see csr_staged_kernel): a
+    // segment's ~1000 entries in one or two batches
+    constexpr int B = kWaveStageBatch;
+    for (int64_t kb = e0 + lane; kb < e1; kb += B * WSEG) {
+      S e[B];
 #pragma unroll
-      for (int u = 0; u < kStageBatch; ++u) {
+      for (int u = 0; u < B; ++u) {
         const int64_t k = kb + u * WSEG < e1 ? kb + u * WSEG : e1 - 1;
         e[u].col = ci[k];
         e[u].val = va[k];
       }
 #pragma unroll
-      for (int u = 0; u < kStageBatch; ++u) e[u].old = HAS_X ? xin[e[u].col] : T(0);
+      for (int u = 0; u < B; ++u) e[u].old = HAS_X ? xin[e[u].col] : T(0);
 #pragma unroll
-      for (int u = 0; u < kStageBatch; ++u)
+      for (int u = 0; u < B; ++u)
         if (kb + u * WSEG < e1) st.put(kb + u * WSEG - e0, e[u]);
     }
   }
