@@ -100,6 +100,10 @@ def lib():
             "or_stencil_row_f32": (C.c_int, [sp, C.c_int64, i64p, f32p]),
             "or_stencil_rhs_f64": (None, [sp, f64p]),
             "or_stencil_rhs_f32": (None, [sp, f32p]),
+            "or_spmv_banded_f64": (None, [sp, f64p, f64p, C.c_int]),
+            "or_spmv_banded_f32": (None, [sp, f32p, f32p, C.c_int]),
+            "or_sor_banded_f64": (None, [sp, f64p, f64p, f64p, C.c_double, C.c_int, C.c_int]),
+            "or_sor_banded_f32": (None, [sp, f32p, f32p, f32p, C.c_float, C.c_int, C.c_int]),
             "or_spmv_f64": (None, [mp, f64p, f64p]),
             "or_spmv_f32": (None, [mp, f32p, f32p]),
             "or_residual_f64": (None, [mp, f64p, f64p, f64p]),
@@ -281,6 +285,26 @@ def spmv(A, x):
     y = np.zeros_like(x)
     _fn("or_spmv", x.dtype)(C.byref(A.mat), _pf(x), _pf(y))
     return y
+
+
+def spmv_banded(S, x, fused=True):
+    """y = A x in the reference's vectorised row order (banded_iterate):
+    -x, +x, -y, +y, -z, +z, the diagonal last; fused or separately rounded
+    multiply-adds (a compiler may contract `sum += a * x` or not)."""
+    y = np.zeros_like(x)
+    _fn("or_spmv_banded", x.dtype)(C.byref(S.s), _pf(x), _pf(y), int(fused))
+    return y
+
+
+def sor_banded(S, x_in, b, omega=1.0, forward=True, ultra=False):
+    """One lexicographic SOR sweep in the reference's vectorised form
+    (compute_base_simd: w / d formed first, the z-recurrence x = add - factor
+    x_prev)."""
+    x_out = np.zeros_like(b)
+    xin = None if x_in is None else np.ascontiguousarray(x_in, b.dtype)
+    _fn("or_sor_banded", b.dtype)(C.byref(S.s), _pf(xin), _pf(x_out), _pf(b), omega, int(forward),
+                                  int(ultra))
+    return x_out
 
 
 def residual(A, b, x):

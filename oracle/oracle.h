@@ -77,6 +77,14 @@ void or_stencil_rhs_f64(const or_stencil* s, double* b);
 void or_stencil_rhs_f32(const or_stencil* s, float* b);
 
 /* y = A x */
+/* the reference's vectorised row order (banded_iterate): y = A x with fused
+ * or separately rounded steps, and one-segment SOR (compute_base_simd) */
+void or_spmv_banded_f64(const or_stencil* s, const double* x, double* y, int fused);
+void or_spmv_banded_f32(const or_stencil* s, const float* x, float* y, int fused);
+void or_sor_banded_f64(const or_stencil* s, const double* x_in, double* x_out, const double* b,
+                       double w, int forward, int ultra);
+void or_sor_banded_f32(const or_stencil* s, const float* x_in, float* x_out, const float* b,
+                       float w, int forward, int ultra);
 void or_spmv_f64(const or_mat* a, const double* x, double* y);
 void or_spmv_f32(const or_mat* a, const float* x, float* y);
 /* r = b - A x */

@@ -261,3 +261,34 @@ def test_advection_kernels(dims, kind, dtype):
                     out = A.sor(xd if with_x else None, la.DenseVector(n, dtype), la.DenseVector(b), 1.1,
                                 forward, ultra)
                     same(out.numpy(), O.sor(S, x if with_x else None, b, dtype(1.1), forward, ultra))
+
+
+@pytest.mark.parametrize("dims", [(33, 17, 65), (64, 64, 64)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("velocity", [0.0, 0.3])
+def test_kernels_within_rounding_of_the_vectorised_order(dims, dtype, velocity):
+    """The reference's vectorised build accumulates each row in the order
+    -x,+x,-y,+y,-z,+z, diagonal last (its SOR through a recurrence along z);
+    the kernels follow its scalar order.  The two differ by rounding only:
+    SpMV by at most 16 eps of the row's magnitude sum |a_ij x_j| + |y_i|,
+    one SOR sweep by at most 64 eps of the largest value."""
+    box = la.BoxInfo(dims, p_start=1.0, p_end=0.0, velocity_x=velocity)
+    c = problems.lognormal(dims, 1.0, 7)
+    A = la.StencilMatrix.values(box, c)
+    S = O.Stencil(dims, coeff=c, p_start=1.0, p_end=0.0, velocity=velocity)
+    n = S.N
+    eps = np.finfo(dtype).eps
+    x = problems.uniform(n, 8, dtype)
+    b = problems.uniform(n, 9, dtype)
+    y = A.spmv(la.DenseVector(x), la.DenseVector(n, dtype)).numpy().astype(np.float64)
+    C = S.to_csr()[0]
+    mag = O.spmv(O.Csr(C.rowptr, C.cols, np.abs(C.vals), symmetric=False), np.abs(x).astype(np.float64))
+    for fused in (True, False):
+        d = np.abs(O.spmv_banded(S, x, fused).astype(np.float64) - y)
+        assert np.all(d <= 16 * eps * (mag + np.abs(y))), f"fused={fused}: {d.max()}"
+    for forward in (True, False):
+        for ultra in (False, True):
+            out = A.sor(la.DenseVector(x), la.DenseVector(n, dtype), la.DenseVector(b), 1.0, forward,
+                        ultra).numpy().astype(np.float64)
+            ref = O.sor_banded(S, x, b, dtype(1.0), forward, ultra).astype(np.float64)
+            assert np.abs(out - ref).max() <= 64 * eps * np.abs(ref).max(), f"forward={forward} ultra={ultra}"

@@ -1192,3 +1192,139 @@ def test_bcsr_spmv_against_dense(bsz):
             D32[i * bsz:(i + 1) * bsz, cols[p] * bsz:(cols[p] + 1) * bsz] += v32[p]
     np.testing.assert_allclose(O.bcsr_spmv(rp, cols, vals, x32), (D32 @ x32.astype(np.float64)).astype(np.float32),
                                rtol=1e-6, atol=1e-6)
+
+
+# ---- the reference's vectorised row order (banded_iterate, compute_base_simd)
+
+def _fma(a, b, c):
+    """a * b + c rounded once (exact rational arithmetic, then one rounding)."""
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _banded_rows_dense(S):
+    """Each cell's six off-diagonals in the vectorised order -x,+x,-y,+y,-z,+z
+    (None for a missing neighbour) and its diagonal summed in that order with
+    the x-border summand 2 q_0 c_i at the place of the missing x neighbour;
+    no advection (its -x entry folds the velocity in before the negation).
+    Built from the scalar rows, whose off-diagonal values are -w."""
+    nx, ny, nz = S.dims
+    q0 = S.quotients()[0]
+    stride = {0: ny * nz, 1: nz, 2: 1}
+    rows = []
+    for i in range(S.N):
+        cols, vals = S.row(i)
+        w = {int(c): -float(v) for c, v in zip(cols, vals) if c != i}
+        pos = (i // (ny * nz), (i // nz) % ny, i % nz)
+        off, diag = [], 0.0
+        for d in range(3):
+            for side in (-1, 1):
+                j = i + side * stride[d]
+                inside = 0 <= pos[d] + side < S.dims[d]
+                if inside:
+                    off.append((j, -w[j]))
+                    diag = diag + w[j]
+                else:
+                    off.append(None)
+                    if d == 0:
+                        diag = diag + (2.0 * q0) * float(S.coeff[i])
+        if S.zero_to_one and diag == 0.0:
+            diag = 1.0
+        rows.append((off, diag))
+    return rows
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_spmv_banded_against_dense_restatement(fused):
+    """The vectorised-order product against a restatement from the scalar
+    rows: -x,+x,-y,+y,-z,+z then the diagonal, each step rounded once
+    (fused) or twice; anisotropic cells and ZeroToOne."""
+    dims = (5, 4, 6)
+    c = problems.lognormal(dims, 1.0, 11)
+    c[7] = 0.0
+    S = O.Stencil(dims, coeff=c, lengths=(1.0, 2.0, 0.5), zero_to_one=True)
+    x = problems.uniform(S.N, 12)
+    want = np.zeros(S.N)
+    for i, (off, diag) in enumerate(_banded_rows_dense(S)):
+        s = 0.0
+        for e in off:
+            if e is not None:
+                s = _fma(e[1], x[e[0]], s) if fused else s + e[1] * x[e[0]]
+        s = _fma(diag, x[i], s) if fused else s + diag * x[i]
+        want[i] = s
+    np.testing.assert_array_equal(O.spmv_banded(S, x, fused), want)
+
+
+@pytest.mark.parametrize("forward", [True, False])
+@pytest.mark.parametrize("ultra", [False, True])
+def test_sor_banded_against_dense_restatement(forward, ultra):
+    """compute_base_simd restated from the scalar rows: accum over the x/y
+    connections and the z-connection ahead (new values behind the sweep),
+    quot = w / d, add = fma(accum, quot, (1 - w) x_in) (ultra: x_in +
+    accum quot), x_i = fma(-nb quot, x_new(behind in z), add)."""
+    dims = (4, 5, 6)
+    S = O.Stencil(dims, coeff=problems.lognormal(dims, 1.0, 13), lengths=(2.0, 1.0, 3.0))
+    x_in = problems.uniform(S.N, 14)
+    b = problems.uniform(S.N, 15)
+    w = 1.3
+    rows = _banded_rows_dense(S)
+    out = np.zeros(S.N)
+    order = range(S.N) if forward else range(S.N - 1, -1, -1)
+    for i in order:
+        off, diag = rows[i]
+        acc, nb = b[i], 0.0
+        for k, e in enumerate(off):
+            behind = (k % 2 == 0) if forward else (k % 2 == 1)
+            if k // 2 == 2 and behind:
+                nb = 0.0 if e is None else e[1]
+                continue
+            if e is None:
+                continue
+            xj = out[e[0]] if behind else x_in[e[0]]
+            acc = _fma(-e[1], xj, acc)
+        if ultra:
+            acc = _fma(-diag, x_in[i], acc)
+        dv = abs(diag) if ultra else diag
+        quot = w / dv
+        add = x_in[i] + acc * quot if ultra else _fma(acc, quot, (1.0 - w) * x_in[i])
+        factor = nb * quot
+        z = i % dims[2]
+        has_prev = z > 0 if forward else z + 1 < dims[2]
+        prev = out[i - 1 if forward else i + 1] if has_prev else 0.0
+        out[i] = _fma(-factor, prev, add)
+    np.testing.assert_array_equal(O.sor_banded(S, x_in, b, w, forward, ultra), out)
+
+
+def test_banded_order_equals_scalar_on_exact_data():
+    """With unit coefficients and cells (every weight 1) and small integers,
+    every step is exact, so the vectorised and the scalar orders must agree
+    bit for bit: a dropped, doubled or misplaced term would not."""
+    dims = (5, 6, 7)
+    S = O.Stencil(dims, coeff=np.ones(int(np.prod(dims))), p_start=1.0, p_end=0.0)
+    x = problems.uniform(S.N, 16).round(0) * 4.0
+    for fused in (True, False):
+        np.testing.assert_array_equal(O.spmv_banded(S, x, fused), O.spmv(S, x))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_banded_order_within_rounding_of_scalar(dtype):
+    """The reference's vectorised results and its scalar ones (which the GPU
+    reproduces bit for bit) differ only by rounding: SpMV by a few units of
+    the row's magnitude, one SOR sweep relatively by a few ulps."""
+    dims = (24, 20, 16)
+    S = O.Stencil(dims, coeff=problems.lognormal(dims, 1.0, 17), velocity=0.3)
+    eps = np.finfo(dtype).eps
+    x = problems.uniform(S.N, 18, dtype)
+    b = problems.uniform(S.N, 19, dtype)
+    ref = O.spmv(S, x)
+    C = S.to_csr()[0]
+    absA = O.Csr(C.rowptr, C.cols, np.abs(C.vals), symmetric=False)
+    mag = O.spmv(absA, np.abs(x).astype(np.float64))
+    for fused in (True, False):
+        d = np.abs(O.spmv_banded(S, x, fused).astype(np.float64) - ref.astype(np.float64))
+        assert np.all(d <= 16 * eps * (mag + np.abs(ref.astype(np.float64))))
+    for forward in (True, False):
+        for ultra in (False, True):
+            s0 = O.sor(S, x, b, dtype(1.0), forward, ultra).astype(np.float64)
+            s1 = O.sor_banded(S, x, b, dtype(1.0), forward, ultra).astype(np.float64)
+            assert np.abs(s1 - s0).max() <= 64 * eps * np.abs(s0).max()
