@@ -111,6 +111,15 @@ __global__ __launch_bounds__(NT) void csr_staged_kernel(int64_t n, const int64_t
   const int64_t r0 = blockIdx.x * int64_t(NT);
   const int64_t r1 = min(r0 + NT, n);
   const int64_t e0 = rp[r0], e1 = rp[r1];
+  // the thread's row bounds and vector entries, requested before the staging
+  // so that their memory round trip overlaps it (rows past n read row r0)
+  const int64_t i = r0 + threadIdx.x;
+  const int64_t ir = i < n ? i : r0;
+  const int64_t rk0 = rp[ir], rk1 = rp[ir + 1];
+  constexpr bool NEED_B = MODE == SM_RESID || MODE == SM_RESID_NORM || MODE == SM_JACOBI;
+  constexpr bool NEED_X = MODE == SM_SPMV_DOT || MODE == SM_JACOBI;
+  const T bi = NEED_B ? b[ir] : T(0);
+  const T xi = NEED_X && x ? x[ir] : T(0);
   // batches of kStageBatch entries per thread: the column and value loads of a
   // batch, then its gathers, are all in flight together
   for (int64_t kb = e0 + threadIdx.x; kb < e1; kb += kStageBatch * NT) {
@@ -136,20 +145,19 @@ __global__ __launch_bounds__(NT) void csr_staged_kernel(int64_t n, const int64_t
     }
   }
   __syncthreads();
-  const int64_t i = r0 + threadIdx.x;
   double part = 0.0;
   if (i < n) {
-    const int64_t k0 = rp[i] - e0, k1 = rp[i + 1] - e0;
+    const int64_t k0 = rk0 - e0, k1 = rk1 - e0;
     if constexpr (MODE == SM_JACOBI) {
       // jacobi/sweep.hpp:66-87
-      T acc = b[i], d = T(1);
+      T acc = bi, d = T(1);
       for (int64_t k = k0; k < k1; ++k) {
         const T aij = T(sv[k]);
         if (sc[k] == i) d = aij;
         else acc = fma(-aij, sx[k], acc);
       }
       const T quot = acc / d;
-      y[i] = fma(omega, quot, (T(1) - omega) * (x ? x[i] : T(0)));
+      y[i] = fma(omega, quot, (T(1) - omega) * xi);
     } else {
       T sum = T(0);
 #pragma unroll 4
@@ -158,11 +166,11 @@ __global__ __launch_bounds__(NT) void csr_staged_kernel(int64_t n, const int64_t
         y[i] = sum;
       } else if constexpr (MODE == SM_SPMV_DOT) {
         y[i] = sum;
-        part = double(x[i]) * double(sum);
+        part = double(xi) * double(sum);
       } else if constexpr (MODE == SM_RESID) {
-        y[i] = b[i] - sum;
+        y[i] = bi - sum;
       } else {
-        const T r = b[i] - sum;
+        const T r = bi - sum;
         y[i] = r;
         part = double(r) * double(r);
       }
@@ -390,13 +398,19 @@ __device__ __forceinline__ int wave_max_int(int v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+// The lane's row bounds, right-hand side and old value, loaded before the
+// segment's staging so that their memory round trip overlaps it.
+template <typename T>
+struct RowHead {
+  int64_t k0, k1;
+  T b, x;
+};
+
 template <typename T, typename V, bool FWD, bool ULTRA, bool HAS_X, int NM, int TM>
 __device__ __forceinline__ bool wave_segment_registers(const StageView<T, V>& st, T* s_new,
                                                        int64_t s0, int64_t s1, int64_t e0,
-                                                       const int64_t* __restrict__ rp,
-                                                       const T* __restrict__ xin,
-                                                       T* __restrict__ xout,
-                                                       const T* __restrict__ b, T omega) {
+                                                       const RowHead<T>& rh,
+                                                       T* __restrict__ xout, T omega) {
   using S = Staged<T, V>;
   static_assert(NM % 4 == 0 && TM % 4 == 0, "lists run in groups of four");
   const int lane = int(threadIdx.x % WSEG);
@@ -418,10 +432,10 @@ __device__ __forceinline__ bool wave_segment_registers(const StageView<T, V>& st
     }
   };
   if (live) {
-    int64_t k = rp[i] - e0;
-    const int64_t kend = rp[i + 1] - e0;
-    acc = b[i];
-    xi = HAS_X ? xin[i] : T(0);
+    int64_t k = rh.k0 - e0;
+    const int64_t kend = rh.k1 - e0;
+    acc = rh.b;
+    xi = HAS_X ? rh.x : T(0);
     // leading old-value entries: forward the columns below s0, backward below i
     const int64_t lead = FWD ? s0 : i;
     while (k < kend) {
@@ -554,6 +568,9 @@ __device__ __forceinline__ void wave_sweep_segment(int64_t s0, int64_t n, const 
   const int lane = int(threadIdx.x % WSEG);
   const int64_t s1 = min(s0 + WSEG, n);
   const int64_t e0 = rp[s0], e1 = rp[s1];
+  const int64_t i = s0 + lane;
+  const int64_t ir = i < s1 ? i : s0;
+  const RowHead<T> rh{rp[ir], rp[ir + 1], b[ir], HAS_X ? xin[ir] : T(0)};
   if constexpr (STAGED) {
     // batches of kWaveStageBatch entries per lane (see csr_staged_kernel): a
     // segment's ~1000 entries in one or two batches
@@ -578,8 +595,8 @@ __device__ __forceinline__ void wave_sweep_segment(int64_t s0, int64_t n, const 
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if constexpr (STAGED) {
     constexpr int NM = SHORT ? kNewShort : kNewLong, TM = SHORT ? kTrailShort : kTrailLong;
-    if (wave_segment_registers<T, V, FWD, ULTRA, HAS_X, NM, TM>(st, s_new, s0, s1, e0, rp, xin,
-                                                                xout, b, omega))
+    if (wave_segment_registers<T, V, FWD, ULTRA, HAS_X, NM, TM>(st, s_new, s0, s1, e0, rh, xout,
+                                                                omega))
       return;
   }
   auto entry = [&](int64_t k) {
@@ -593,7 +610,6 @@ __device__ __forceinline__ void wave_sweep_segment(int64_t s0, int64_t n, const 
       return e;
     }
   };
-  const int64_t i = s0 + lane;
   bool finished = i >= s1;
   int64_t k = 0, kend = 0;
   T acc = T(0), d = T(1), div = T(0), xi = T(0);
@@ -619,10 +635,10 @@ __device__ __forceinline__ void wave_sweep_segment(int64_t s0, int64_t n, const 
     }
   };
   if (!finished) {
-    k = rp[i] - e0;
-    kend = rp[i + 1] - e0;
-    acc = b[i];
-    xi = HAS_X ? xin[i] : T(0);
+    k = rh.k0 - e0;
+    kend = rh.k1 - e0;
+    acc = rh.b;
+    xi = rh.x;
     if constexpr (FWD) {
       run_old(s0);
     } else {
