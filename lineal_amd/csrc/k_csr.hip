@@ -11,6 +11,7 @@
 // raises its own flag.  Dependencies only point to rows claimed earlier, so
 // every wait ends; waits are bounded and report a timeout on the error word.
 #include <algorithm>
+#include <type_traits>
 
 #include "device.hpp"
 
@@ -495,58 +496,52 @@ __device__ __forceinline__ bool wave_segment_registers(const StageView<T, V>& st
   // Unused list slots hold value 0 with old value 0 (trailing) or point at the
   // lane's own row, which reads 0 until the lane finishes (new values), and
   // fma(-0, 0, a) == a for every a: padded slots leave the accumulator's bits
-  // unchanged, so every lane runs the same chain.  Its length is the wave's
-  // longest list, rounded up to groups of four.
+  // unchanged, so every lane runs the same chain.  The wave runs the shortest
+  // of the chain lengths (4, 8), (8, 16), (NM, TM) its longest lists fit: a
+  // round is a fixed FMA chain with no per-group test (tested groups came out
+  // as every group evaluated plus a select each).
   s_new[lane] = T(0);
   const int nn_w = wave_max_int(nn), nt_w = wave_max_int(nt);
   const T m1w = T(1) - omega;
-  bool finished = !live;
-  uint64_t done = __ballot(finished);
-  while (done != ~uint64_t(0)) {
-    if (!finished && (deps & ~done) == 0) {
-      T nv[NM];
+  auto rounds = [&](auto NMC, auto TMC) {
+    constexpr int CN = decltype(NMC)::value, CT = decltype(TMC)::value;
+    bool finished = !live;
+    uint64_t done = __ballot(finished);
+    while (done != ~uint64_t(0)) {
+      if (!finished && (deps & ~done) == 0) {
+        T nv[CN];
 #pragma unroll
-      for (int c = 0; c < NM; c += 4) {
-        if (c < nn_w) {
+        for (int u = 0; u < CN; ++u) nv[u] = s_new[nidx[u]];
+        T a = acc;
 #pragma unroll
-          for (int u = c; u < c + 4; ++u) nv[u] = s_new[nidx[u]];
+        for (int u = 0; u < CN; ++u) a = fma(-T(nval[u]), nv[u], a);
+        if constexpr (FWD && ULTRA) {
+          if (has_diag) a = fma(-d, xi, a);
         }
-      }
-      T a = acc;
 #pragma unroll
-      for (int c = 0; c < NM; c += 4) {
-        if (c < nn_w) {
-#pragma unroll
-          for (int u = c; u < c + 4; ++u) a = fma(-T(nval[u]), nv[u], a);
+        for (int u = 0; u < CT; ++u) a = fma(-T(tval[u]), told[u], a);
+        T xn;
+        if (ULTRA) {
+          const T ad = d < T(0) ? -d : d;
+          xn = xi + (omega * a) / (ad + T(0.5) * div);
+        } else {
+          const T quot = a / d;
+          xn = fma(omega, quot, m1w * xi);
         }
+        s_new[lane] = xn;
+        xout[i] = xn;
+        finished = true;
       }
-      if constexpr (FWD && ULTRA) {
-        if (has_diag) a = fma(-d, xi, a);
-      }
-#pragma unroll
-      for (int c = 0; c < TM; c += 4) {
-        if (c < nt_w) {
-#pragma unroll
-          for (int u = c; u < c + 4; ++u) a = fma(-T(tval[u]), told[u], a);
-        }
-      }
-      T xn;
-      if (ULTRA) {
-        const T ad = d < T(0) ? -d : d;
-        xn = xi + (omega * a) / (ad + T(0.5) * div);
-      } else {
-        const T quot = a / d;
-        xn = fma(omega, quot, m1w * xi);
-      }
-      s_new[lane] = xn;
-      xout[i] = xn;
-      finished = true;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      done = __ballot(finished);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    done = __ballot(finished);
-  }
+  };
+  static_assert(NM >= 8 && TM >= 16, "the chain tiers need lists of at least 8 and 16 slots");
+  if (nn_w <= 4 && nt_w <= 8) rounds(std::integral_constant<int, 4>{}, std::integral_constant<int, 8>{});
+  else if (nn_w <= 8 && nt_w <= 16) rounds(std::integral_constant<int, 8>{}, std::integral_constant<int, 16>{});
+  else rounds(std::integral_constant<int, NM>{}, std::integral_constant<int, TM>{});
   return true;
 }
 
