@@ -81,17 +81,40 @@ __global__ __launch_bounds__(NT) void reduce_kernel(const double* part, int64_t 
 
 // CG: alpha = scal[num] / scal[den]; x += alpha p; r -= alpha q; partial += r^2
 // (conjugate-gradients/solver.hpp:84-95: sol + dir*alpha, res - prod*alpha)
-__global__ __launch_bounds__(NT) void cg_update_kernel(double* x, double* r, float* r32,
-                                                       const double* p, const double* q,
-                                                       const double* scal, int num, int den,
-                                                       int64_t n, double* part) {
+__global__ __launch_bounds__(NT) void cg_update_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                       float* __restrict__ r32,
+                                                       const double* __restrict__ p,
+                                                       const double* __restrict__ q,
+                                                       const double* __restrict__ scal, int num,
+                                                       int den, int64_t n, double* __restrict__ part) {
   const double alpha = scal[num] / scal[den];
   double s = 0;
-  for (int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * NT) {
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  int64_t i = blockIdx.x * int64_t(NT) + threadIdx.x;
+  // two elements per trip with all eight loads issued before the stores (the
+  // arrays are distinct); the partial sum still takes the thread's elements
+  // in order
+  for (; i + stride < n; i += 2 * stride) {
+    const int64_t j = i + stride;
+    const double xa = x[i], pa = p[i], ra = r[i], qa = q[i];
+    const double xb = x[j], pb = p[j], rb = r[j], qb = q[j];
+    x[i] = xa + pa * alpha;
+    x[j] = xb + pb * alpha;
+    const double ua = ra - qa * alpha, ub = rb - qb * alpha;
+    r[i] = ua;
+    r[j] = ub;
+    if (r32) {  // the FP32 V-cycle's input, as vec_convert rounds it
+      r32[i] = float(ua);
+      r32[j] = float(ub);
+    }
+    s += ua * ua;
+    s += ub * ub;
+  }
+  if (i < n) {
     x[i] = x[i] + p[i] * alpha;
     const double ri = r[i] - q[i] * alpha;
     r[i] = ri;
-    if (r32) r32[i] = float(ri);  // the FP32 V-cycle's input, as vec_convert rounds it
+    if (r32) r32[i] = float(ri);
     s += ri * ri;
   }
   const double t = block_sum<NT>(s);
