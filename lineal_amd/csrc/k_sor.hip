@@ -788,13 +788,14 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
   // A chunk's windows (steps t0 .. t0 + K - 1, logical z0 = t0 - a - b):
   //   rec [K][4]   the cells' records, in step order
   //   hx, hy [K]   -x / -y faces of the border lanes (halo)
-  //   xw [K + 1]   old values of the own line, z0 .. z0 + K
+  //   xw [K]       old values of the own line, z0 .. z0 + K - 1 (the value
+  //                at z0 + K is the next chunk's xw[0])
   //   exx, exy [K] old values of the +x / +y line beyond the column
   //   bw [K]       right-hand side
   struct Win {
     T rec[K][kRecVals];
     T hx[K], hy[K];
-    T xw[K + 1], exx[K], exy[K], bw[K];
+    T xw[K], exx[K], exy[K], bw[K];
   };
   auto load_elements = [&](Win& v, int32_t z0) {
 #pragma unroll
@@ -805,7 +806,7 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
       v.exy[j] = HAS_X ? rload<T>(rxi, give_y && has_sy ? boff(e + hsy, ST) : kOff) : T(0);
     }
 #pragma unroll
-    for (int j = 0; j < K + 1; ++j)
+    for (int j = 0; j < K; ++j)
       v.xw[j] = HAS_X ? rload<T>(rxi, boff(zorg + dz * (z0 + j), ST)) : T(0);
   };
   // chunk index c (0 .. nc - 1) -> its windows; the natural-layout windows
@@ -843,20 +844,19 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
     }
     const int32_t z0 = int32_t(t0) - a - b;
     const int32_t zk = FWD ? zorg + z0 : zorg - z0 - (K - 1);  // K elements
-    const int32_t z1 = FWD ? zorg + z0 : zorg - z0 - K;        // xw: K + 1
     auto out = [emax](int32_t lo, int n) { return lo < 0 || lo + n - 1 > emax; };
-    const bool bad = line && (out(zk, K) || (HAS_X && out(z1, K + 1)) ||
+    const bool bad = line && (out(zk, K) ||
                               (HAS_X && give_x && has_sx && out(zk + hsx, K)) ||
                               (HAS_X && give_y && has_sy && out(zk + hsy, K)));
     const bool ok = line && !bad;
     load_run<T, K, !FWD>(rbv, ok ? uint32_t(zk) * ST : kRunOff, v.bw);
     if constexpr (HAS_X) {
-      load_run<T, K + 1, !FWD>(rxi, ok ? uint32_t(z1) * ST : kRunOff, v.xw);
+      load_run<T, K, !FWD>(rxi, ok ? uint32_t(zk) * ST : kRunOff, v.xw);
       load_run<T, K, !FWD>(rxi, ok && give_x && has_sx ? uint32_t(zk + hsx) * ST : kRunOff, v.exx);
       load_run<T, K, !FWD>(rxi, ok && give_y && has_sy ? uint32_t(zk + hsy) * ST : kRunOff, v.exy);
     } else {
 #pragma unroll
-      for (int j = 0; j < K + 1; ++j) v.xw[j] = T(0);
+      for (int j = 0; j < K; ++j) v.xw[j] = T(0);
 #pragma unroll
       for (int j = 0; j < K; ++j) v.exx[j] = v.exy[j] = T(0);
     }
@@ -915,8 +915,11 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
       }
       // old values of the logical +x / +y neighbours (same logical z) and the
       // new values of the logical -x / -y neighbours (one step ago)
-      T xsx = __shfl(cur.xw[k + 1], lane + CB < 64 ? lane + CB : lane, 64);
-      T xsy = shift<kWaveShl1>(cur.xw[k + 1], cur.xw[k + 1]);
+      // the old value one step ahead in z: the next chunk's first at the
+      // chunk's last step
+      const T xz1 = k + 1 < K ? cur.xw[k + 1 < K ? k + 1 : 0] : nxt.xw[0];
+      T xsx = __shfl(xz1, lane + CB < 64 ? lane + CB : lane, 64);
+      T xsy = shift<kWaveShl1>(xz1, xz1);
       const T npx = __shfl(prev, lane >= CB ? lane - CB : lane, 64);
       const T npy = shift<kWaveShr1>(prev, prev);
       if (give_x) xsx = cur.exx[k];
@@ -933,7 +936,7 @@ __global__ __launch_bounds__(64) void sor_wavefront_r(StencilDesc s, RecGeom g,
       const T n_pz = has_pz ? prev : T(0);
       const T o_sx = has_sx ? xsx : T(0);
       const T o_sy = has_sy ? xsy : T(0);
-      const T o_sz = has_sz ? cur.xw[k + 1] : T(0);
+      const T o_sz = has_sz ? xz1 : T(0);
       const T vmx = FWD ? n_px : o_sx, vpx = FWD ? o_sx : n_px;
       const T vmy = FWD ? n_py : o_sy, vpy = FWD ? o_sy : n_py;
       const T vmz = FWD ? n_pz : o_sz, vpz = FWD ? o_sz : n_pz;
