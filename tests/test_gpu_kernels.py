@@ -202,6 +202,54 @@ def test_csr_segment_sor_register_and_fallback_paths(long_rows, dtype):
                 same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.1), forward, ultra, bounds))
 
 
+def tiered_csr(seed):
+    """Five 64-row segments whose rows need, in turn, at most (new-value,
+    trailing) entries of (1, 1), (4, 8), (8, 16), (16, 24) and more: each
+    wave of the segment sweep runs a different chain length (the last one the
+    LDS walk).  Rows have old-value entries before the segment too."""
+    rng = np.random.default_rng(seed)
+    limits = [(1, 1), (4, 8), (8, 16), (16, 24), (20, 30)]
+    n = 64 * len(limits)
+    rp, cols, vals = [0], [], []
+    for i in range(n):
+        seg = i // 64
+        s0 = seg * 64
+        nb, na = limits[seg]
+        before = np.arange(s0, i)[-nb:] if nb else np.array([], np.int64)
+        pool = np.arange(i + 1, n)
+        after = rng.choice(pool, size=min(na, pool.size), replace=False) if pool.size else np.array([], np.int64)
+        lead = rng.integers(0, s0, size=3) if s0 else np.array([], np.int64)
+        c = np.union1d(np.union1d(np.union1d(before, after), lead), [i]).astype(np.int64)
+        v = rng.uniform(-1.0, 1.0, size=c.size)
+        v[c == i] = np.abs(v).sum() + 1.0
+        cols.extend(c.tolist())
+        vals.extend(v.tolist())
+        rp.append(len(cols))
+    return np.array(rp, np.int64), np.array(cols, np.int64), np.array(vals)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_csr_segment_sor_chain_tiers(dtype):
+    """Every chain length of the 64-row segment sweep's register path (and
+    the LDS walk) against the oracle, bit for bit, both directions, regular
+    and ultra."""
+    rp, cols, vals = tiered_csr(29)
+    n = rp.size - 1
+    if dtype == np.float32:
+        vals = vals.astype(np.float32).astype(np.float64)
+    C = O.Csr(rp, cols, vals, symmetric=False, is_f32=dtype == np.float32)
+    G = la.CsrMatrix.from_arrays(rp, cols, vals, dtype, symmetric=False)
+    x = problems.uniform(n, 6, dtype)
+    b = problems.uniform(n, 7, dtype)
+    bounds = list(range(0, n, 64)) + [n]
+    for forward in (True, False):
+        for ultra in (False, True):
+            out = G.sor(la.DenseVector(x), la.DenseVector(n, dtype), la.DenseVector(b), 0.9, forward, ultra,
+                        segment_rows=64)
+            same(out.numpy(), O.sor(C, x, b, dtype(0.9), forward, ultra, bounds),
+                 f"forward={forward} ultra={ultra}: ")
+
+
 @pytest.mark.parametrize("long_rows", [(), (70, 130, 400), (3, 64, 100, 190, 250, 599)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_csr_scheduled_segment_sor(long_rows, dtype):
