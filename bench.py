@@ -241,6 +241,16 @@ def main():
     x_fp64 = x.numpy()
     log(f"fp64: {t_fp64 * 1e3:.1f} ms/solve, iterations {its64}")
     rel_err = float(np.abs(x_mixed - x_fp64).max() / np.abs(x_fp64).max())
+    # the solution both 1e-8 solves approximate: the FP64 solve run to 1e-13
+    # (untimed); each 1e-8 solve's error against it, max-norm relative
+    x.fill(0.0)
+    r_tight = la.ConjugateGradientsSolver(8, M64).solve(A, x, b, rel_tol=1e-13, max_it=1000)
+    x_tight = x.numpy()
+    scale = float(np.abs(x_tight).max())
+    err_mixed = float(np.abs(x_mixed - x_tight).max() / scale)
+    err_fp64 = float(np.abs(x_fp64 - x_tight).max() / scale)
+    log(f"error against the FP64 solution at 1e-13 ({r_tight.iterations} it): mixed {err_mixed:.2e}, "
+        f"fp64 {err_fp64:.2e}")
 
     # ---- kernel rooflines (HIP events on the library stream) ----
     rot = 3  # distinct buffer sets so the 256 MB infinity cache cannot serve repeats
@@ -307,6 +317,8 @@ def main():
             "iterations_mixed": its32[-1],
             "iterations_fp64": its64[-1],
             "mixed_vs_fp64_rel_error": rel_err,
+            "mixed_vs_converged_fp64_rel_error": err_mixed,
+            "fp64_vs_converged_fp64_rel_error": err_fp64,
             "setup_seconds_mixed": M32.setup_seconds,
             "setup_seconds_fp64": M64.setup_seconds,
             "setup_split_fp64": M64.setup_split,

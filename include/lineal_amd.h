@@ -241,9 +241,10 @@ typedef struct {
   int setup_on_device;
   /* coarse levels with at most this many rows are swept in one segment
    * (lexicographically, as the reference with one thread) whatever
-   * coarse_segment_rows says; 0 (default): none.  Fewer CG iterations
-   * (256^3 lognormal: 98 -> 94 with 1024), but on the GPU such a sweep runs
-   * one dependency level at a time and costs more than the iterations save */
+   * coarse_segment_rows says; default 512, 0: none.  Levels of at most 512
+   * rows run a round program (one dependency level of rows per round, in one
+   * workgroup); larger ones the scheduled sweep.  Fewer CG iterations
+   * (256^3 lognormal: 98 -> 94 with the default) */
   int64_t sequential_rows;
 } lgpu_amg_params;
 enum { LGPU_CYCLE_REGULAR = 0, LGPU_CYCLE_DECREASING = 1 };

@@ -92,7 +92,7 @@ class _AmgParams(C.Structure):
 CYCLE_REGULAR, CYCLE_DECREASING = 0, 1
 # lgpu_amg_params.sequential_rows by default: coarse levels of at most this
 # many rows are swept in one segment (0: none)
-SEQUENTIAL_ROWS = 0
+SEQUENTIAL_ROWS = 512
 
 
 class _AmgLevel(C.Structure):

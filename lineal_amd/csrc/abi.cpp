@@ -158,6 +158,116 @@ struct CsrDev {
     LGPU_HIP_CHECK(hipMemcpy(sched_bwd.ptr, bwd.data(), bwd.size() * sizeof(int2), hipMemcpyHostToDevice));
     sched_seg = seg;
   }
+  // one-segment round programs of both directions ([0] forward, [1]
+  // backward; nrounds = 0 when the level is not swept that way)
+  RoundSweep rounds[2];
+  DevBuf rounds_buf[2];
+  // Builds the round programs of a level of at most kRoundMaxRows rows swept
+  // in one segment (the dependency levels of sched_positions with the whole
+  // matrix as the segment).  Returns false, building nothing, when a program
+  // would exceed the LDS budget.
+  bool build_rounds(const setup::RowMatrix& m, size_t lds_budget) {
+    rounds[0] = rounds[1] = RoundSweep{};
+    if (m.n <= 0 || m.n > kRoundMaxRows) return false;
+    std::vector<int2> pos[2];
+    sched_positions(m.n, m.rowptr.data(), m.cols.data(), m.n, pos[0], pos[1]);
+    const int64_t es = f32 ? 4 : 8;
+    auto r16 = [](int64_t v) { return (v + 15) & ~int64_t(15); };
+    for (int dir = 0; dir < 2; ++dir) {
+      const std::vector<int2>& ps = pos[dir];
+      // rounds: one dependency level at a time, at most 64 rows each, the
+      // chain length the round's longest row rounded up to a multiple of 8,
+      // at least the sweep's register slots
+      std::vector<int4> rs;
+      int64_t entries = 0;
+      for (int64_t p = 0; p < n;) {
+        int64_t q = p;
+        int longest = 0;
+        while (q < n && ps[size_t(q)].y == ps[size_t(p)].y && q - p < 64) {
+          const int64_t i = ps[size_t(q)].x;
+          longest = std::max(longest, int(m.rowptr[size_t(i + 1)] - m.rowptr[size_t(i)]));
+          ++q;
+        }
+        const int chain = std::max(f32 ? kRoundSlots32 : kRoundSlots64, (longest + 7) / 8 * 8);
+        rs.push_back(int4{int(entries), int(p), int(q - p), chain});
+        entries += (q - p) * chain;
+        p = q;
+      }
+      if (round_lds_bytes(n, entries, int(rs.size()), f32) > lds_budget) {
+        rounds[0] = rounds[1] = RoundSweep{};
+        return false;
+      }
+      // the staging region the sweep copies whole (rows, diagonals, values,
+      // round headers), then the two column arrays (byte offsets)
+      const int64_t ep = entries + kRoundPad;
+      const int64_t o_rows = 0;
+      const int64_t o_diag = o_rows + r16(n * 4);
+      const int64_t o_vals = o_diag + r16(n * es);
+      const int64_t o_hdr = o_vals + ep * es;
+      const int64_t o_cp = o_hdr + int64_t(rs.size()) * 16;
+      const int64_t o_cu = o_cp + 4 * ep;
+      const int64_t total = o_cu + 4 * ep;
+      if (o_cp != round_stage_bytes(n, entries, int(rs.size()), es))
+        throw Error{LGPU_HIP_ERROR, "round program layout mismatch"};
+      std::vector<unsigned char> h(size_t(total), 0);
+      auto put = [&](int64_t off, double v) {
+        if (f32) {
+          const float f = float(v);
+          std::memcpy(h.data() + off, &f, 4);
+        } else {
+          std::memcpy(h.data() + off, &v, 8);
+        }
+      };
+      auto put32 = [&](int64_t off, int64_t col) {
+        const uint32_t b = uint32_t(col * es);
+        std::memcpy(h.data() + off, &b, 4);
+      };
+      std::memcpy(h.data() + o_hdr, rs.data(), rs.size() * sizeof(int4));
+      for (int64_t e = entries; e < ep; ++e) {  // the padding: value 0 at column n
+        put32(o_cp + e * 4, n);
+        put32(o_cu + e * 4, n);
+      }
+      for (const int4& r : rs) {
+        for (int j = 0; j < r.z; ++j) {
+          const int64_t p = r.y + j;
+          const int32_t i = ps[size_t(p)].x;
+          std::memcpy(h.data() + o_rows + p * 4, &i, 4);
+          double d = 1.0;  // no stored diagonal: the sweep divides by 1
+          const int64_t e0 = int64_t(r.x) + int64_t(j) * r.w;
+          const int64_t k0 = m.rowptr[size_t(i)], k1 = m.rowptr[size_t(i + 1)];
+          for (int64_t u = 0; u < r.w; ++u) {
+            const int64_t e = e0 + u;
+            if (k0 + u < k1) {
+              const int32_t c = m.cols[size_t(k0 + u)];
+              const double v = m.vals[size_t(k0 + u)];
+              if (c == i) d = v;
+              put(o_vals + e * es, v);
+              put32(o_cp + e * 4, c == i ? n : c);
+              put32(o_cu + e * 4, c);
+            } else {  // padding: value 0 at column n
+              put(o_vals + e * es, 0.0);
+              put32(o_cp + e * 4, n);
+              put32(o_cu + e * 4, n);
+            }
+          }
+          put(o_diag + p * es, d);
+        }
+      }
+      rounds_buf[dir].alloc(size_t(total));
+      LGPU_HIP_CHECK(hipMemcpy(rounds_buf[dir].ptr, h.data(), size_t(total), hipMemcpyHostToDevice));
+      const auto b0 = static_cast<unsigned char*>(rounds_buf[dir].ptr);
+      RoundSweep& g = rounds[dir];
+      g.nrounds = int(rs.size());
+      g.entries = entries;
+      g.rounds = reinterpret_cast<const int4*>(b0 + o_hdr);
+      g.rows = reinterpret_cast<const int32_t*>(b0 + o_rows);
+      g.diag = b0 + o_diag;
+      g.vals = b0 + o_vals;
+      g.cols_plain = reinterpret_cast<const uint32_t*>(b0 + o_cp);
+      g.cols_ultra = reinterpret_cast<const uint32_t*>(b0 + o_cu);
+    }
+    return true;
+  }
   void upload(const setup::RowMatrix& m, bool as_f32) {
     n = m.n;
     auto most = [&](int64_t rows) {
@@ -520,7 +630,9 @@ void dot_into(lgpu_ctx ctx, const void* a, const void* b, bool f32, int64_t n, d
 void coarse_sweep(lgpu_amg amg, CsrDev& a, int64_t seg, const void* x_in, void* x_out,
                   const void* b, double omega, bool fwd, bool ultra) {
   KTimed kt(amg->ctx, LGPU_KT_CSR_SOR);
-  if (a.sched_seg > 0)
+  if (seg == 0 && a.rounds[fwd ? 0 : 1].nrounds > 0)
+    csr_sor_rounds(a.desc(), a.rounds[fwd ? 0 : 1], x_in, x_out, b, omega, ultra, amg->ctx->stream);
+  else if (a.sched_seg > 0)
     csr_sor_scheduled(a.desc(), x_in, x_out, b, omega, fwd, ultra, amg->ctx->stream);
   else if (seg > 0)
     csr_sor_segmented(a.desc(), x_in, x_out, b, omega, fwd, ultra, int(seg), amg->ctx->stream);
@@ -1247,7 +1359,14 @@ extern "C" int lgpu_op_sor_segmented(lgpu_op op, lgpu_vec x_in, lgpu_vec x_out, 
     LGPU_REQUIRE(x_in->f32 == x_out->f32 && x_in->ptr != x_out->ptr, "x_in must match x_out and not alias it");
   }
   LGPU_REQUIRE(b->f32 == x_out->f32 && op->csr.f32 == x_out->f32, "vectors and values must share a dtype");
-  if (segment_rows <= kMaxSegmentRows) {
+  const int64_t n = op->rows();
+  // one segment over more rows than a wavefront: the round program, built on
+  // first use, when the level is small enough
+  if (segment_rows >= n && n > kMaxSegmentRows && n <= kRoundMaxRows &&
+      (op->csr.rounds[0].nrounds > 0 || op->csr.build_rounds(op->host_csr, tail_lds_budget()))) {
+    csr_sor_rounds(op->csr.desc(), op->csr.rounds[forward ? 0 : 1], x_in ? x_in->ptr : nullptr, x_out->ptr,
+                   b->ptr, omega, ultra != 0, op->ctx->stream);
+  } else if (segment_rows <= kMaxSegmentRows) {
     csr_sor_segmented(op->csr.desc(), x_in ? x_in->ptr : nullptr, x_out->ptr, b->ptr, omega,
                       forward != 0, ultra != 0, int(segment_rows), op->ctx->stream);
   } else {
@@ -1357,7 +1476,7 @@ extern "C" void lgpu_amg_default_params(lgpu_amg_params* p) {
   p->cycle_kind = LGPU_CYCLE_REGULAR;
   p->cycle_iterations = 1;
   p->setup_on_device = 1;
-  p->sequential_rows = 0;
+  p->sequential_rows = kRoundMaxRows;
 }
 
 namespace {
@@ -1391,17 +1510,27 @@ void build_tail(lgpu_amg amg) {
   if (L < 2 || amg->lv[size_t(L - 1)].n <= 0) return;
   const size_t budget = tail_lds_budget();
   int t0 = L - 1;
-  size_t stride = 0;
+  size_t stride = 0, block_need = 0;  // LDS per wavefront (64-row segments), per workgroup (rounds)
   while (t0 - 1 >= 1) {
     const AmgLevel& a = amg->lv[size_t(t0 - 1)];
-    if (a.n <= 0 || a.n > kTailMaxRows || a.seg != kMaxSegmentRows || a.a.sched_seg != 0) break;
-    const size_t need = tail_wave_lds_bytes(a.a.seg_entries, amg->f32);
-    if (need > budget) break;
-    stride = std::max(stride, need);
+    if (a.n <= 0 || a.n > kTailMaxRows) break;
+    if (a.seg == 0 && a.a.rounds[0].nrounds > 0) {
+      const size_t need = std::max(round_lds_bytes(a.n, a.a.rounds[0].entries, a.a.rounds[0].nrounds, amg->f32),
+                                   round_lds_bytes(a.n, a.a.rounds[1].entries, a.a.rounds[1].nrounds, amg->f32));
+      if (need + 16 * kTailMaxWaves > budget) break;  // room for the per-wavefront rounding below
+      block_need = std::max(block_need, need);
+    } else {
+      if (a.seg != kMaxSegmentRows || a.a.sched_seg != 0) break;
+      const size_t need = tail_wave_lds_bytes(a.a.seg_entries, amg->f32);
+      if (need > budget) break;
+      stride = std::max(stride, need);
+    }
     --t0;
   }
   if (t0 >= L - 1) return;  // no level besides the coarsest
-  const int waves = int(std::min<size_t>(kTailMaxWaves, budget / stride));
+  const int waves = stride > 0 ? int(std::min<size_t>(kTailMaxWaves, budget / stride)) : kTailMaxWaves;
+  // the round sweeps use the workgroup's whole LDS from its start
+  stride = std::max(stride, (block_need / size_t(waves) + 15) & ~size_t(15));
   std::vector<TailLevel> tl;
   for (int l = t0; l < L; ++l) {
     AmgLevel& a = amg->lv[size_t(l)];
@@ -1419,6 +1548,10 @@ void build_tail(lgpu_amg amg) {
     t.sol = a.sol.ptr;
     t.tmp = a.tmp.ptr;
     t.aux = a.aux.ptr;
+    if (l < L - 1 && a.seg == 0) {
+      t.rounds_fwd = a.a.rounds[0];
+      t.rounds_bwd = a.a.rounds[1];
+    }
     tl.push_back(t);
   }
   amg->tail.alloc(tl.size() * sizeof(TailLevel));
@@ -1450,6 +1583,8 @@ void upload_hierarchy(lgpu_amg amg) {
       // segments longer than a wavefront, run the scheduled sweep
       if (a.n > 0 && ((a.seg == 0 && a.n <= kSchedMaxRows) || a.seg > kMaxSegmentRows))
         a.a.build_schedule(m, a.seg == 0 ? a.n : a.seg);
+      // a small level in one segment: the round programs
+      if (a.seg == 0 && a.n > 0 && a.n <= kRoundMaxRows) a.a.build_rounds(m, tail_lds_budget());
       // the level vectors persist between cycles, as the reference's
       // hierarchy vectors do (hierarchy/wright.hpp:227-228): a DECREASING
       // cycle prolongs a coarse solution no visit of the same cycle wrote.

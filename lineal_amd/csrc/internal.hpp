@@ -188,6 +188,52 @@ void stencil_sor_weights(const StencilDesc& s, bool f32, void* w, hipStream_t st
 int64_t csr_partials(const CsrDesc& a);
 void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y, double omega,
                double* partials, hipStream_t st);
+// A small level swept in one segment as a program of rounds: the rows of one
+// dependency level of sched_positions (no row of a round reads another's
+// value, and every value a row reads holds what the sequential sweep would
+// read), at most 64 per round, one wavefront lane per row.  Each round's rows
+// carry their entries in column order padded to the round's chain length (a
+// multiple of 8, at least the register slots) with value 0 at column n, where
+// the sweep keeps a 0, so
+// every lane runs the same FMA chain and padding changes no bit.  Columns are
+// stored as byte offsets into the sweep's vector (column x element size);
+// the diagonal stays at its column in `cols_ultra` (ultra SOR applies it
+// there) and reads column n in `cols_plain`.  Host-built per direction at
+// upload, in one buffer: the staging region (rows, diagonals, values, round
+// headers), then the two column arrays, each section 16-byte aligned.
+struct RoundSweep {
+  int nrounds = 0;
+  int64_t entries = 0;
+  const int4* rounds = nullptr;         // {first entry, first position, rows, chain length}
+  const int32_t* rows = nullptr;        // row of each position (the staging region starts here)
+  const void* diag = nullptr;           // each position's diagonal (1 where none), T
+  const void* vals = nullptr;           // T[entries + kRoundPad]
+  const uint32_t* cols_plain = nullptr;  // byte offsets, entries + kRoundPad
+  const uint32_t* cols_ultra = nullptr;
+};
+// levels of at most this many rows may be swept by a round program
+constexpr int64_t kRoundMaxRows = 512;
+// padding entries (value 0 at column n) after the last row's, so that a lane
+// may read a full register block of entries from any row
+constexpr int64_t kRoundPad = 32;
+// entries a round sweep holds in registers per row (f32 / f64); every round's
+// chain is at least this long
+constexpr int kRoundSlots32 = 32, kRoundSlots64 = 16;
+// bytes of a round program's staging region: rows (i32), diagonals (T),
+// values (T; the entries and the padding), then the round headers; each
+// section 16-byte aligned (es = sizeof(T)).  The column array (u32) follows
+// it in LDS.
+__host__ __device__ constexpr int64_t round_stage_bytes(int64_t n, int64_t entries, int nrounds, int64_t es) {
+  return ((n * 4 + 15) & ~int64_t(15)) + ((n * es + 15) & ~int64_t(15)) + (entries + kRoundPad) * es +
+         int64_t(nrounds) * 16;
+}
+// LDS bytes of a round sweep (x, b, the staging region and one column array)
+size_t round_lds_bytes(int64_t n, int64_t entries, int nrounds, bool f32);
+// one-segment SOR of a level by the round program of the sweep's direction,
+// one workgroup
+void csr_sor_rounds(const CsrDesc& a, const RoundSweep& rs, const void* x_in, void* x_out,
+                    const void* b, double omega, bool ultra, hipStream_t st);
+
 // One level of the multigrid cycle's tail, the small levels that
 // amg_tail runs in one workgroup: the level operator (CSR, values of the
 // cycle's precision) and vectors, its 64-row segment sweeps' staging size and
@@ -207,6 +253,9 @@ struct TailLevel {
   void* sol;
   void* tmp;
   void* aux;
+  // one-segment levels: the round programs of both directions (nrounds > 0),
+  // else 64-row segments
+  RoundSweep rounds_fwd, rounds_bwd;
 };
 // register-list sizes of the 64-row segment sweeps' short variant (new-value
 // and trailing entries per row)

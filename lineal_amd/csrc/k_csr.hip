@@ -1197,6 +1197,175 @@ void dispatch_segment(const CsrDesc& a, const void* x_in, void* x_out, const voi
   }
 }
 
+// Round programs (RoundSweep): sections 16-byte aligned.
+__host__ __device__ constexpr int64_t round16(int64_t bytes) { return (bytes + 15) & ~int64_t(15); }
+
+// Block-wide copy of `bytes` (a multiple of 16) with 16-byte loads, eight
+// per thread in flight before their stores.
+__device__ __forceinline__ void block_copy16(void* dst, const void* src, int64_t bytes) {
+  using V4 = unsigned int __attribute__((ext_vector_type(4)));
+  const V4* s = static_cast<const V4*>(src);
+  V4* d = static_cast<V4*>(dst);
+  const int64_t nch = bytes / 16, nt = blockDim.x;
+  constexpr int B = 8;
+  for (int64_t k0 = threadIdx.x; k0 < nch; k0 += B * nt) {
+    V4 v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int64_t k = k0 + u * nt;
+      v[u] = s[k < nch ? k : nch - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int64_t k = k0 + u * nt;
+      if (k < nch) d[k] = v[u];
+    }
+  }
+}
+
+// One-segment SOR of a small level by its round program (RoundSweep), by
+// the whole workgroup: x (x_in or 0, and x[n] = 0), b and the program's
+// staging region are copied to LDS; wavefront 0 then runs the rounds, lane j
+// of a round computing the round's j-th row in place in x: b_i, then
+// fma(-a_ij, x_j, acc) over the row's entries in column order (the diagonal
+// at its column for ultra, else reading x[n] = 0 with its value), then the
+// regular or ultra update of the oracle's segment sweep (one segment, so no
+// off-segment entries).  The rows of a round read only values the rounds
+// before finished or that no round before changed, which is what the
+// sequential sweep reads.  Last, the workgroup writes x out.
+//
+// A round's critical path is its gathers of x, the FMA chain and the
+// division.  Everything else a round needs does not depend on x (header,
+// row, diagonal, right-hand side, the first kRegSlots columns and values),
+// so it is read one round ahead into the other of two register buffers,
+// after the current round's gathers have been issued.
+template <typename T>
+struct RoundRegs {
+  static constexpr int kSlots = sizeof(T) == 4 ? kRoundSlots32 : kRoundSlots64;  // entries held in registers
+  using U4 = unsigned int __attribute__((ext_vector_type(4)));
+  using TV = T __attribute__((ext_vector_type(16 / sizeof(T))));
+  static constexpr int kPer = 16 / int(sizeof(T));
+  int4 h;
+  int i, e;
+  bool act;
+  T d, b;
+  U4 c[kSlots / 4];
+  TV v[kSlots / kPer];
+  __device__ __forceinline__ unsigned col(int u) const { return c[u / 4][u % 4]; }
+  __device__ __forceinline__ T val(int u) const { return v[u / kPer][u % kPer]; }
+};
+
+template <typename T, bool ULTRA, bool HAS_X>
+__device__ void round_sweep(const RoundSweep& g, int64_t n64, const T* __restrict__ xin,
+                            T* __restrict__ xout, const T* __restrict__ b, T omega, double* lds) {
+  using R = RoundRegs<T>;
+  constexpr int RS = R::kSlots;
+  const int n = int(n64);
+  const int64_t E = g.entries + kRoundPad;
+  char* const base = reinterpret_cast<char*>(lds);
+  T* const x = reinterpret_cast<T*>(base);  // at byte 0: column byte offsets address it directly
+  T* const bs = reinterpret_cast<T*>(base + round16((n + 1) * int64_t(sizeof(T))));
+  char* const stage = base + round16((n + 1) * int64_t(sizeof(T))) + round16(n * int64_t(sizeof(T)));
+  const int32_t* const rows = reinterpret_cast<const int32_t*>(stage);
+  const T* const dg = reinterpret_cast<const T*>(stage + round16(n * 4));
+  const T* const vals = reinterpret_cast<const T*>(stage + round16(n * 4) + round16(n * int64_t(sizeof(T))));
+  const int4* const hdr = reinterpret_cast<const int4*>(vals + E);
+  const int64_t stage_bytes = round_stage_bytes(n, g.entries, g.nrounds, sizeof(T));
+  uint32_t* const cols = reinterpret_cast<uint32_t*>(stage + stage_bytes);
+  // the staging region follows the rows pointer in the program buffer
+  block_copy16(stage, g.rows, stage_bytes);
+  block_copy16(cols, ULTRA ? g.cols_ultra : g.cols_plain, 4 * E);
+  for (int k = threadIdx.x; k <= n; k += blockDim.x) {
+    x[k] = HAS_X && k < n ? xin[k] : T(0);
+    if (k < n) bs[k] = b[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < WSEG) {
+    const int lane = int(threadIdx.x);
+    const T m1w = T(1) - omega;
+    auto xat = [&](unsigned off) { return *reinterpret_cast<const T*>(base + off); };
+    // what round r needs besides x (its right-hand side follows in rhs())
+    auto fetch = [&](int r, R& q) {
+      q.h = hdr[r];
+      q.act = lane < q.h.z;
+      const int p = q.h.y + (q.act ? lane : 0);
+      q.e = q.h.x + (q.act ? lane : 0) * q.h.w;
+      q.i = rows[p];
+      q.d = dg[p];
+#pragma unroll
+      for (int k = 0; k < RS / 4; ++k) q.c[k] = *reinterpret_cast<const typename R::U4*>(cols + q.e + 4 * k);
+#pragma unroll
+      for (int k = 0; k < RS / R::kPer; ++k)
+        q.v[k] = *reinterpret_cast<const typename R::TV*>(vals + q.e + R::kPer * k);
+    };
+    auto rhs = [&](R& q) { q.b = bs[q.i]; };
+    // round r from cur; the next round's data into nxt
+    auto round = [&](R& cur, R& nxt, int r) {
+      const T xi = x[cur.i];
+      T acc = cur.b;
+      auto run = [&](auto CTc) {
+        constexpr int CT = decltype(CTc)::value;  // slots taken from registers
+        T xv[CT];
+#pragma unroll
+        for (int u = 0; u < CT; ++u) xv[u] = xat(cur.col(u));
+        if (r + 1 < g.nrounds) fetch(r + 1, nxt);
+#pragma unroll
+        for (int u = 0; u < CT; ++u) acc = fma(-cur.val(u), xv[u], acc);
+        // entries beyond the register slots, eight at a time from LDS
+        for (int k = CT; k < cur.h.w; k += 8) {
+          unsigned c[8];
+          T v[8], xw[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            c[u] = cols[cur.e + k + u];
+            v[u] = vals[cur.e + k + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) xw[u] = xat(c[u]);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc = fma(-v[u], xw[u], acc);
+        }
+      };
+      // every chain holds at least the register slots (the host pads rows to
+      // them), so one path serves every round
+      run(std::integral_constant<int, RS>{});
+      T xn;
+      if constexpr (ULTRA) {
+        const T ad = cur.d < T(0) ? -cur.d : cur.d;
+        xn = xi + (omega * acc) / (ad + T(0.5) * T(0));
+      } else {
+        const T quot = acc / cur.d;
+        xn = fma(omega, quot, m1w * xi);
+      }
+      if (cur.act) x[cur.i] = xn;
+      if (r + 1 < g.nrounds) rhs(nxt);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+  R A, B;
+    if (g.nrounds > 0) {
+      fetch(0, A);
+      rhs(A);
+    }
+    for (int r = 0; r < g.nrounds; r += 2) {
+      round(A, B, r);
+      if (r + 1 >= g.nrounds) break;
+      round(B, A, r + 1);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += blockDim.x) xout[k] = x[k];
+}
+
+template <typename T, bool ULTRA, bool HAS_X>
+__global__ __launch_bounds__(512) void csr_round_kernel(RoundSweep g, int64_t n, const T* __restrict__ xin,
+                                                        T* __restrict__ xout, const T* __restrict__ b,
+                                                        T omega) {
+  extern __shared__ double round_lds[];
+  round_sweep<T, ULTRA, HAS_X>(g, n, xin, xout, b, omega, round_lds);
+}
+
 // The cycle's tail in one workgroup (amg_tail): every phase of the small
 // levels' V-cycle in turn, with a barrier between phases, so the tail costs
 // one launch instead of five per level.  Each phase repeats the standalone
@@ -1206,7 +1375,12 @@ void dispatch_segment(const CsrDesc& a, const void* x_in, void* x_out, const voi
 // order, the dense coarsest solve and the prolongation.
 template <typename T, bool FWD, bool ULTRA, bool HAS_X>
 __device__ __forceinline__ void tail_sweep(const TailLevel& a, const T* xin, T* xout, const T* b,
-                                           double* my_lds) {
+                                           double* my_lds, double* block_lds) {
+  const RoundSweep& rs = FWD ? a.rounds_fwd : a.rounds_bwd;
+  if (rs.nrounds > 0) {
+    round_sweep<T, ULTRA, HAS_X>(rs, a.n, xin, xout, b, T(1), block_lds);
+    return;
+  }
   const int wid = int(threadIdx.x / WSEG), nw = int(blockDim.x / WSEG);
   const int64_t nseg = (a.n + WSEG - 1) / WSEG;
   const auto va = static_cast<const T*>(a.vals);
@@ -1231,7 +1405,7 @@ __global__ __launch_bounds__(512) void amg_tail_kernel(const TailLevel* __restri
     const TailLevel a = lv[t];
     T* const x = static_cast<T*>(a.sol);
     const T* const b = static_cast<const T*>(a.rhs);
-    tail_sweep<T, true, ULTRA, false>(a, nullptr, x, b, my_lds);
+    tail_sweep<T, true, ULTRA, false>(a, nullptr, x, b, my_lds, tail_lds);
     __syncthreads();
     // residual (sm_resid: the row's products summed with fma in column
     // order), the entries in batches whose loads are all in flight together
@@ -1293,7 +1467,7 @@ __global__ __launch_bounds__(512) void amg_tail_kernel(const TailLevel* __restri
       aux[i] = x[i] + refined * inv_relax;
     }
     __syncthreads();
-    tail_sweep<T, false, ULTRA, true>(a, aux, x, static_cast<const T*>(a.rhs), my_lds);
+    tail_sweep<T, false, ULTRA, true>(a, aux, x, static_cast<const T*>(a.rhs), my_lds, tail_lds);
     __syncthreads();
   }
 }
@@ -1389,6 +1563,36 @@ size_t tail_wave_lds_bytes(int64_t cap, bool f32) {
 }
 
 size_t tail_lds_budget() { return wave_lds_budget(); }
+
+size_t round_lds_bytes(int64_t n, int64_t entries, int nrounds, bool f32) {
+  const int64_t es = f32 ? 4 : 8;
+  return size_t(round16((n + 1) * es) + round16(n * es) + round_stage_bytes(n, entries, nrounds, es) +
+                4 * (entries + kRoundPad));
+}
+
+void csr_sor_rounds(const CsrDesc& a, const RoundSweep& rs, const void* x_in, void* x_out,
+                    const void* b, double omega, bool ultra, hipStream_t st) {
+  LGPU_REQUIRE(rs.nrounds > 0 && a.n <= kRoundMaxRows, "no round program for this level");
+  const size_t lds = round_lds_bytes(a.n, rs.entries, rs.nrounds, a.f32 != 0);
+  LGPU_REQUIRE(lds <= wave_lds_budget(), "round program exceeds the LDS budget");
+  const dim3 grid(1), block(512);
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    const auto xi = static_cast<const T*>(x_in);
+    const auto xo = static_cast<T*>(x_out);
+    const auto bb = static_cast<const T*>(b);
+    if (ultra) {
+      if (x_in) hipLaunchKernelGGL((csr_round_kernel<T, true, true>), grid, block, lds, st, rs, a.n, xi, xo, bb, T(omega));
+      else hipLaunchKernelGGL((csr_round_kernel<T, true, false>), grid, block, lds, st, rs, a.n, xi, xo, bb, T(omega));
+    } else {
+      if (x_in) hipLaunchKernelGGL((csr_round_kernel<T, false, true>), grid, block, lds, st, rs, a.n, xi, xo, bb, T(omega));
+      else hipLaunchKernelGGL((csr_round_kernel<T, false, false>), grid, block, lds, st, rs, a.n, xi, xo, bb, T(omega));
+    }
+  };
+  if (a.f32) go(float{});
+  else go(double{});
+  LGPU_HIP_CHECK(hipGetLastError());
+}
 
 void amg_tail(const TailLevel* levels, int count, int64_t coarsest_n, const double* L,
               const double* U, bool f32, bool ultra, double inv_relax, int waves, size_t wave_lds,

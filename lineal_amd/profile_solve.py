@@ -43,7 +43,8 @@ def run(args):
     A = la.StencilMatrix.values(la.BoxInfo(dims), problems.lognormal(dims, args.sigma, 1), ctx)
     b = A.rhs()
     prec = np.float32 if args.precision == "f32" else np.float64
-    M = la.MultiGridSolver(precision=prec, coarse_segment_rows=args.coarse_segment_rows).instantiate(A)
+    M = la.MultiGridSolver(precision=prec, coarse_segment_rows=args.coarse_segment_rows,
+                           sequential_rows=args.sequential_rows).instantiate(A)
     x = la.DenseVector(n ** 3, ctx=ctx)
 
     def solve():
@@ -199,6 +200,8 @@ def main(argv=None):
     r.add_argument("--sigma", type=float, default=1.0)
     r.add_argument("--precision", choices=["f32", "f64"], default="f32")
     r.add_argument("--coarse-segment-rows", type=int, default=64)
+    r.add_argument("--sequential-rows", type=int, default=None,
+                   help="levels of at most this many rows swept in one segment (default: the library's)")
     r.add_argument("--levels", required=True)
     s = sub.add_parser("summary")
     s.add_argument("trace")

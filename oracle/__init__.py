@@ -357,7 +357,7 @@ def strength_graph(A, theta=1.0 / 3.0):
 
 # coarse levels of at most this many rows are swept in one segment (the
 # library's default lgpu_amg_params.sequential_rows; 0: none)
-SEQUENTIAL_ROWS = 0
+SEQUENTIAL_ROWS = 512
 
 
 def default_params(coarse_f32=False):

@@ -250,6 +250,29 @@ def test_csr_segment_sor_chain_tiers(dtype):
                  f"forward={forward} ultra={ultra}: ")
 
 
+@pytest.mark.parametrize("n,long_rows", [(65, ()), (300, ()), (300, (3, 64, 100, 190, 250)), (512, (7, 300, 511))])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_csr_round_program_sor(n, long_rows, dtype):
+    """One-segment SOR of a small level by its round program (rows grouped by
+    dependency level, chains of 8 to 64 entries): bit for bit the oracle's
+    one-segment sweep, forward and backward, regular and ultra, with and
+    without x_in."""
+    rp, cols, vals = long_row_csr(n, 41 + len(long_rows), set(long_rows))
+    if dtype == np.float32:
+        vals = vals.astype(np.float32).astype(np.float64)
+    C = O.Csr(rp, cols, vals, symmetric=False, is_f32=dtype == np.float32)
+    G = la.CsrMatrix.from_arrays(rp, cols, vals, dtype, symmetric=False)
+    x = problems.uniform(n, 8, dtype)
+    b = problems.uniform(n, 9, dtype)
+    for forward in (True, False):
+        for ultra in (False, True):
+            for with_x in (True, False):
+                out = G.sor(la.DenseVector(x) if with_x else None, la.DenseVector(n, dtype),
+                            la.DenseVector(b), 1.2, forward, ultra, segment_rows=n)
+                same(out.numpy(), O.sor(C, x if with_x else None, b, dtype(1.2), forward, ultra, [0, n]),
+                     f"forward={forward} ultra={ultra} x_in={with_x}: ")
+
+
 @pytest.mark.parametrize("long_rows", [(), (70, 130, 400), (3, 64, 100, 190, 250, 599)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_csr_scheduled_segment_sor(long_rows, dtype):
