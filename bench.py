@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the finest-level f32 SOR sweeps at 256^3: 2 x
 # FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc runs (the factor 2 is what a plain
 # copy kernel of known size gives), profiles/stencil_kernels_256_hbm_counters.md
-TRAFFIC_256 = {"stencil_sor_fwd": 826_157_144, "stencil_sor_bwd": 973_550_112}
+TRAFFIC_256 = {"stencil_sor_fwd": 826_077_768, "stencil_sor_bwd": 1_000_488_656}
 
 
 def sor_cache_bytes(n: int, esize: int) -> int:
