@@ -172,6 +172,16 @@ struct CsrDev {
     std::vector<int2> pos[2];
     sched_positions(m.n, m.rowptr.data(), m.cols.data(), m.n, pos[0], pos[1]);
     const int64_t es = f32 ? 4 : 8;
+    // register slots per row: in f32 the smallest of 16, 24, 32 that holds
+    // nine rows in ten of the level (longer rows read the rest from LDS)
+    int slots = kRoundSlots64;
+    if (f32) {
+      std::vector<int64_t> len(static_cast<size_t>(n));
+      for (int64_t i = 0; i < n; ++i) len[size_t(i)] = m.rowptr[size_t(i + 1)] - m.rowptr[size_t(i)];
+      std::sort(len.begin(), len.end());
+      const int64_t p90 = len[size_t(std::min<int64_t>(n - 1, (9 * n) / 10))];
+      slots = p90 <= 16 ? 16 : (p90 <= 24 ? 24 : 32);
+    }
     auto r16 = [](int64_t v) { return (v + 15) & ~int64_t(15); };
     for (int dir = 0; dir < 2; ++dir) {
       const std::vector<int2>& ps = pos[dir];
@@ -188,7 +198,7 @@ struct CsrDev {
           longest = std::max(longest, int(m.rowptr[size_t(i + 1)] - m.rowptr[size_t(i)]));
           ++q;
         }
-        const int chain = std::max(f32 ? kRoundSlots32 : kRoundSlots64, (longest + 7) / 8 * 8);
+        const int chain = std::max(slots, (longest + 7) / 8 * 8);
         rs.push_back(int4{int(entries), int(p), int(q - p), chain});
         entries += (q - p) * chain;
         p = q;
@@ -258,6 +268,7 @@ struct CsrDev {
       const auto b0 = static_cast<unsigned char*>(rounds_buf[dir].ptr);
       RoundSweep& g = rounds[dir];
       g.nrounds = int(rs.size());
+      g.slots = slots;
       g.entries = entries;
       g.rounds = reinterpret_cast<const int4*>(b0 + o_hdr);
       g.rows = reinterpret_cast<const int32_t*>(b0 + o_rows);

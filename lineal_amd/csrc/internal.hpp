@@ -203,6 +203,7 @@ void csr_apply(const CsrDesc& a, int mode, const void* x, const void* b, void* y
 // headers), then the two column arrays, each section 16-byte aligned.
 struct RoundSweep {
   int nrounds = 0;
+  int slots = 0;  // entries per row held in registers (the shortest chain)
   int64_t entries = 0;
   const int4* rounds = nullptr;         // {first entry, first position, rows, chain length}
   const int32_t* rows = nullptr;        // row of each position (the staging region starts here)
@@ -216,9 +217,10 @@ constexpr int64_t kRoundMaxRows = 512;
 // padding entries (value 0 at column n) after the last row's, so that a lane
 // may read a full register block of entries from any row
 constexpr int64_t kRoundPad = 32;
-// entries a round sweep holds in registers per row (f32 / f64); every round's
-// chain is at least this long
-constexpr int kRoundSlots32 = 32, kRoundSlots64 = 16;
+// entries a round sweep holds in registers per row: 16, 24 or 32 in f32
+// (chosen per level, RoundSweep::slots), 16 in f64; every round's chain is
+// at least this long
+constexpr int kRoundSlots64 = 16;
 // bytes of a round program's staging region: rows (i32), diagonals (T),
 // values (T; the entries and the padding), then the round headers; each
 // section 16-byte aligned (es = sizeof(T)).  The column array (u32) follows

@@ -1239,9 +1239,9 @@ __device__ __forceinline__ void block_copy16(void* dst, const void* src, int64_t
 // row, diagonal, right-hand side, the first kRegSlots columns and values),
 // so it is read one round ahead into the other of two register buffers,
 // after the current round's gathers have been issued.
-template <typename T>
+template <typename T, int SLOTS>
 struct RoundRegs {
-  static constexpr int kSlots = sizeof(T) == 4 ? kRoundSlots32 : kRoundSlots64;  // entries held in registers
+  static constexpr int kSlots = SLOTS;  // entries held in registers
   using U4 = unsigned int __attribute__((ext_vector_type(4)));
   using TV = T __attribute__((ext_vector_type(16 / sizeof(T))));
   static constexpr int kPer = 16 / int(sizeof(T));
@@ -1255,10 +1255,10 @@ struct RoundRegs {
   __device__ __forceinline__ T val(int u) const { return v[u / kPer][u % kPer]; }
 };
 
-template <typename T, bool ULTRA, bool HAS_X>
-__device__ void round_sweep(const RoundSweep& g, int64_t n64, const T* __restrict__ xin,
-                            T* __restrict__ xout, const T* __restrict__ b, T omega, double* lds) {
-  using R = RoundRegs<T>;
+template <typename T, bool ULTRA, bool HAS_X, int SLOTS>
+__device__ void round_sweep_slots(const RoundSweep& g, int64_t n64, const T* __restrict__ xin,
+                                  T* __restrict__ xout, const T* __restrict__ b, T omega, double* lds) {
+  using R = RoundRegs<T, SLOTS>;
   constexpr int RS = R::kSlots;
   const int n = int(n64);
   const int64_t E = g.entries + kRoundPad;
@@ -1356,6 +1356,19 @@ __device__ void round_sweep(const RoundSweep& g, int64_t n64, const T* __restric
   }
   __syncthreads();
   for (int k = threadIdx.x; k < n; k += blockDim.x) xout[k] = x[k];
+}
+
+// the register-slot count the host chose for the level (f64: always 16)
+template <typename T, bool ULTRA, bool HAS_X>
+__device__ void round_sweep(const RoundSweep& g, int64_t n, const T* __restrict__ xin, T* __restrict__ xout,
+                            const T* __restrict__ b, T omega, double* lds) {
+  if constexpr (sizeof(T) == 8) {
+    round_sweep_slots<T, ULTRA, HAS_X, kRoundSlots64>(g, n, xin, xout, b, omega, lds);
+  } else {
+    if (g.slots <= 16) round_sweep_slots<T, ULTRA, HAS_X, 16>(g, n, xin, xout, b, omega, lds);
+    else if (g.slots <= 24) round_sweep_slots<T, ULTRA, HAS_X, 24>(g, n, xin, xout, b, omega, lds);
+    else round_sweep_slots<T, ULTRA, HAS_X, 32>(g, n, xin, xout, b, omega, lds);
+  }
 }
 
 template <typename T, bool ULTRA, bool HAS_X>

@@ -273,6 +273,43 @@ def test_csr_round_program_sor(n, long_rows, dtype):
                      f"forward={forward} ultra={ultra} x_in={with_x}: ")
 
 
+def banded_csr(n, per_row, width, seed):
+    """Random diagonally dominant CSR: about per_row entries per row within
+    `width` of the diagonal, sorted columns."""
+    rng = np.random.default_rng(seed)
+    rp, cols, vals = [0], [], []
+    for i in range(n):
+        c = np.union1d(np.clip(i + rng.integers(-width, width + 1, size=per_row), 0, n - 1), [i])
+        v = rng.uniform(-1.0, 1.0, size=c.size)
+        v[c == i] = np.abs(v).sum() + 1.0
+        cols.extend(c.tolist())
+        vals.extend(v.tolist())
+        rp.append(len(cols))
+    return np.array(rp, np.int64), np.array(cols, np.int64), np.array(vals)
+
+
+@pytest.mark.parametrize("per_row", [12, 22, 30, 40])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_csr_round_program_register_slots(per_row, dtype):
+    """Round programs whose levels hold 16, 24 or 32 entries per row in
+    registers (f32; f64 always 16), the rest of longer rows read from LDS:
+    bit for bit the oracle's one-segment sweep."""
+    n = 377
+    rp, cols, vals = banded_csr(n, per_row, 40, 50 + per_row)
+    if dtype == np.float32:
+        vals = vals.astype(np.float32).astype(np.float64)
+    C = O.Csr(rp, cols, vals, symmetric=False, is_f32=dtype == np.float32)
+    G = la.CsrMatrix.from_arrays(rp, cols, vals, dtype, symmetric=False)
+    x = problems.uniform(n, 10, dtype)
+    b = problems.uniform(n, 11, dtype)
+    for forward in (True, False):
+        for ultra in (False, True):
+            out = G.sor(la.DenseVector(x), la.DenseVector(n, dtype), la.DenseVector(b), 0.8, forward, ultra,
+                        segment_rows=n)
+            same(out.numpy(), O.sor(C, x, b, dtype(0.8), forward, ultra, [0, n]),
+                 f"forward={forward} ultra={ultra}: ")
+
+
 @pytest.mark.parametrize("long_rows", [(), (70, 130, 400), (3, 64, 100, 190, 250, 599)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_csr_scheduled_segment_sor(long_rows, dtype):
