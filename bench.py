@@ -46,8 +46,13 @@ def log(msg):
 
 
 def kernel_ms(ctx, fn, launches):
-    """Average duration of one launch, HIP events on the library's stream."""
-    fn()
+    """Average duration of one launch in steady state, HIP events on the
+    library's stream.  An untimed pass of the same launches comes first: the
+    first launches after other work run up to 20% slower (measured on the
+    256^3 SpMV: 0.114 ms in the first 60 launches, 0.094 ms from the third
+    pass of 60 on)."""
+    for k in range(launches):
+        fn(k)
     ctx.synchronize()
     ctx.timer_start()
     for k in range(launches):
@@ -167,6 +172,32 @@ def main():
     c = problems.lognormal(dims, args.sigma, 1)
     A = la.StencilMatrix.values(la.BoxInfo(dims), c, ctx)
     b = A.rhs()
+    # ---- single-kernel rooflines (HIP events on the library stream) ----
+    rot = 3  # distinct buffer sets so the 256 MB infinity cache cannot serve repeats
+    xs = [la.DenseVector(problems.uniform(N, 10 + k), ctx=ctx) for k in range(rot)]
+    ys = [la.DenseVector(N, ctx=ctx) for _ in range(rot)]
+    launches = 120
+    t_spmv = kernel_ms(ctx, lambda k=0: A.spmv(xs[k % rot], ys[k % rot]), launches)
+    spmv_bytes = 24 * N  # coefficient + x read, y written (f64)
+    xs32 = [la.DenseVector(problems.uniform(N, 20 + k, np.float32), ctx=ctx) for k in range(rot)]
+    ys32 = [la.DenseVector(N, np.float32, ctx=ctx) for _ in range(rot)]
+    t_spmv32 = kernel_ms(ctx, lambda k=0: A.spmv(xs32[k % rot], ys32[k % rot]), launches)
+    bs = [la.DenseVector(problems.uniform(N, 30 + k), ctx=ctx) for k in range(rot)]
+    t_sor = kernel_ms(ctx, lambda k=0: A.sor(xs[k % rot], ys[k % rot], bs[k % rot], 1.0, True, False), 10)
+    sor_bytes = 32 * N  # coefficient, x_in, b read, x_out written
+    bs32 = [la.DenseVector(problems.uniform(N, 40 + k, np.float32), ctx=ctx) for k in range(rot)]
+    t_sor32 = kernel_ms(ctx, lambda k=0: A.sor(None, ys32[k % rot], bs32[k % rot], 1.0, True, False), 10)
+    sor32_bytes = 12 * N  # coefficient, b read, x_out written (pre-smoother from zero)
+    kern = {
+        "spmv_values_f64": {"ms": t_spmv, "GB/s": spmv_bytes / t_spmv / 1e6},
+        "spmv_values_f32": {"ms": t_spmv32, "GB/s": 12 * N / t_spmv32 / 1e6},
+        "sor_fwd_values_f64": {"ms": t_sor, "GB/s": sor_bytes / t_sor / 1e6},
+        "sor_fwd_zero_values_f32": {"ms": t_sor32, "GB/s": sor32_bytes / t_sor32 / 1e6},
+    }
+    for k, v in kern.items():
+        log(f"{k}: {v['ms']:.3f} ms, {v['GB/s']:.0f} GB/s")
+    del xs, ys, xs32, ys32, bs, bs32
+
     log("AMG setup (FP32 hierarchy)")
     M32 = la.MultiGridSolver(precision=np.float32).instantiate(A)
     log(f"  {M32.setup_seconds:.2f} s, levels {[M32.level_size(l) for l in range(M32.level_num)]}")
@@ -251,31 +282,6 @@ def main():
     err_fp64 = float(np.abs(x_fp64 - x_tight).max() / scale)
     log(f"error against the FP64 solution at 1e-13 ({r_tight.iterations} it): mixed {err_mixed:.2e}, "
         f"fp64 {err_fp64:.2e}")
-
-    # ---- kernel rooflines (HIP events on the library stream) ----
-    rot = 3  # distinct buffer sets so the 256 MB infinity cache cannot serve repeats
-    xs = [la.DenseVector(problems.uniform(N, 10 + k), ctx=ctx) for k in range(rot)]
-    ys = [la.DenseVector(N, ctx=ctx) for _ in range(rot)]
-    launches = 60
-    t_spmv = kernel_ms(ctx, lambda k=0: A.spmv(xs[k % rot], ys[k % rot]), launches)
-    spmv_bytes = 24 * N  # coefficient + x read, y written (f64)
-    xs32 = [la.DenseVector(problems.uniform(N, 20 + k, np.float32), ctx=ctx) for k in range(rot)]
-    ys32 = [la.DenseVector(N, np.float32, ctx=ctx) for _ in range(rot)]
-    t_spmv32 = kernel_ms(ctx, lambda k=0: A.spmv(xs32[k % rot], ys32[k % rot]), launches)
-    bs = [la.DenseVector(problems.uniform(N, 30 + k), ctx=ctx) for k in range(rot)]
-    t_sor = kernel_ms(ctx, lambda k=0: A.sor(xs[k % rot], ys[k % rot], bs[k % rot], 1.0, True, False), 10)
-    sor_bytes = 32 * N  # coefficient, x_in, b read, x_out written
-    bs32 = [la.DenseVector(problems.uniform(N, 40 + k, np.float32), ctx=ctx) for k in range(rot)]
-    t_sor32 = kernel_ms(ctx, lambda k=0: A.sor(None, ys32[k % rot], bs32[k % rot], 1.0, True, False), 10)
-    sor32_bytes = 12 * N  # coefficient, b read, x_out written (pre-smoother from zero)
-    kern = {
-        "spmv_values_f64": {"ms": t_spmv, "GB/s": spmv_bytes / t_spmv / 1e6},
-        "spmv_values_f32": {"ms": t_spmv32, "GB/s": 12 * N / t_spmv32 / 1e6},
-        "sor_fwd_values_f64": {"ms": t_sor, "GB/s": sor_bytes / t_sor / 1e6},
-        "sor_fwd_zero_values_f32": {"ms": t_sor32, "GB/s": sor32_bytes / t_sor32 / 1e6},
-    }
-    for k, v in kern.items():
-        log(f"{k}: {v['ms']:.3f} ms, {v['GB/s']:.0f} GB/s")
 
     # The dominant kernel of the mixed solve: the finest-level SOR sweep in
     # f32 (a forward sweep from zero as pre-smoother, a backward sweep from the
